@@ -1,0 +1,233 @@
+/*
+ * xbg.h - C-ABI of the MI355X surf-beat step library (libxbg.so).
+ *
+ * The library advances one XBeach_GPU domain through the per-step call
+ * sequence of the reference main loop (Wave_gpu.cu:702-728):
+ *     flow boundary (ubnd1D) -> wave action balance -> shallow-water flow
+ *     -> suspended sediment + bed update
+ * with hand-written HIP kernels for gfx950.  Everything crossing this
+ * boundary is plain C: POD structs, host pointers, sizes and status codes.
+ *
+ * Layout at the boundary is the reference's own (Wave_kernel.cu:264,
+ * Flow_kernel.cu:196-197):
+ *   2D fields    i = ix + iy*nx              (ix cross-shore, offshore at ix=0)
+ *   directional  i + itheta*nx*ny            (ee, rr)
+ *   St           iy + itheta*ny
+ *   qbnd         iy + k*ny, k = 0..3
+ * All fields are FP32 (DECNUM = float, XBeachGPU.h:37).  Scalars (dt, time)
+ * pass by value as double, as the reference host code holds them; they are
+ * rounded to float where the reference passes them to a kernel.
+ *
+ * Every call returns 0 on success or a negative XBG_E* code; the message of
+ * the last failure is available from xbg_last_error().  This replaces the
+ * reference's CUDA_CHECK log-and-exit (Wavestep.cu:4-18) and its
+ * "Model crashed" exit (Wave_gpu.cu:642-647): the caller decides to exit.
+ *
+ * One context is used by one host thread.  Each context owns one HIP stream
+ * and all of its device memory, allocated once in xbg_create; no call after
+ * xbg_create allocates device memory, except xbg_set_bnd_series (which sizes
+ * the resident boundary record buffers) and xbg_graph_step (which builds its
+ * HIP graph once per dt/time signature change, not per step).
+ */
+#ifndef XBG_H
+#define XBG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XBG_OK 0
+#define XBG_E_ARG (-1)      /* bad argument (size mismatch, unknown field, NULL) */
+#define XBG_E_HIP (-2)      /* a HIP runtime call failed */
+#define XBG_E_CRASH (-3)    /* non-finite or zero time step: the model crashed */
+#define XBG_E_STATE (-4)    /* call out of order (e.g. forcing not set) */
+
+/*
+ * Numeric mirror of XBGPUParam (XBeachGPU.h:62-164).  The reference class
+ * holds std::string / std::vector members and cannot cross a C boundary;
+ * this struct carries every number the step uses.  Defaults and the derived
+ * values of checkparamsanity (read_input.cpp:1227-1444) come from
+ * xbg_default_params() followed by xbg_finalize_params().
+ */
+typedef struct xbg_params {
+    int nx, ny;          /* grid size (Wave_gpu.cu:1743-1777) */
+    int ntheta;          /* number of direction bins, Wavestep.cu:33-44 */
+    double dx;           /* grid spacing (m), dy = dx */
+    double grdalpha;     /* grid rotation (rad), used by the boundary generator */
+
+    /* flow (XBeachGPU.h:74-87) */
+    double g, rho, eps, cf, cfsand, cfreef;
+    double nuh, nuhfac;
+    int usesmago;
+    double smag, lat, fc, Cd, wci, hwci;
+
+    /* waves (XBeachGPU.h:90-103) */
+    int breakmodel;
+    double gammaa, n, alpha, gammax, beta, fw, fwsand, fwreef;
+    int roller;
+    double thetamin, thetamax, dtheta;   /* radians */
+
+    /* sediment (XBeachGPU.h:105-115) */
+    double D50, D90, rhosed, wws, drydzmax, wetdzmax, maxslpchg, por, morfac;
+    double sus, bed, facsk, facas;
+
+    /* run control (XBeachGPU.h:66, 128-131) */
+    double CFL, sedstart, endtime, outputtimestep;
+    int swave, flow, sedtrans, morphology;
+
+    /* wave boundary generation (XBeachGPU.h:143-150) */
+    double dtbc, rtlength, sprdthr, nmax, fcutoff;
+    int random, nspr;
+
+    /*
+     * Bed-level accumulation (no counterpart in the reference, which adds
+     * dzg to FP32 zb, Sediment_kernel.cu:957):
+     *   0 = FP32 as the reference, 1 = Kahan-compensated FP32,
+     *   2 = FP64 accumulator (zb is its FP32 rounding).
+     */
+    int zb_accum;
+} xbg_params;
+
+/* Reference defaults (XBeachGPU.h:62-164).  Grid size is left 0. */
+void xbg_default_params(xbg_params* p);
+
+/*
+ * Derived values of checkparamsanity (read_input.cpp:1321-1365) and of the
+ * main init (Wave_gpu.cu:2058-2064, Wavestep.cu:33-44): cfsand/cfreef/
+ * fwsand/fwreef fall back to cf/fw, nuh := smag when usesmago==1,
+ * fc from lat, ntheta from the theta range.  Idempotent.
+ */
+void xbg_finalize_params(xbg_params* p);
+
+/*
+ * Direction bin centres and their cosines/sines exactly as waveinitGPU
+ * computes them (Wavestep.cu:79-86): theta[i] = i*dtheta + thetamin +
+ * 0.5f*dtheta evaluated in double and stored as float, cxsth = cos(theta),
+ * sxnth = sin(theta) in float.  Arrays have p->ntheta entries.
+ */
+void xbg_theta_tables(const xbg_params* p, float* theta, float* cxsth, float* sxnth);
+
+/*
+ * Time-step rule of mainloopGPU (Wave_gpu.cu:621-698) on the host:
+ * dt = 0.5*min(flow CFL), then min with 0.5*min(wave CFL) when waves are on
+ * and totaltime > 0, a +-10 % rate limit against the previous dt, and
+ * alignment on the next output time or endtime.  Returns XBG_E_CRASH when
+ * the flow CFL minimum gives a zero, infinite or NaN step.
+ */
+int xbg_next_dt(double dtflow_min, double dtwave_min, int with_wave, double prev_dt,
+                double totaltime, double nextoutputtime, double endtime, double* dt_out);
+
+/* Linear time interpolation of tools.cpp:22-25 (interptime). */
+double xbg_interptime(double next, double prev, double timenext, double time);
+
+typedef struct xbg_ctx xbg_ctx;
+
+/*
+ * Create a domain.  Allocates every device buffer once (the reference
+ * allocates ~60 buffers at init and ~25 scratch buffers per step,
+ * Wave_gpu.cu:2187-2280, Wavestep.cu:801-950).  Initial state is zero
+ * except sigm; load it with xbg_set_field ("zb", "zs", "hh", "uu", "vv",
+ * "stdep", "ee", "rr", ...), then call xbg_init_state.
+ */
+int xbg_create(const xbg_params* p, xbg_ctx** out);
+int xbg_destroy(xbg_ctx* ctx);
+const char* xbg_last_error(const xbg_ctx* ctx);   /* ctx may be NULL: create errors */
+
+/*
+ * Host <-> device copy of a named field (names as the reference output map,
+ * Wave_gpu.cu:319-607, plus the carried state).  n must equal the field's
+ * element count (nx*ny, or nx*ny*ntheta for "ee", "rr", "ctheta", or ny for
+ * "umeanbnd").  Reading an intra-step field ("Fx", "Sxx", ...) returns its
+ * value from the last step that produced it.
+ */
+int xbg_set_field(xbg_ctx* ctx, const char* name, const float* host, size_t n);
+int xbg_get_field(xbg_ctx* ctx, const char* name, float* host, size_t n);
+/* Number of elements of a named field, 0 when unknown. */
+size_t xbg_field_size(const xbg_ctx* ctx, const char* name);
+/* Names of all fields, NUL-separated, terminated by an empty name. */
+const char* xbg_field_names(void);
+
+/*
+ * Finish initialisation after the fields are loaded (Wave_gpu.cu:2629-2691):
+ * friction maps from stdep (updatezom), hu/hv from the initial state
+ * (depthhu/depthhv), sigm from Trep (set_bnd).  Trep is the representative
+ * period of the first boundary record.
+ */
+int xbg_init_state(xbg_ctx* ctx, double Trep);
+
+/*
+ * Per-step forcing, the device-side part of wavebnd/flowbnd
+ * (Wavestep.cu:722-750, Wave_gpu.cu:1030-1101).
+ *   St       ny*ntheta wave-energy boundary, already time-interpolated
+ *   qbndold/qbndnew  4*ny bound long-wave flux records
+ * Host buffers; each call is one small H2D copy on the context stream.
+ */
+int xbg_set_wave_forcing(xbg_ctx* ctx, const float* St, double Trep);
+int xbg_set_flow_forcing(xbg_ctx* ctx, const float* qbndold, const float* qbndnew,
+                         double timesincelast, double timenext,
+                         double zsbndi, double zsbndn, double windu, double windv);
+
+/*
+ * Resident boundary records (no counterpart in the reference, which copies
+ * St and qbnd host->device every step, Wavestep.cu:749-779): upload a whole
+ * series of nrec records once (Stfile: nrec*ny*ntheta, qfile: nrec*4*ny,
+ * record-major as the reference Stfile/qfile, Wavestep.cu:722-737), then
+ * select and time-interpolate on the device per step.
+ */
+int xbg_set_bnd_series(xbg_ctx* ctx, const float* Stfile, const float* qfile, int nrec);
+int xbg_use_bnd_record(xbg_ctx* ctx, int rec, double timesincelast, double timenext,
+                       double Trep, double zsbndi, double zsbndn, double windu, double windv);
+
+/* Sub-steps (XBeachGPU.h:349-353).  dt and totaltime as the main loop holds them. */
+int xbg_flow_bnd(xbg_ctx* ctx, double dt, double totaltime);   /* ubnd1D, Flow_kernel.cu:167 */
+int xbg_wave_step(xbg_ctx* ctx, double dt, double totaltime);  /* wavestep, Wavestep.cu:755 */
+int xbg_flow_step(xbg_ctx* ctx, double dt);                    /* flowstep, Wave_gpu.cu:1106 */
+int xbg_sed_step(xbg_ctx* ctx, double dt);                     /* sedimentstep, Wave_gpu.cu:1405 */
+
+/*
+ * One whole step in the order of Wave_gpu.cu:702-728: flow boundary, wave,
+ * flow, and sediment when sedtrans==1 and totaltime >= sedstart.
+ * totaltime is the time at the end of the step (the reference increments
+ * it before the sub-steps, Wave_gpu.cu:698).
+ */
+int xbg_step(xbg_ctx* ctx, double dt, double totaltime);
+
+/*
+ * CFL limits for the next step (FLOWDT / WAVEDT + min reductions,
+ * Wave_kernel.cu:161-249): dt_flow = min over cells of CFL*dx/sqrt(9.81 hh),
+ * dt_wave = min over cells and bins of CFL*dtheta/|ctheta| of the last wave
+ * step.  Both are the true global minima (not halved); reduced on the
+ * device, 2 doubles cross PCIe.  The "initial" flag uses 0.5*CFL as the
+ * reference's first step does (Wave_gpu.cu:2639).
+ */
+int xbg_cfl_dt(xbg_ctx* ctx, int initial, double* dt_flow, double* dt_wave);
+
+/* Running means and max of the output fields (Wave_gpu.cu:731-757, a61). */
+int xbg_accumulate_means(xbg_ctx* ctx);
+int xbg_reset_means(xbg_ctx* ctx);
+int xbg_divide_means(xbg_ctx* ctx, double nstep);
+
+/*
+ * Water volume sum(hh)*dx*dx and bed volume sum(zb)*dx*dx in FP64
+ * (compensated device reduction; no counterpart in the reference).
+ */
+int xbg_mass_balance(xbg_ctx* ctx, double* water_volume, double* bed_volume);
+
+int xbg_sync(xbg_ctx* ctx);
+void* xbg_stream(xbg_ctx* ctx);   /* the context's hipStream_t */
+
+/*
+ * Kernel timing with HIP events on the context stream.  When enabled each
+ * kernel launch is bracketed by events; xbg_profile_report writes one line
+ * per kernel "name calls total_ms avg_us" into buf and resets the counters.
+ */
+int xbg_profile(xbg_ctx* ctx, int enable);
+int xbg_profile_report(xbg_ctx* ctx, char* buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XBG_H */

@@ -1,0 +1,99 @@
+// Device-side building blocks of the surf-beat step: index clamps, the
+// reference's min/max semantics, and the per-cell physics shared by the
+// fused kernels in xbg_kernels.hip.  Every formula keeps the operation order
+// and the FP32/FP64 promotion points of the reference routine it cites, so
+// that the fused kernels reproduce the sequential reference step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define XDEV __device__ __forceinline__
+
+namespace xbg {
+
+constexpr double PI_ = 3.14159265;  // XBeachGPU.h:4 (double literal)
+constexpr int MAXT = 64;            // largest direction count of the generic kernel
+
+// wave_CPU.cpp:10-16 (ternary min/max)
+XDEV float fmx(float a, float b) { return (a < b) ? b : a; }
+XDEV float fmn(float a, float b) { return (b < a) ? b : a; }
+XDEV double dmx(double a, double b) { return (a < b) ? b : a; }
+
+// Flow_CPU.cpp:16-70 / Wave_kernel.cu:23-77 index clamps
+XDEV int m1(int i) { return i <= 0 ? 0 : i - 1; }
+XDEV int p1(int i, int n) { return i >= n - 1 ? n - 1 : i + 1; }
+XDEV int m2(int i) { return i <= 1 ? 0 : i - 2; }
+XDEV int p2(int i, int n) { return i >= n - 2 ? n - 1 : i + 2; }
+
+// Kernel scalars, rounded to float where the reference passes DECNUM.
+struct KP {
+    int nx, ny, nt;
+    uint32_t N;
+    float dx, g, rho, eps, nuh, nuhfac, fc, Cd, wci, hwci;
+    float gammaa, nexp, alpha, gammax, beta, dtheta;
+    float D50, D90, rhosed, wws, drydzmax, wetdzmax, maxslpchg, por, morfac, sus, bed;
+    float facsk, facas, cfsand, cfreef, fwsand, fwreef, CFL;
+    float twopi, aphi, bphi;
+    int usesmago, breakmodel, roller, zb_accum, swave;
+    float theta[MAXT], cxsth[MAXT], sxnth[MAXT];
+};
+
+// Per-step scalars.
+struct SP {
+    float dt, totaltime, Trep, sigm, taper;   // taper of offshorebndWavCPU
+    float tsl, tnext, zsbndi, zsbndn, windu, windv;
+    int signwu, signwv;
+};
+
+// Friction maps derived from the sediment thickness, updatezom
+// (Sediment_kernel.cu:1162-1189 == Sediment_CPU.cpp:13-40).
+XDEV float cfm_of(const KP& p, float stdep) { return stdep < 0.05f ? p.cfreef : p.cfsand; }
+XDEV float fwm_of(const KP& p, float stdep) { return stdep < 0.05f ? p.fwreef : p.fwsand; }
+
+// wave_CPU.cpp:142-191 dispersionCPU: golden-ratio relaxation with a double
+// tanh (2*pi is a double literal), explicit fallback after 199 iterations.
+XDEV void dispersion_cell(const KP& p, float sig, float h, float& k, float& c, float& cg) {
+    float L0 = p.twopi * p.g / (sig * sig);
+    float L1 = L0, L2, err;
+    for (int n = 1; n < 200; n++) {
+        L2 = (float)((double)L0 * tanh(2 * PI_ * (double)h / (double)L1));
+        err = fabsf(L2 - L1);
+        L1 = (L1 * p.aphi + L2 * p.bphi);
+        if (err <= 0.0001f) break;
+        if (n == 199) {
+            L1 = L0 * powf(tanhf(powf(sig * sig * h / p.g, 3.0f / 4.0f)), 2.0f / 3.0f);
+            break;
+        }
+    }
+    float kk = (float)(2 * PI_ / (double)L1);
+    k = kk;
+    float cc = sig / kk;
+    c = cc;
+    float kkhh = fmn(kk * h, 10.0f);
+    float s2kh = sinhf(2.0f * kkhh);
+    cg = cc * (0.5f + kkhh / s2kh);
+}
+
+// 2nd-order upwind value with first-order fallback; the x/y advection of
+// wave_CPU.cpp evaluates 1.5*e-0.5*e in double (lines 426-455, 524-551),
+// the theta advection in float (lines 745-786).
+XDEV float upw2d(float e1, float e2) {
+    float a = (float)(1.5 * (double)e1 - 0.5 * (double)e2);
+    return a < 0.0f ? e1 : a;
+}
+XDEV float upw2f(float e1, float e2) {
+    float a = 1.5f * e1 - 0.5f * e2;
+    return a < 0.0f ? e1 : a;
+}
+
+// Flow_kernel.cu:23-43
+XDEV float minmod(float d1, float d2) {
+    float m = 0.0f;
+    if (d1 * d2 > 0.0f) {
+        if (d1 > 0.0f) m = fmn(d1, d2);
+        if (d1 < 0.0f) m = fmx(d1, d2);
+    }
+    return m;
+}
+
+}  // namespace xbg

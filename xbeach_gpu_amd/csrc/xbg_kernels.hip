@@ -1,0 +1,1909 @@
+// HIP kernels of the surf-beat step for gfx950 (MI355X).
+//
+// One step of the reference (Wave_gpu.cu:702-728) is ~92 launches, each a
+// 16x16-block stencil followed by a device synchronisation, plus ~25
+// cudaMalloc/cudaFree pairs and two full-field D2H copies for the time step
+// (SURVEY appendix B).  Here the same step is 17 stream-ordered launches over
+// buffers allocated once:
+//
+//   k_bnd_offshore  ubnd1D                                  (1 workgroup)
+//   k_dispersion    dispersion -> k, c, cg
+//   k_wave<NT>      offshorebndWav .. breakerdelay fused per cell: the
+//                   nTheta bins of ee and rr live in registers; one read and
+//                   one write of each directional field per step
+//   k_wave_force    wavforce + twodimbnd(noix) for Fx, Fy, ust
+//   k_flow_nuh      smago + wetu/wetv masks
+//   k_flow_u        u momentum (wlevslopes .. eulerustep, latbnd)
+//   k_flow_v        v momentum (vdvdy_adv .. eulervstep, vvlatbnd)
+//   k_flow_wrk      wrkuu2Ocorr + wrkvv2Ocorr
+//   k_flow_corr     uu2Ocorr + vv2Ocorr + depthhu/depthhv
+//   k_flow_cont     CalcQFlow + continuity
+//   k_flow_wrkzs    wrkzs2Ocorr
+//   k_flow_zs       zs2Ocorr + latbnd(zs) + hsbnd
+//   k_flow_uvvu     calcuvvu + uvlatbnd
+//   k_sed_flux      longturb + Rvr + Susp
+//   k_sed_conc      Sbvr + Erosus + Conc + CClatbnd + bedupdate + zblatbnd
+//   k_sed_aval      avalanching (gather form, no scatter race)
+//   k_sed_update    updatezb + zblatbnd
+//
+// In-place boundary copies of the reference (rollerlatbnd, latbnd,
+// vvlatbnd, uvlatbnd, CClatbnd, zblatbnd, twodimbnd) are folded into the
+// producing kernel: the thread of a boundary cell evaluates the producer at
+// the source cell of the copy and stores that value, so no kernel reads a
+// cell another thread of the same launch writes.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/xbg.h"
+#include "xbg_device.h"
+
+using namespace xbg;
+
+namespace {
+
+constexpr int BX = 64;  // block: 64 cells along x (one wavefront per row)
+constexpr int BY = 4;   // 4 rows -> 256 threads
+
+struct Fld {
+    const float* St;   // ny*nt wave boundary of this step
+    const float* qbo;  // 4*ny
+    const float* qbn;
+    const float* ee_in;
+    float* ee_out;
+    const float* rr_in;
+    float* rr_out;
+    // carried 2D state
+    float *zs, *uu, *vv, *hh, *zb, *stdep, *C, *hu, *hv, *ueu, *vev, *vmageu, *vmagev;
+    float *vu, *uv, *qx, *qy, *uuold, *vvold, *zsold, *umean, *dzsdt, *rolthick, *dzb;
+    double* zb64;
+    float* zbc;
+    // wave products of the step
+    float *k, *c, *cg, *H, *E, *Dw, *DR, *R, *urms, *ust_raw, *ust, *thetamean;
+    float *Sxx, *Sxy, *Syy, *Fx, *Fy;
+    // flow intermediates
+    float* nuh;
+    uint8_t* wet;  // bit0 wetu, bit1 wetv (udepthmomcont / vdepthmomcont)
+    float *uu_b, *vv_b, *w1u, *w2u, *w1v, *w2v, *zsn, *w1z, *w2z;
+    // sediment intermediates
+    float *Sus, *Svs, *C_out, *hh_b, *zb_b, *stdep_b, *dzbx, *dzby, *ceqsg;
+    double* zb64_b;
+    float* zbc_b;
+    // per-block partial minima of the CFL reductions
+    float *red_wave, *red_flow;
+    int* red_bad;
+    float* red_out;  // [0] flow min, [1] wave min, [2] flag
+    double* red_sum;
+};
+
+XDEV int gx() { return blockIdx.x * BX + threadIdx.x; }
+XDEV int gy() { return blockIdx.y * BY + threadIdx.y; }
+
+// ---------------------------------------------------------------------------
+// Flow boundary: ubnd1D (Flow_kernel.cu:167-242) / ubnd1Dnowaves (332-403).
+// The alongshore mean of uu[0,:] is taken over the pre-update column and
+// summed in row order by one lane, as a sequential sweep would.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bnd_offshore(KP p, SP s, Fld f, int waves) {
+    __shared__ float col[1024];
+    __shared__ float avg;
+    const int nx = p.nx, ny = p.ny;
+    float acc = 0.0f;
+    for (int base = 0; base < ny; base += 1024) {
+        int cnt = min(1024, ny - base);
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) col[t] = f.uu[(size_t)(base + t) * nx];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int t = 0; t < cnt; t++) acc = acc + col[t];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) avg = acc / (float)ny;
+    __syncthreads();
+    const float uuavg = avg;
+    const float factime = waves ? 1.0f / 4.0f / s.Trep * s.dt : 1.0f / 60.0f * s.dt;
+    const float taper = fmn(s.totaltime / 100.0f, 1.0f);
+    for (int iy = threadIdx.x; iy < ny; iy += blockDim.x) {
+        size_t i = (size_t)iy * nx;
+        size_t ip = i + p1(0, nx);
+        float zsbnd = s.zsbndi + (s.zsbndn - s.zsbndi) * ((float)iy / (float)ny);
+        float zsplus = f.zs[ip];
+        float ui = 0.0f;
+        if (waves) {
+            float qx = (f.qbo[iy] + s.tsl / s.tnext * (f.qbn[iy] - f.qbo[iy])) * taper;
+            float ht = zsbnd + f.zb[i];
+            ui = qx / ht;
+        }
+        float ur = -1.0f * sqrtf(p.g / f.hh[i]) * (zsplus - zsbnd);
+        float um = factime * uuavg + f.umean[iy] * (1 - factime);
+        f.umean[iy] = um;
+        f.uu[i] = (2.0f) * ui + ur + um;
+        f.zs[i] = zsplus;
+        f.vv[i] = f.vv[ip];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wave action balance
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    size_t i = (size_t)x + (size_t)y * p.nx;
+    float k, c, cg;
+    dispersion_cell(p, s.sigm, f.hh[i], k, c, cg);
+    f.k[i] = k;
+    f.c[i] = c;
+    f.cg[i] = cg;
+}
+
+// Wave action after offshorebndWav + sanity + action (wave_CPU.cpp:24-309).
+XDEV float act_ee(const KP& p, const SP& s, const Fld& f, int x, int y, int th) {
+    float v = (x == 0) ? f.St[y + th * p.ny] * s.taper
+                       : f.ee_in[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
+    return fmx(v, 0.000f) / s.sigm;
+}
+XDEV float act_rr(const KP& p, const Fld& f, int x, int y, int th) {
+    return f.rr_in[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
+}
+
+// propagthetaCPU (wave_CPU.cpp:259-287) with slopesCPU (209-257) at (x, y).
+template <int NT>
+XDEV void ctheta_cell(const KP& p, const SP& s, const Fld& f, int x, int y, float* ct) {
+    const int nt = NT > 0 ? NT : p.nt;
+    const int nx = p.nx, ny = p.ny;
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    size_t i = (size_t)x + (size_t)y * nx;
+    size_t il = (size_t)xm + (size_t)y * nx, ir = (size_t)xp + (size_t)y * nx;
+    size_t ib = (size_t)x + (size_t)ym * nx, it = (size_t)x + (size_t)yp * nx;
+    float ddx = (float)(unsigned)(xp - xm) * p.dx;
+    float ddy = (float)(unsigned)(yp - ym) * p.dx;
+    float dhdx = (f.hh[ir] - f.hh[il]) / ddx;
+    float dudx = (f.uu[ir] - f.uu[il]) / ddx;
+    float dvdx = (f.vv[ir] - f.vv[il]) / ddx;
+    float dhdy = (f.hh[it] - f.hh[ib]) / ddy;
+    float dudy = (f.uu[it] - f.uu[ib]) / ddy;
+    float dvdy = (f.vv[it] - f.vv[ib]) / ddy;
+    float h = f.hh[i];
+    float kh = fmn(f.k[i] * h, 10.0f);
+    float sig = s.sigm / sinhf(2.0f * kh);
+    float w = p.wci * fmn(h / p.hwci, 1.0f);
+#pragma unroll
+    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
+        if (th >= nt) break;
+        float cs = p.cxsth[th], sn = p.sxnth[th];
+        float ctht = sig * (dhdx * sn - dhdy * cs) + w * (cs * (sn * dudx - cs * dudy) + sn * (sn * dvdx - cs * dvdy));
+        ct[th] = fmn(fmx(ctht, -0.25f * s.sigm), 0.25f * s.sigm);
+    }
+}
+
+// x/y upwind advection (xadvecupwind2CPU, yadvecupwind2CPU,
+// wave_CPU.cpp:367-564) + theta advection (thetaadvecuw2hoCPU, 706-808) +
+// Euler update (eulerupwindCPU, 810-828) of ee (IS_EE, speed cg) or rr
+// (speed c) at cell (x, y).  out[th] = the field after the Euler update.
+template <int NT, bool IS_EE>
+XDEV void euler_cell(const KP& p, const SP& s, const Fld& f, int x, int y, const float* ct, float* out) {
+    const int nt = NT > 0 ? NT : p.nt;
+    const int nx = p.nx, ny = p.ny;
+    const size_t N = p.N;
+    int xm = m1(x), xp = p1(x, nx), xm2 = m2(x), xp2 = p2(x, nx);
+    int ym = m1(y), yp = p1(y, ny), ym2 = m2(y), yp2 = p2(y, ny);
+    size_t row = (size_t)y * nx;
+    const float* S = IS_EE ? f.cg : f.c;
+    float sc = S[x + row], sxm = S[xm + row], sxp = S[xp + row];
+    float sym = S[x + (size_t)ym * nx], syp = S[x + (size_t)yp * nx];
+    float w = p.wci * fmn(f.hh[x + row] / p.hwci, 1.0f);
+    float ui = f.uu[x + row] * w, uxm = f.uu[xm + row] * w, uxp = f.uu[xp + row] * w;
+    float vi = w * f.vv[x + row], vym = w * f.vv[x + (size_t)ym * nx], vyp = w * f.vv[x + (size_t)yp * nx];
+    float dxi = 1.0f / p.dx;
+    float e0[NT > 0 ? NT : MAXT], adv[NT > 0 ? NT : MAXT];
+#pragma unroll
+    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
+        if (th >= nt) break;
+        float ec, exm, exm2, exp_, exp2, eym, eym2, eyp, eyp2;
+        if (IS_EE) {
+            ec = act_ee(p, s, f, x, y, th);
+            exm = act_ee(p, s, f, xm, y, th);
+            exm2 = act_ee(p, s, f, xm2, y, th);
+            exp_ = act_ee(p, s, f, xp, y, th);
+            exp2 = act_ee(p, s, f, xp2, y, th);
+            eym = act_ee(p, s, f, x, ym, th);
+            eym2 = act_ee(p, s, f, x, ym2, th);
+            eyp = act_ee(p, s, f, x, yp, th);
+            eyp2 = act_ee(p, s, f, x, yp2, th);
+        } else {
+            const float* r = f.rr_in + (size_t)th * N;
+            ec = r[x + row];
+            exm = r[xm + row];
+            exm2 = r[xm2 + row];
+            exp_ = r[xp + row];
+            exp2 = r[xp2 + row];
+            eym = r[x + (size_t)ym * nx];
+            eym2 = r[x + (size_t)ym2 * nx];
+            eyp = r[x + (size_t)yp * nx];
+            eyp2 = r[x + (size_t)yp2 * nx];
+        }
+        float cs = p.cxsth[th], sn = p.sxnth[th];
+        float cgx = 0.5f * (sc * cs + ui + sxp * cs + uxp);
+        float cgxmin = 0.5f * (sc * cs + ui + sxm * cs + uxm);
+        float ain = cgx > 0.0f ? upw2d(ec, exm) * cgx : upw2d(exp_, exp2) * cgx;
+        float amin = cgxmin > 0.0f ? upw2d(exm, exm2) * cgxmin : upw2d(ec, exp_) * cgxmin;
+        float xa = (ain - amin) * dxi;
+        float cgy = 0.5f * (sc * sn + vi + syp * sn + vyp);
+        float cgymin = 0.5f * (sc * sn + vi + sym * sn + vym);
+        ain = cgy > 0.0f ? upw2d(ec, eym) * cgy : upw2d(eyp, eyp2) * cgy;
+        amin = cgymin > 0.0f ? upw2d(eym, eym2) * cgymin : upw2d(ec, eyp) * cgymin;
+        float ya = (ain - amin) * dxi;
+        e0[th] = ec;
+        adv[th] = xa + ya;
+    }
+#pragma unroll
+    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
+        if (th >= nt) break;
+        float ta;
+        if (nt > 1) {
+            int tm = m1(th), tp = p1(th, nt), tm2 = m2(th), tp2 = p2(th, nt);
+            float cb = 0.5f * (ct[th] + ct[tp]);
+            float eu = cb > 0.0f ? upw2f(e0[th], e0[tm]) : upw2f(e0[tp], e0[tp2]);
+            float arrint = cb * eu;
+            cb = 0.5f * (ct[tm] + ct[th]);
+            eu = cb > 0.0f ? upw2f(e0[tm], e0[tm2]) : upw2f(e0[th], e0[tp]);
+            float arrmint = cb * eu;
+            ta = (arrint - arrmint) / p.dtheta;
+            if (th == nt - 1) ta = (0.0f - arrmint) / p.dtheta;
+            if (th == 0) ta = (arrint - 0.0f) / p.dtheta;
+        } else {
+            ta = 0.0f;
+        }
+        out[th] = e0[th] - s.dt * (adv[th] + ta);
+    }
+}
+
+// roelvinkCPU (wave_CPU.cpp:932-985): Qb and D in double as written.
+XDEV float roelvink_cell(const KP& p, float Trep, float E, float h, float k, float fw) {
+    float fac = (float)(8.0 / (double)p.rho / (double)p.g);
+    float Ht = sqrtf(E * fac);
+    float Qb = (float)(1.0 - exp(dmx(-1.0 * (double)powf(Ht / p.gammaa / h, p.nexp), -100.0)));
+    Qb = fmn(Qb, 1.0f);
+    float Dv = (float)((double)Qb * 2.0 * (double)p.alpha / (double)Trep * (double)E * (double)Ht / (double)h);
+    float uorb = (float)(PI_ * (double)Ht / (double)(Trep * sinhf(fmn(k * h, 10.0f))));
+    return (float)((double)Dv + (double)0.66666666f / PI_ * (double)p.rho * (double)fw * (double)uorb *
+                   (double)uorb * (double)uorb);
+}
+
+// baldockCPU (wave_CPU.cpp:987-1027), breakmodel != 1.
+XDEV float baldock_cell(const KP& p, float Trep, float Ht, float h, float k, float fw) {
+    float khh = k * h;
+    float Hb = (0.88f / k) * tanhf(p.gammaa * khh / 0.88f);
+    float Rr = Hb / fmx(Ht, 0.00001f);
+    float Qb = expf(-1.0f * Rr * Rr);
+    float Dv = 0.25f * p.alpha * Qb * p.rho * (1.0f / Trep) * p.g * (Hb * Hb + Ht * Ht);
+    float uorb = (float)(PI_ * (double)Ht / (double)(Trep * sinhf(fmn(k * h, 10.0f))));
+    return (float)((double)Dv + (double)0.66666666f / PI_ * (double)p.rho * (double)fw * (double)uorb *
+                   (double)uorb * (double)uorb);
+}
+
+// energyCPU + energintCPU + roelvink/baldock + dissipationCPU
+// (wave_CPU.cpp:873-1089) at cell i: e (action after Euler) -> ee after
+// dissipation, r (roller after Euler) -> rr after dissipation.
+template <int NT>
+XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e, float* r, float& Rout,
+                      float& DRout) {
+    const int nt = NT > 0 ? NT : p.nt;
+    float h = f.hh[i];
+    float Et = 0.0f;
+#pragma unroll
+    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
+        if (th >= nt) break;
+        e[th] = fmx(e[th] * s.sigm, 0.0f);
+        Et = Et + e[th];
+    }
+    Et = Et * p.dtheta;
+    float Ht = sqrtf(Et / (p.rho * p.g / 8.0f));
+    float idiv = fmx(1.0f, powf(Ht / (p.gammax * h), 2.0f));
+    Ht = fmn(Ht, p.gammax * h);
+    float E = (p.rho * p.g * Ht * Ht) / 8.0f;
+    float kk = f.k[i];
+    float fw = fwm_of(p, f.stdep[i]);
+    float Dv = p.breakmodel == 1 ? roelvink_cell(p, s.Trep, E, h, kk, fw) : baldock_cell(p, s.Trep, Ht, h, kk, fw);
+    float cc = f.c[i];
+    float w = p.wci * fmn(h / p.hwci, 1.0f);
+    float uui = f.uu[i] * w, vvi = f.vv[i] * w;
+    float RR = 0.0f, drr = 0.0f;
+    bool wetc = h > p.eps;
+    float Emx = fmx(E, 0.0000001f);
+#pragma unroll
+    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
+        if (th >= nt) break;
+        float eet = e[th] / idiv;
+        float cx = cc * p.cxsth[th] + uui;
+        float cy = cc * p.sxnth[th] + vvi;
+        float dd = eet * Dv / Emx;
+        float rrt;
+        if (wetc) {
+            rrt = r[th];
+            float rrd = 2.0f * p.g * p.beta * rrt / sqrtf(cx * cx + cy * cy);
+            eet = fmx(eet - s.dt * dd, 0.0f);
+            rrt = fmx(rrt + s.dt * (dd - rrd), 0.0f);
+            drr = drr + rrd;
+        } else {
+            eet = 0.0f;
+            rrt = 0.0f;
+            drr = 0.0f;
+        }
+        RR = RR + rrt;
+        e[th] = eet;
+        r[th] = rrt;
+    }
+    Rout = RR * p.dtheta;
+    DRout = drr * p.dtheta;
+}
+
+// The fused wave step.  Thread = cell.  Rows 0 and ny-1 take the
+// rollerlatbnd copies (wave_CPU.cpp:830-871) from rows 1 and ny-2, so their
+// threads also evaluate the pipeline of that source row.
+template <int NT>
+__global__ __launch_bounds__(256) void k_wave(KP p, SP s, Fld f) {
+    constexpr int A = NT > 0 ? NT : MAXT;
+    const int nt = NT > 0 ? NT : p.nt;
+    __shared__ float wred[256];
+    int x = gx(), y = gy();
+    float wmin = 9999999.9f;
+    if (x < p.nx && y < p.ny) {
+        const int nx = p.nx, ny = p.ny;
+        const size_t N = p.N;
+        size_t i = (size_t)x + (size_t)y * nx;
+        int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
+        size_t is = (size_t)x + (size_t)ys * nx;
+        float wet = f.hh[i] > p.eps ? 1.0f : 0.0f;
+        float ct[A], e[A], r[A];
+        // WAVEDT (Wave_kernel.cu:232-249) uses this cell's ctheta.
+        ctheta_cell<NT>(p, s, f, x, y, ct);
+#pragma unroll
+        for (int th = 0; th < A; th++) {
+            if (th >= nt) break;
+            float v = p.CFL * p.dtheta / fabsf(ct[th]);
+            wmin = (v < wmin) ? v : wmin;  // fminf: a NaN speed is ignored
+        }
+        if (ys != y) ctheta_cell<NT>(p, s, f, x, ys, ct);
+        euler_cell<NT, true>(p, s, f, x, ys, ct, e);
+        if (p.roller == 1) {
+            euler_cell<NT, false>(p, s, f, x, ys, ct, r);
+        } else {
+#pragma unroll
+            for (int th = 0; th < A; th++) {
+                if (th >= nt) break;
+                r[th] = act_rr(p, f, x, y, th);
+            }
+        }
+        float Rv, DRv;
+        float efin[A];
+        if (ys != y) {
+            // source row pipeline for the second rollerlatbnd(ee)
+            float es[A], rs[A];
+#pragma unroll
+            for (int th = 0; th < A; th++) {
+                if (th >= nt) break;
+                es[th] = e[th];
+                rs[th] = r[th];
+                e[th] = e[th] * wet;
+                if (p.roller == 1) r[th] = r[th] * wet;
+            }
+            if (p.roller != 1) {
+#pragma unroll
+                for (int th = 0; th < A; th++) {
+                    if (th >= nt) break;
+                    rs[th] = act_rr(p, f, x, ys, th);
+                }
+            }
+            float Rs, DRs;
+            dissip_cell<NT>(p, s, f, is, es, rs, Rs, DRs);
+            dissip_cell<NT>(p, s, f, i, e, r, Rv, DRv);
+#pragma unroll
+            for (int th = 0; th < A; th++) {
+                if (th >= nt) break;
+                efin[th] = es[th] * wet;
+            }
+        } else {
+            dissip_cell<NT>(p, s, f, i, e, r, Rv, DRv);
+#pragma unroll
+            for (int th = 0; th < A; th++) {
+                if (th >= nt) break;
+                efin[th] = e[th];
+            }
+        }
+        // meandirCPU (wave_CPU.cpp:1092-1116) and radstressCPU (1118-1157)
+        float cgi = f.cg[i], ci = f.c[i];
+        float nn = cgi / ci;
+        float sume = 0.0f, sumethet = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f, rxx = 0.0f, ryy = 0.0f, rxy = 0.0f;
+#pragma unroll
+        for (int th = 0; th < A; th++) {
+            if (th >= nt) break;
+            float ev = efin[th], rv = r[th];
+            float cs = p.cxsth[th], sn = p.sxnth[th];
+            sume = sume + ev;
+            sumethet = sumethet + ev * p.theta[th];
+            txx = txx + ((1.0f + cs * cs) * ev);
+            tyy = tyy + ((1.0f + sn * sn) * ev);
+            txy = txy + (sn * cs * ev);
+            rxx = rxx + ((cs * cs) * rv);
+            ryy = ryy + ((sn * sn) * rv);
+            rxy = rxy + (sn * cs * rv);
+            f.ee_out[i + (size_t)th * N] = ev;
+            f.rr_out[i + (size_t)th * N] = rv;
+        }
+        float sumr = txx;  // keep radstress sums in their own registers
+        (void)sumr;
+        f.Sxx[i] = (nn * txx - 0.5f * sume) * p.dtheta + rxx * p.dtheta;
+        f.Syy[i] = (nn * tyy - 0.5f * sume) * p.dtheta + ryy * p.dtheta;
+        f.Sxy[i] = nn * txy * p.dtheta + rxy * p.dtheta;
+        float sm = fmx(sume, 0.00001f);
+        f.thetamean[i] = (sumethet / (float)nt) / (sm / (float)nt);
+        float Em = sm * p.dtheta;
+        float Hm = sqrtf(sm * p.dtheta / (p.rho * p.g / 8));
+        f.E[i] = Em;
+        f.H[i] = Hm;
+        f.R[i] = Rv;
+        f.DR[i] = DRv;
+        // breakerdelayCPU (wave_CPU.cpp:1334-1374), called with R
+        float hmin = 0.2f;
+        float hhi = fmx(f.hh[i], hmin);
+        float cib = fmx(ci, sqrtf(hmin * p.g));
+        float uorb = (float)(PI_ * (double)Hm / (double)s.Trep /
+                             (double)sinhf(fmn(fmx(f.k[i], 0.01f) * hhi, 10.0f)));
+        f.urms[i] = uorb / (sqrtf(2.0f));
+        float ustw = Em / cib / p.rho / hhi;
+        float ustr = 2.0f * Rv / cib / p.rho / hhi;
+        f.ust_raw[i] = ustw + ustr;
+    }
+    // block minimum of the wave CFL limit
+    int t = threadIdx.x + threadIdx.y * BX;
+    wred[t] = wmin;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st) wred[t] = fmn(wred[t], wred[t + st]);
+        __syncthreads();
+    }
+    if (t == 0) f.red_wave[blockIdx.x + blockIdx.y * gridDim.x] = wred[0];
+}
+
+// wavforceCPU (wave_CPU.cpp:1159-1223) at (x, y)
+XDEV void wavforce_at(const KP& p, const Fld& f, int x, int y, float& Fxv, float& Fyv) {
+    const int nx = p.nx, ny = p.ny;
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    auto I = [nx](int a, int b) { return (size_t)a + (size_t)b * nx; };
+    float Sxxi = f.Sxx[I(x, y)], Sxxr = f.Sxx[I(xp, y)];
+    float Sxyt = f.Sxy[I(x, yp)], Sxytr = f.Sxy[I(xp, yp)];
+    float Sxyb = f.Sxy[I(x, ym)], Sxybr = f.Sxy[I(xp, ym)];
+    float Syyi = f.Syy[I(x, y)], Syyt = f.Syy[I(x, yp)];
+    float Sxyr = f.Sxy[I(xp, y)], Sxyl = f.Sxy[I(xm, y)], Sxytl = f.Sxy[I(xm, yp)];
+    Fxv = -1 * (Sxxr - Sxxi) / p.dx - 0.5f * (Sxyt + Sxytr - Sxyb - Sxybr) / (2.0f * p.dx);
+    Fyv = -1 * (Syyt - Syyi) / p.dx - 0.5f * (Sxyr + Sxytr - Sxyl - Sxytl) / (2.0f * p.dx);
+}
+
+// wavforce + twodimbndnoix(Fx) + twodimbnd(Fy) + twodimbnd(ust)
+// (Wavestep.cu:1099-1125; CPU twins wave_CPU.cpp:1224-1331: the neighbour
+// value read is the pre-update one and the ix==0 rule wins).
+__global__ __launch_bounds__(256) void k_wave_force(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    const int nx = p.nx, ny = p.ny;
+    size_t i = (size_t)x + (size_t)y * nx;
+    float wet = f.hh[i] > p.eps ? 1.0f : 0.0f;
+    int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
+    float Fxv, Fyv;
+    wavforce_at(p, f, x, ys, Fxv, Fyv);
+    float fx = ys != y ? Fxv * wet : Fxv;
+    float fy, ust;
+    if (x == 0) {
+        float a, b;
+        wavforce_at(p, f, p1(0, nx), y, a, b);
+        fy = b * wet;
+        ust = f.ust_raw[(size_t)p1(0, nx) + (size_t)y * nx] * wet;
+    } else if (ys != y) {
+        fy = Fyv * wet;
+        ust = f.ust_raw[(size_t)x + (size_t)ys * nx] * wet;
+    } else {
+        fy = Fyv;
+        ust = f.ust_raw[i];
+    }
+    f.Fx[i] = fx;
+    f.Fy[i] = fy;
+    f.ust[i] = ust;
+}
+
+// ---------------------------------------------------------------------------
+// Shallow-water flow
+// ---------------------------------------------------------------------------
+XDEV size_t ix2(const KP& p, int x, int y) { return (size_t)x + (size_t)y * p.nx; }
+// udepthmomcont / vdepthmomcont (Flow_kernel.cu:546-651)
+XDEV float hum_at(const KP& p, const Fld& f, int x, int y) {
+    return fmx(0.5f * (f.hh[ix2(p, x, y)] + f.hh[ix2(p, p1(x, p.nx), y)]), p.eps);
+}
+XDEV float hvm_at(const KP& p, const Fld& f, int x, int y) {
+    return fmx(0.5f * (f.hh[ix2(p, x, y)] + f.hh[ix2(p, x, p1(y, p.ny))]), p.eps);
+}
+XDEV float wetu_at(const KP& p, const Fld& f, int x, int y) { return (float)(f.wet[ix2(p, x, y)] & 1); }
+XDEV float wetv_at(const KP& p, const Fld& f, int x, int y) { return (float)((f.wet[ix2(p, x, y)] >> 1) & 1); }
+
+// smago (Flow_kernel.cu:1282-1317) and the wet masks of
+// udepthmomcont/vdepthmomcont from the previous step's hu/hv.
+__global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    const int nx = p.nx, ny = p.ny;
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    size_t i = ix2(p, x, y);
+    float h = f.hh[i];
+    float humi = 0.5f * (h + f.hh[ix2(p, xp, y)]);
+    float hvmi = 0.5f * (h + f.hh[ix2(p, x, yp)]);
+    uint8_t m = 0;
+    if (f.hu[i] > p.eps && humi > p.eps) m |= 1;
+    if (f.hv[i] > p.eps && hvmi > p.eps) m |= 2;
+    f.wet[i] = m;
+    float turb = p.nuhfac * h * powf(f.DR[i] / p.rho, 1.0f / 3.0f);
+    if (p.usesmago == 1) {
+        const float* uu = f.uu;
+        const float* vv = f.vv;
+        float dudx = (uu[i] - uu[ix2(p, xm, y)]) / p.dx;
+        float dudy = 0.50f * (uu[ix2(p, x, yp)] - uu[ix2(p, x, ym)] + uu[ix2(p, xm, yp)] - uu[ix2(p, xm, ym)]) / p.dx;
+        float dvdy = (vv[i] - vv[ix2(p, x, ym)]) / p.dx;
+        float dvdx = 0.50f * (vv[ix2(p, xp, y)] - vv[ix2(p, xm, y)] + vv[ix2(p, xp, ym)] - vv[ix2(p, xm, ym)]) / p.dx;
+        float tau = sqrtf(2.0f * dudx * dudx + 2.0f * dvdy * dvdy + powf(dudy + dvdx, 2.0f));
+        f.nuh[i] = fmx(p.nuh * p.nuh * p.dx * p.dx * tau, turb);
+    } else {
+        f.nuh[i] = fmx(p.nuh, turb);
+    }
+}
+
+// eulerustep (Flow_kernel.cu:1487-1544) at (x, y) with ududx_adv (752-823),
+// vdudy_adv (887-949), viscou1/viscou2 (1319-1397), wlevslopes (410-466).
+XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
+    const int nx = p.nx, ny = p.ny;
+    size_t i = ix2(p, x, y);
+    if (x == 0) return f.uu[i];
+    if (!(f.wet[i] & 1)) return 0.0f;
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    const float* uu = f.uu;
+    const float* vv = f.vv;
+    const float dx = p.dx;
+    size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), br = ix2(p, xp, ym);
+    float hum = hum_at(p, f, x, y);
+    // ududx_adv
+    float res = 0.0f;
+    float qin = 0.5f * (f.hu[i] * uu[i] + f.hu[l] * uu[l]);
+    float du = uu[i] - uu[l];
+    if (qin > 0.0f) {
+        if (du > 0.5f) res = res + 0.5f * (uu[l] + uu[i]) * (uu[i] - uu[l]) / dx;
+        else res = res + qin / hum * (uu[i] - uu[l]) / dx;
+    }
+    qin = -0.5f * (f.hu[i] * uu[i] + f.hu[r] * uu[r]);
+    du = uu[r] - uu[i];
+    if (qin > 0.0f) {
+        if (du > 0.5f) res = res - 0.5f * (uu[r] + uu[i]) * (uu[i] - uu[r]) / dx;
+        else res = res + qin / hum * (uu[i] - uu[r]) / dx;
+    }
+    float ududx = res;
+    // vdudy_adv
+    res = 0.0f;
+    qin = 0.5f * (vv[b] * f.hv[b] + vv[br] * f.hv[br]);
+    float dv = vv[i] - vv[b];
+    if (qin > 0.0f) {
+        if (dv > 0.5f) res = res + 0.5f * (vv[b] + vv[br]) * (uu[i] - uu[b]) / dx;
+        else res = res + qin / hum * (uu[i] - uu[b]) / dx;
+    }
+    qin = -0.5f * (vv[i] * f.hv[i] + vv[r] * f.hv[r]);
+    dv = vv[t] - vv[i];
+    if (qin > 0.0f) {
+        if (dv > 0.5f) res = res - 0.5f * (vv[i] + vv[r]) * (uu[i] - uu[t]) / dx;
+        else res = res + qin / hum * (uu[i] - uu[t]) / dx;
+    }
+    float vdudy = res;
+    // viscou1 + viscou2
+    const float* nuh = f.nuh;
+    float dudx1 = nuh[r] * f.hh[r] * (uu[r] - uu[i]) / dx;
+    float dudx2 = nuh[i] * f.hh[i] * (uu[i] - uu[l]) / dx;
+    float viscu = (1.0f / hum) * ((dudx1 - dudx2) / (dx));
+    float nuh1 = 0.25f * (nuh[i] + nuh[r] + nuh[ix2(p, xp, yp)] + nuh[t]);
+    float nuh2 = 0.25f * (nuh[i] + nuh[r] + nuh[br] + nuh[b]);
+    float dudy1 = nuh1 * 0.5f * (hvm_at(p, f, x, y) + hvm_at(p, f, xp, y)) * (uu[t] - uu[i]) / dx;
+    float dudy2 = nuh2 * 0.5f * (hvm_at(p, f, x, ym) + hvm_at(p, f, xp, ym)) * (uu[i] - uu[b]) / dx;
+    viscu = viscu + (1.0f / hum) * ((dudy1 - dudy2) / dx * wetu_at(p, f, x, yp) * wetu_at(p, f, x, ym));
+    // wlevslopes
+    float dzsdx = (f.zs[r] - f.zs[i]) / dx;
+    float taubx = cfm_of(p, f.stdep[i]) * p.rho * f.ueu[i] *
+                  sqrtf(1.3456f * f.urms[i] * f.urms[i] + f.vmageu[i] * f.vmageu[i]);
+    float hmin = 0.2f;
+    return uu[i] - s.dt * (ududx + vdudy - viscu + p.g * dzsdx + taubx / (p.rho * f.hu[i]) -
+                           f.Fx[i] / (p.rho * fmx(hum, hmin)) -
+                           1.25f * p.Cd * (float)s.signwu * s.windu * s.windu / (p.rho * hum) - p.fc * f.vu[i]);
+}
+
+// eulerustep + latbnd(uu) (Flow_kernel.cu:1778-1809): rows 0 / ny-1 copy
+// the updated rows 1 / ny-2.
+__global__ __launch_bounds__(256) void k_flow_u(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
+    f.uu_b[ix2(p, x, y)] = eulerustep_at(p, s, f, x, ys);
+}
+
+// viscov1 (Flow_kernel.cu:1399-1429) at (x, y)
+XDEV float viscov1_at(const KP& p, const Fld& f, int x, int y) {
+    int ym = m1(y), yp = p1(y, p.ny);
+    size_t i = ix2(p, x, y), t = ix2(p, x, yp), b = ix2(p, x, ym);
+    const float* vv = f.vv;
+    float dvdy1 = f.nuh[t] * f.hh[t] * (vv[t] - vv[i]) / p.dx;
+    float dvdy2 = f.nuh[i] * f.hh[i] * (vv[i] - vv[b]) / p.dx;
+    return (1.0f / hvm_at(p, f, x, y)) * (dvdy1 - dvdy2) / p.dx * wetv_at(p, f, x, yp) * wetv_at(p, f, x, ym);
+}
+
+// eulervstep (Flow_kernel.cu:1546-1599) at (x, y) with udvdx_adv
+// (1159-1210, on the updated uu), vdvdy_adv (1024-1074), viscov1 +
+// viscovbnd + viscov2 (1399-1483), wlevslopes.
+XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
+    const int nx = p.nx, ny = p.ny;
+    size_t i = ix2(p, x, y);
+    if (x == 0 || y >= ny - 2) return f.vv[i];
+    if (!((f.wet[i] >> 1) & 1)) return 0.0f;
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    const float* un = f.uu_b;
+    const float* vv = f.vv;
+    const float dx = p.dx;
+    size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), tl = ix2(p, xm, yp);
+    float hvm = hvm_at(p, f, x, y);
+    // udvdx_adv
+    float res = 0.0f;
+    float qin = (float)(0.5 * (double)(un[l] * f.hu[l] + un[tl] * f.hu[tl]));
+    float du = un[i] - un[l];
+    if (qin > 0.0f) {
+        if (du > 0.5f) res = res + 0.5f * (un[l] + un[tl]) * (vv[i] - vv[l]) / (dx);
+        else res = res + qin / hvm * (vv[i] - vv[l]) / (dx);
+    }
+    qin = (float)(-0.5 * (double)(un[i] * f.hu[i] + un[t] * f.hu[t]));
+    du = un[r] - un[i];
+    if (qin > 0.0f) {
+        if (du > 0.5f) res = res - 0.5f * (un[i] + un[r]) * (vv[i] - vv[r]) / (dx);
+        else res = res + qin / hvm * (vv[i] - vv[r]) / (dx);
+    }
+    float udvdx = res;
+    // vdvdy_adv (interior: ix>0, 0<iy<ny-2)
+    res = 0.0f;
+    if (y > 0) {
+        qin = 0.5f * (vv[i] * f.hv[i] + vv[b] * f.hv[b]);
+        float dv = vv[i] - vv[b];
+        if (qin > 0.0f) {
+            if (dv > 0.5f) res = res + 0.5f * (vv[b] - vv[i]) * (vv[i] - vv[b]) / (dx);
+            else res = res + qin / hvm * (vv[i] - vv[b]) / (dx);
+        }
+        qin = -0.5f * (f.hv[i] * vv[i] + f.hv[t] * vv[t]);
+        dv = vv[t] - vv[i];
+        if (qin > 0.0f) {
+            if (dv > 0.5f) res = res - 0.5f * (vv[t] + vv[i]) * (vv[i] - vv[t]) / (dx);
+            else res = res + qin / hvm * (vv[i] - vv[t]) / (dx);
+        }
+    }
+    float vdvdy = res;
+    // viscov1 (row 0 takes row 1 through viscovbnd) + viscov2
+    float viscv = viscov1_at(p, f, x, y == 0 ? p1(0, ny) : y);
+    const float* nuh = f.nuh;
+    float nuh1 = 0.25f * (nuh[i] + nuh[r] + nuh[ix2(p, xp, yp)] + nuh[t]);
+    float nuh2 = 0.25f * (nuh[i] + nuh[l] + nuh[tl] + nuh[t]);
+    float dvdx1 = nuh1 * 0.5f * (hum_at(p, f, x, y) + hum_at(p, f, x, yp)) * (vv[r] - vv[i]) / dx;
+    float dvdx2 = nuh2 * 0.5f * (hum_at(p, f, xm, y) + hum_at(p, f, xm, yp)) * (vv[i] - vv[l]) / dx;
+    viscv = viscv + (1.0f / hvm) * ((dvdx1 - dvdx2) / (dx) * wetv_at(p, f, xp, y) * wetv_at(p, f, xm, y));
+    float dzsdy = (f.zs[t] - f.zs[i]) / dx;
+    float tauby = cfm_of(p, f.stdep[i]) * p.rho * f.vev[i] *
+                  sqrtf(1.3456f * f.urms[i] * f.urms[i] + f.vmagev[i] * f.vmagev[i]);
+    float hmin = 0.2f;
+    return vv[i] - s.dt * (udvdx + vdvdy - viscv + p.g * dzsdy + tauby / (p.rho * f.hv[i]) -
+                           f.Fy[i] / (p.rho * fmx(hvm, hmin)) + p.fc * f.uv[i] -
+                           1.25f * p.Cd * (float)s.signwv * s.windv * s.windv / (p.rho * hvm));
+}
+
+// eulervstep + vvlatbnd (Flow_kernel.cu:1812-1844): rows 0 / ny-2 copy the
+// updated rows 1 / ny-3.
+__global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    int ys = (y == 0) ? 1 : (y == p.ny - 2 ? p.ny - 3 : y);
+    f.vv_b[ix2(p, x, y)] = eulervstep_at(p, s, f, x, ys);
+}
+
+// wrkuu2Ocorr (Flow_kernel.cu:1960-2044) and wrkvv2Ocorr (2046-2128)
+__global__ __launch_bounds__(256) void k_flow_wrk(KP p, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    const int nx = p.nx, ny = p.ny;
+    int xm = m1(x), xm2 = m2(x), xp = p1(x, nx), xp2 = p2(x, nx);
+    int ym = m1(y), ym2 = m2(y), yp = p1(y, ny), yp2 = p2(y, ny);
+    size_t i = ix2(p, x, y);
+    const float *zs = f.zs, *zb = f.zb, *qx = f.qx, *qy = f.qy;
+    auto I = [&](int a, int b) { return ix2(p, a, b); };
+    const float two_eps = p.eps * 2.0f;
+    // u
+    {
+        const float *uu = f.uu_b, *uo = f.uuold;
+        float w1 = 0.0f, w2 = 0.0f;
+        float minzs = fmn(fmn(fmn(fmn(zs[I(xm2, y)], zs[I(xm, y)]), zs[i]), zs[I(xp, y)]), zs[I(xp2, y)]);
+        float minzb = fmn(fmn(fmn(fmn(zb[I(xm2, y)], zb[I(xm, y)]), zb[i]), zb[I(xp, y)]), zb[I(xp2, y)]);
+        if (x > 0 && minzs + minzb > two_eps) {
+            float qs = qx[i] + qx[I(xm, y)];
+            if (qs > 0.0f) w1 = 0.5f * minmod(uu[i] - uo[I(xm, y)], uu[I(xm, y)] - uo[I(xm2, y)]);
+            if (qs < 0.0f) w1 = -0.5f * minmod(uo[i] - uu[I(xm, y)], uo[I(xp, y)] - uu[i]);
+        }
+        minzs = fmn(fmn(fmn(zs[i], zs[I(x, ym)]), zs[I(x, yp)]), zs[I(x, yp2)]);
+        minzs = fmn(minzs, fmn(fmn(fmn(zs[I(xp, y)], zs[I(xp, ym)]), zs[I(xp, yp)]), zs[I(xp, yp2)]));
+        minzb = fmn(fmn(fmn(zb[i], zb[I(x, ym)]), zb[I(x, yp)]), zb[I(x, yp2)]);
+        minzb = fmn(minzb, fmn(fmn(fmn(zb[I(xp, y)], zb[I(xp, ym)]), zb[I(xp, yp)]), zb[I(xp, yp2)]));
+        if (x > 0 && y > 0 && minzs + minzb > two_eps) {
+            float qs = qy[I(xp, y)] + qy[i];
+            if (qs > 0.0f) w2 = 0.5f * minmod(uu[I(x, yp)] - uo[i], uu[i] - uo[I(x, ym)]);
+            if (qs < 0.0f) w2 = -0.5f * minmod(uo[I(x, yp)] - uu[i], uo[I(x, yp2)] - uu[I(x, yp)]);
+        }
+        f.w1u[i] = w1;
+        f.w2u[i] = w2;
+    }
+    // v
+    {
+        const float *vv = f.vv_b, *vo = f.vvold;
+        float w1 = 0.0f, w2 = 0.0f;
+        float minzs = fmn(fmn(fmn(fmn(zs[I(x, ym2)], zs[I(x, ym)]), zs[i]), zs[I(x, yp)]), zs[I(x, yp2)]);
+        float minzb = fmn(fmn(fmn(fmn(zb[I(x, ym2)], zb[I(x, ym)]), zb[i]), zb[I(x, yp)]), zb[I(x, yp2)]);
+        bool in = x > 0 && y > 0 && y < ny - 2;
+        if (in && minzs + minzb > two_eps) {
+            float qs = qy[i] + qy[I(x, ym)];
+            if (qs > 0.0f) w1 = 0.5f * minmod(vv[i] - vo[I(x, ym)], vv[I(x, ym)] - vo[I(x, ym2)]);
+            if (qs < 0.0f) w1 = -0.5f * minmod(vo[i] - vv[I(x, ym)], vo[I(x, yp)] - vv[i]);
+        }
+        minzs = fmn(fmn(fmn(zs[I(xm, y)], zs[i]), zs[I(xp, y)]), zs[I(xp2, y)]);
+        minzs = fmn(minzs, fmn(fmn(fmn(zs[I(xm, yp)], zs[I(x, yp)]), zs[I(xp, yp)]), zs[I(xp2, yp)]));
+        minzb = fmn(fmn(fmn(zb[I(xm, y)], zb[i]), zb[I(xp, y)]), zb[I(xp2, y)]);
+        minzb = fmn(minzb, fmn(fmn(fmn(zb[I(xm, yp)], zb[I(x, yp)]), zb[I(xp, yp)]), zb[I(xp2, yp)]));
+        if (in && minzs + minzb > two_eps) {
+            float qs = qx[I(x, yp)] + qx[i];
+            if (qs > 0.0f) w2 = 0.5f * minmod(vv[I(xp, y)] - vo[i], vv[i] - vo[I(xm, y)]);
+            if (qs < 0.0f) w2 = -0.5f * minmod(vo[I(xp, y)] - vv[i], vo[I(xp2, y)] - vv[I(xp, y)]);
+        }
+        f.w1v[i] = w1;
+        f.w2v[i] = w2;
+    }
+}
+
+// uu2Ocorr (Flow_kernel.cu:2210-2245), vv2Ocorr (2247-2282), then
+// depthhu / depthhv (653-750, uumin = 0) on the corrected velocities.
+__global__ __launch_bounds__(256) void k_flow_corr(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    const int nx = p.nx, ny = p.ny;
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    size_t i = ix2(p, x, y);
+    auto I = [&](int a, int b) { return ix2(p, a, b); };
+    const float *qx = f.qx, *qy = f.qy;
+    const float dx = p.dx, dt = s.dt;
+    float u = f.uu_b[i], v = f.vv_b[i];
+    if (x > 0 && y > 0) {
+        float qe = 0.5f * (qx[I(xp, y)] + qx[i]);
+        float qw = 0.5f * (qx[i] + qx[I(xm, y)]);
+        float qn = 0.5f * (qy[I(x, ym)] + qy[I(xp, ym)]);
+        float qs = 0.5f * (qy[i] + qy[I(xm, y)]);
+        u = u - dt / hum_at(p, f, x, y) *
+                    ((qe * f.w1u[I(xp, y)] - qw * f.w1u[i]) / dx + (qs * f.w2u[i] - qn * f.w2u[I(x, ym)]) / dx);
+        qe = 0.5f * (qx[I(x, yp)] + qx[i]);
+        qw = 0.5f * (qx[I(xm, y)] + qx[I(xm, yp)]);
+        qn = 0.5f * (qy[i] + qy[I(x, ym)]);
+        qs = 0.5f * (qy[I(x, yp)] + qy[i]);
+        v = v - dt / hvm_at(p, f, x, y) *
+                    ((qe * f.w2v[i] - qw * f.w2v[I(xm, y)]) / dx + (qs * f.w1v[I(x, yp)] - qn * f.w1v[i]) / dx);
+    }
+    f.uu[i] = u;
+    f.vv[i] = v;
+    const float *zs = f.zs, *zb = f.zb;
+    const float ummn = 0.0f;
+    size_t r = I(xp, y), t = I(x, yp);
+    float hb = fmx(-1.0f * zb[i], -1.0f * zb[r]);
+    float h;
+    if (u > ummn) h = zs[i] - hb;
+    else if (u < -1.0f * ummn) h = zs[r] - hb;
+    else h = fmx(fmx(zs[i], zs[r]) - hb, p.eps);
+    f.hu[i] = fmx(h, 0.0f);
+    hb = fmx(-1.0f * zb[i], -1.0f * zb[t]);
+    if (v > ummn) h = zs[i] - hb;
+    else if (v < -1 * ummn) h = zs[t] - hb;
+    else h = fmx(fmx(zs[i], zs[t]) - hb, p.eps);
+    f.hv[i] = fmx(h, 0.0f);
+}
+
+// CalcQFlow (Flow_kernel.cu:1917-1958) + continuity (1626-1681)
+__global__ __launch_bounds__(256) void k_flow_cont(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    size_t i = ix2(p, x, y);
+    float z = f.zs[i];
+    if (x > 0) {
+        size_t l = ix2(p, x - 1, y), b = ix2(p, x, m1(y));
+        float qxi = f.uu[i] * f.hu[i], qxl = f.uu[l] * f.hu[l];
+        float qyi = f.vv[i] * f.hv[i], qyb = f.vv[b] * f.hv[b];
+        float dzdt = (qxl - qxi + qyb - qyi) / p.dx;
+        f.dzsdt[i] = dzdt;
+        z = z + dzdt * s.dt;
+    }
+    f.zsn[i] = z;
+}
+
+// wrkzs2Ocorr (Flow_kernel.cu:2130-2208)
+__global__ __launch_bounds__(256) void k_flow_wrkzs(KP p, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    const int nx = p.nx, ny = p.ny;
+    int xm = m1(x), xp = p1(x, nx), xp2 = p2(x, nx);
+    int ym = m1(y), yp = p1(y, ny), yp2 = p2(y, ny);
+    size_t i = ix2(p, x, y);
+    auto I = [&](int a, int b) { return ix2(p, a, b); };
+    const float *zs = f.zsn, *zo = f.zsold, *zb = f.zb;
+    const float two_eps = p.eps * 2.0f;
+    float u = f.uu[i], v = f.vv[i];
+    float w1 = 0.0f, w2 = 0.0f;
+    float minzs = fmn(fmn(fmn(zs[I(xm, y)], zs[i]), zs[I(xp, y)]), zs[I(xp2, y)]);
+    float minzb = fmn(fmn(fmn(zb[I(xm, y)], zb[i]), zb[I(xp, y)]), zb[I(xp2, y)]);
+    if (x > 0 && minzs + minzb > two_eps) {
+        if (u > 0.0f) w1 = u * 0.5f * minmod(zs[I(xp, y)] - zo[i], zs[i] - zo[I(xm, y)]);
+        if (u < 0.0f) w1 = -1.0f * u * 0.5f * minmod(zo[I(xp2, y)] - zs[I(xp, y)], zo[I(xp, y)] - zs[i]);
+    }
+    minzs = fmn(fmn(fmn(zs[I(x, ym)], zs[i]), zs[I(x, yp)]), zs[I(x, yp2)]);
+    minzb = fmn(fmn(fmn(zb[I(x, ym)], zb[i]), zb[I(x, yp)]), zb[I(x, yp2)]);
+    if (x > 0 && y > 0 && minzs + minzb > two_eps) {
+        if (v > 0.0f) w2 = v * 0.5f * minmod(zs[I(x, yp)] - zo[i], zs[i] - zo[I(x, ym)]);
+        if (v < 0.0f) w2 = v * 0.5f * minmod(zo[I(x, yp2)] - zs[I(x, yp)], zo[I(x, yp)] - zs[i]);
+    }
+    f.w1z[i] = w1;
+    f.w2z[i] = w2;
+}
+
+// zs2Ocorr (Flow_kernel.cu:2284-2319) value at (x, y)
+XDEV float zs2_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
+    size_t i = ix2(p, x, y);
+    float z = f.zsn[i];
+    if (x > 0 && y > 0)
+        z = z - s.dt * ((f.w1z[i] - f.w1z[ix2(p, x - 1, y)]) / p.dx + (f.w2z[i] - f.w2z[ix2(p, x, y - 1)]) / p.dx);
+    return z;
+}
+
+// zs2Ocorr + latbnd(zs) (rows 0 / ny-1 copy rows 1 / ny-2) + hsbnd
+// (Flow_kernel.cu:1683-1710); qx/qy get the correction fluxes.
+__global__ __launch_bounds__(256) void k_flow_zs(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    size_t i = ix2(p, x, y);
+    int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
+    float z = zs2_at(p, s, f, x, ys);
+    float qxv = f.uu[i] * f.hu[i], qyv = f.vv[i] * f.hv[i];
+    if (x > 0 && y > 0) {
+        qxv = qxv + f.w1z[i];
+        qyv = qyv + f.w2z[i];
+    }
+    f.zs[i] = z;
+    f.qx[i] = qxv;
+    f.qy[i] = qyv;
+    f.hh[i] = fmx(f.zb[i] + z, p.eps);
+}
+
+// calcuvvu (Flow_kernel.cu:467-541) at (x, y)
+XDEV void calcuvvu_at(const KP& p, const Fld& f, int x, int y, float* o) {
+    const int nx = p.nx, ny = p.ny;
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    size_t i = ix2(p, x, y), r = ix2(p, xp, y), t = ix2(p, x, yp);
+    const float *uu = f.uu, *vv = f.vv;
+    float wu = wetu_at(p, f, x, y), wv = wetv_at(p, f, x, y);
+    float vu = 0.25f * (vv[ix2(p, x, ym)] + vv[i] + vv[ix2(p, xp, ym)] + vv[r]) * wu;
+    float uv = 0.25f * (uu[ix2(p, xm, y)] + uu[i] + uu[ix2(p, xm, yp)] + uu[t]) * wv;
+    float tmi = f.thetamean[i], tmr = f.thetamean[r], tmt = f.thetamean[t];
+    float si = sinf(tmi), ci = cosf(tmi), sr = sinf(tmr), cr = cosf(tmr), st = sinf(tmt), ctt = cosf(tmt);
+    float usi = f.ust[i], usr = f.ust[r], ust_t = f.ust[t];
+    float vsu = 0.5f * (usi * si + usr * sr) * wu;
+    float usu = 0.5f * (usi * ci + usr * cr) * wu;
+    float ue = uu[i] - usu, ve = vu - vsu;
+    float usv = 0.5f * (usi * ci + ust_t * ctt) * wv;
+    float vsv = 0.5f * (usi * si + ust_t * st) * wv;
+    float vev = vv[i] - vsv, uev = uv - usv;
+    o[0] = uv;
+    o[1] = vu;
+    o[2] = ue;
+    o[3] = vev;
+    o[4] = sqrtf(ue * ue + ve * ve);
+    o[5] = sqrtf(uev * uev + vev * vev);
+}
+
+// calcuvvu + uvlatbnd (Flow_kernel.cu:1712-1775): rows 0 / ny-1 copy rows
+// 1 / ny-2.  Also saves uuold/vvold/zsold.
+__global__ __launch_bounds__(256) void k_flow_uvvu(KP p, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    size_t i = ix2(p, x, y);
+    int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
+    float o[6];
+    calcuvvu_at(p, f, x, ys, o);
+    f.uuold[i] = f.uu[i];
+    f.vvold[i] = f.vv[i];
+    f.zsold[i] = f.zs[i];
+    f.uv[i] = o[0];
+    f.vu[i] = o[1];
+    f.ueu[i] = o[2];
+    f.vev[i] = o[3];
+    f.vmageu[i] = o[4];
+    f.vmagev[i] = o[5];
+}
+
+// ---------------------------------------------------------------------------
+// Sediment transport and bed update
+// ---------------------------------------------------------------------------
+
+// Rvr (Sediment_kernel.cu:477-510) at cell i
+XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
+    const float m1c = 0.0f, m2c = 0.7939f, m3c = -0.6065f, m4c = 0.3539f, m5c = 0.6373f, m6c = 0.5995f;
+    float alph = -1.0f * log10f(expf(1.0f)) / m4c;
+    float bet = expf(m3c / m4c);
+    float kk = (float)(2 * PI_ / (double)(f.c[i] * s.Trep));
+    float Ur = 3.0f / 8.0f * sqrtf(2.0f) * f.H[i] * kk / powf(kk * f.hh[i], 3.0f);
+    Ur = fmx(Ur, 0.00001f);
+    float Bm = m1c + (m2c - m1c) / (1.0f + bet * powf(Ur, alph));
+    float B1 = (float)((double)(-90.0f + 90.0f * tanhf(m5c / powf(Ur, m6c))) * PI_ / 180.0);
+    float Sk = Bm * cosf(B1);
+    float As = Bm * sinf(B1);
+    return (p.facsk * Sk - p.facas * As) * f.urms[i];
+}
+
+// longturb (Sediment_kernel.cu:27-165: kturb is set to 0, rolthick is the
+// live output), Rvr and Susp (534-717).  ceqbg = (1-bulk)*... = 0 with
+// bulk = 1 (Sediment_kernel.cu:196, 320), so the bed-load fluxes Sub/Svb
+// are zero and only Sus/Svs are formed.
+__global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    const int nx = p.nx, ny = p.ny;
+    size_t i = ix2(p, x, y), r = ix2(p, p1(x, nx), y), t = ix2(p, x, p1(y, ny));
+    float rol = f.rolthick[i] + s.dt * (fabsf(f.dzsdt[i]) - p.beta * f.c[i]);
+    f.rolthick[i] = fmx(rol, 0.0f);
+    float uai = rvr_at(p, s, f, i), uar = rvr_at(p, s, f, r), uat = rvr_at(p, s, f, t);
+    float cci = f.C[i], ccr = f.C[r], cct = f.C[t];
+    float tm = f.thetamean[i];
+    float ctm = cosf(tm), stm = sinf(tm);
+    float uau = (float)(0.5 * (double)ctm * (double)(uai + uar));
+    float uav = (float)(0.5 * (double)stm * (double)(uar + uai));
+    float urep = f.ueu[i] + uau;
+    float vmagu = sqrtf(powf((f.uu[i] + uau), 2.0f) + powf((f.vu[i] + uav), 2.0f));
+    uau = (float)(0.5 * (double)ctm * (double)(uai + uat));
+    uav = (float)(0.5 * (double)stm * (double)(uat + uai));
+    float vrep = f.vev[i] + uav;
+    float vmagv = sqrtf(powf(f.uv[i] + uau, 2.0f) + powf(f.vv[i] + uav, 2.0f));
+    float dzbdx = -1.0f * (f.zb[r] - f.zb[i]) / p.dx;
+    float dzbdy = -1.0f * (f.zb[t] - f.zb[i]) / p.dx;
+    float hu = f.hu[i], hv = f.hv[i];
+    float wu = hu > p.eps ? 1.0f : 0.0f, wv = hv > p.eps ? 1.0f : 0.0f;
+    float cu, cv;
+    if (urep > 0.0f) cu = cci;
+    else if (urep < 0.0f) cu = ccr;
+    else cu = 0.50f * (cci + ccr);
+    float dcsdx = (ccr - cci) / p.dx;
+    if (vrep > 0.0f) cv = cci;
+    else if (vrep < 0.0f) cv = cct;
+    else cv = 0.50f * (cci + cct);
+    float dcsdy = (cct - cci) / p.dx;
+    float Dc = p.nuh + p.nuhfac * f.hh[i] * powf(f.DR[i] / p.rho, 1.0f / 3.0f);
+    const float facsl = 1.6f;
+    f.Sus[i] = p.sus * (cu * urep * hu - Dc * hu * dcsdx - facsl * cu * vmagu * hu * dzbdx) * wu;
+    f.Svs[i] = p.sus * (cv * vrep * hv - Dc * hv * dcsdy - facsl * cv * vmagv * hv * dzbdy) * wv;
+}
+
+// Sbvr (Sediment_kernel.cu:167-326) + Erosus (512-531) + Conc (721-779) at
+// (x, y): new concentration and bed change ero - depo.
+XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& Cnew, float& dzg, float& ceqs_out) {
+    const int nx = p.nx, ny = p.ny;
+    size_t i = ix2(p, x, y);
+    const float zo = 0.006f, sedcal = 1.0f, bulk = 1.0f;
+    float h = f.hh[i];
+    float hhi = fmx(h, 0.01f);
+    float drho = (p.rhosed - p.rho) / p.rho;
+    float dester = powf(drho * p.g, 1.0f / 3.0f) / 0.0001f * p.D50;
+    float ue = (float)(0.5 * (double)(f.ueu[ix2(p, m1(x), y)] + f.ueu[i]));
+    float ve = (float)(0.5 * (double)(f.vev[ix2(p, x, m1(y))] + f.vev[i]));
+    float vmags = ue * ue + ve * ve;
+    float ML = fmx(fmn(sqrtf(2.0f * f.R[i] * s.Trep / (p.rho * f.c[i])), hhi), 0.01f);
+    float dcfin = expf(fmn(hhi, 100.0f) / ML);
+    float dcf = fmn(1 / (dcfin - 1), 1.0f);
+    float kb = p.nuhfac * powf(f.DR[i] / p.rho, 0.66666666667f) * dcf;
+    float Urms2 = f.urms[i] * f.urms[i] + 1.45f * (kb + 0.0f);
+    float Tsg = fmx(0.1f * hhi / p.wws, 0.5f);
+    float Ucr;
+    if (p.D50 <= 0.0005f) Ucr = 0.19f * powf(p.D50, 0.1f) * log10f(4.0f * hhi / p.D90);
+    else Ucr = 8.50f * powf(p.D50, 0.6f) * log10f(4.0f * hhi / p.D90);
+    float hdrag = fmx(hhi, 10.0f * zo);
+    float Cdd = 0.4f / (logf(hdrag / zo) - 1.0f);
+    Cdd = Cdd * Cdd;
+    float Asb = 0.005f * hhi * powf(1 / hhi / (drho * p.g), 1.2f);
+    float Ass = 0.012f * p.D50 * powf(dester, -0.6f) / (powf(drho * p.g * p.D50, 1.2f));
+    float cfm = cfm_of(p, f.stdep[i]);
+    float T1 = vmags + 0.018f / Cdd * Urms2;
+    T1 = fmn(T1, 100000.0f * p.g / cfm * p.D50 * drho);
+    T1 = sqrtf(T1);
+    float wetz = hhi > p.eps ? 1.0f : 0.0f;
+    float T2 = 0.0f;
+    if (T1 > Ucr && hhi > p.eps) T2 = powf(T1 - Ucr, 2.4f);
+    float ceqb = Asb * T2;
+    ceqb = fmn(ceqb / hhi, 0.05f);
+    float ceqbg = (1 - bulk) * ceqb * sedcal * wetz;
+    float ceqs = fmn(Ass * T2 / hhi, 0.05f);
+    float ceqsg = (ceqs + bulk * ceqb) * sedcal * wetz;
+    ceqs_out = ceqsg;
+    // Erosus
+    float exp_ero = p.morfac * s.dt / (1.0f - p.por) * h * (ceqsg * 1.0f / Tsg + ceqbg * 1.0f / s.dt);
+    float facero = fmn(1.0f, f.stdep[i] * 1.0f / fmx(0.000001f, exp_ero));
+    // Conc
+    float wz = h > p.eps ? 1.0f : 0.0f;
+    float ero = facero * h * ceqsg * 1.0f / Tsg;
+    float dsusdx = (f.Sus[i] - f.Sus[ix2(p, m1(x), y)]) / p.dx;
+    float dsvsdy = (f.Svs[i] - f.Svs[ix2(p, x, m1(y))]) / p.dx;
+    float cs = (s.dt * Tsg) / (s.dt + Tsg) * (h * f.C[i] / s.dt - (dsusdx + dsvsdy - ero)) * wz;
+    cs = fmx(cs, 0.0f);
+    cs = fmn(cs, 0.1f * h);
+    float depo = cs / Tsg;
+    Cnew = cs / h;
+    // bedupdate (Sediment_kernel.cu:881-964)
+    dzg = p.morfac * s.dt / (1.0f - p.por) * (ero - depo);
+    (void)nx;
+    (void)ny;
+}
+
+// zblatbnd source cell (Sediment_kernel.cu:970-1018): ix==0 wins.
+XDEV void zsrc(const KP& p, int x, int y, int& sx, int& sy) {
+    sx = x;
+    sy = y;
+    if (y == 0) sy = p1(0, p.ny);
+    if (y == p.ny - 1) sy = m1(p.ny - 1);
+    if (x == 0) {
+        sx = p1(0, p.nx);
+        sy = y;
+    }
+}
+
+// Conc + CClatbnd (Sediment_kernel.cu:782-822) + bedupdate + zblatbnd(zb),
+// zblatbnd(stdep).  Writes the *_b buffers.
+__global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    const int ny = p.ny;
+    size_t i = ix2(p, x, y);
+    float Cn, dzg, ceq;
+    conc_at(p, s, f, x, y, Cn, dzg, ceq);
+    f.ceqsg[i] = ceq;
+    float Cfin;
+    int cy = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
+    if (x == 0) {
+        Cfin = 0.0f;
+    } else if (cy != y) {
+        float d2, c2;
+        conc_at(p, s, f, x, cy, Cfin, d2, c2);
+    } else {
+        Cfin = Cn;
+    }
+    f.C_out[i] = Cfin;
+    if (!morph) return;
+    int sx, sy;
+    zsrc(p, x, y, sx, sy);
+    size_t is = ix2(p, sx, sy);
+    float dzs = dzg;
+    if (is != i) {
+        float c2, c3;
+        conc_at(p, s, f, sx, sy, c2, dzs, c3);
+    }
+    f.hh_b[i] = f.hh[i] + dzg;
+    f.dzb[i] = -1 * dzg;
+    f.stdep_b[i] = f.stdep[is] - dzs;
+    if (p.zb_accum == 2) {
+        double z = f.zb64[is] + (double)dzs;
+        f.zb64_b[i] = z;
+        f.zb_b[i] = (float)z;
+    } else if (p.zb_accum == 1) {
+        float zb0 = f.zb[is], c0 = f.zbc[is];
+        float yk = dzs - c0;
+        float tk = zb0 + yk;
+        f.zbc_b[i] = (tk - zb0) - yk;
+        f.zb_b[i] = tk;
+    } else {
+        f.zb_b[i] = f.zb[is] + dzs;
+    }
+}
+
+// avalanching (Sediment_kernel.cu:1020-1137): slope-limited exchange with the
+// +x and +y neighbours, stored per cell and gathered by k_sed_update.
+__global__ __launch_bounds__(256) void k_sed_aval(KP p, SP s, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
+    const float *zb = f.zb_b, *sd = f.stdep_b;
+    float dzmax = f.hh_b[i] > p.eps ? p.wetdzmax : p.drydzmax;
+    float maxchg = p.maxslpchg, dt = s.dt, dx = p.dx;
+    float bx = 0.0f, by = 0.0f;
+    float sxv = (zb[r] - zb[i]) / dx;
+    if (fabsf(sxv) > dzmax) {
+        float sg = sxv / fabsf(sxv);
+        bx = sg * (fabsf(sxv) - dzmax) * dx;
+        if (sg > 0) {
+            bx = fmn(bx, maxchg * dt / dx);
+            bx = fmn(bx, sd[i]);
+        } else {
+            bx = fmx(bx, -1.0f * maxchg * dt / dx);
+            bx = fmx(bx, -1.0f * sd[r]);
+        }
+    }
+    float syv = (zb[t] - zb[i]) / dx;
+    if (fabsf(syv) > dzmax) {
+        float sg = syv / fabsf(syv);
+        by = sg * (fabsf(syv) - dzmax) * dx;
+        if (sg > 0) {
+            by = fmn(by, maxchg * dt / dx);
+            by = fmn(by, sd[i]);
+        } else {
+            by = fmx(by, -1 * maxchg * dt / dx);
+            by = fmx(by, -1 * sd[t]);
+        }
+    }
+    f.dzbx[i] = bx;
+    f.dzby[i] = by;
+}
+
+XDEV float ddzb_at(const KP& p, const Fld& f, int x, int y) {
+    size_t i = ix2(p, x, y);
+    float d = f.dzbx[i] + f.dzby[i];
+    if (x > 0) d = d - f.dzbx[i - 1];
+    if (y > 0) d = d - f.dzby[ix2(p, x, y - 1)];
+    return d;
+}
+
+// updatezb (Sediment_kernel.cu:1139-1159) + zblatbnd(zb), zblatbnd(stdep).
+__global__ __launch_bounds__(256) void k_sed_update(KP p, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    size_t i = ix2(p, x, y);
+    float d = ddzb_at(p, f, x, y);
+    int sx, sy;
+    zsrc(p, x, y, sx, sy);
+    size_t is = ix2(p, sx, sy);
+    float ds = (is == i) ? d : ddzb_at(p, f, sx, sy);
+    f.dzb[i] = f.dzb[i] - d;
+    f.zs[i] = f.zs[i] - d;
+    f.hh[i] = f.hh_b[i] - d;
+    f.stdep[i] = f.stdep_b[is] - ds;
+    if (p.zb_accum == 2) {
+        double z = f.zb64_b[is] + (double)ds;
+        f.zb64[i] = z;
+        f.zb[i] = (float)z;
+    } else if (p.zb_accum == 1) {
+        float zb0 = f.zb_b[is], c0 = f.zbc_b[is];
+        float yk = ds - c0;
+        float tk = zb0 + yk;
+        f.zbc[i] = (tk - zb0) - yk;
+        f.zb[i] = tk;
+    } else {
+        f.zb[i] = f.zb_b[is] + ds;
+    }
+}
+
+// Copy the bed-update products back when morphology is off (C only).
+// ---------------------------------------------------------------------------
+// Init, reductions, output averages
+// ---------------------------------------------------------------------------
+
+// depthhu / depthhv on the initial state (Wave_gpu.cu init order).
+__global__ __launch_bounds__(256) void k_init_depth(KP p, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
+    const float *zs = f.zs, *zb = f.zb;
+    float u = f.uu[i], v = f.vv[i];
+    float hb = fmx(-1.0f * zb[i], -1.0f * zb[r]);
+    float h;
+    if (u > 0.0f) h = zs[i] - hb;
+    else if (u < -0.0f) h = zs[r] - hb;
+    else h = fmx(fmx(zs[i], zs[r]) - hb, p.eps);
+    f.hu[i] = fmx(h, 0.0f);
+    hb = fmx(-1.0f * zb[i], -1.0f * zb[t]);
+    if (v > 0.0f) h = zs[i] - hb;
+    else if (v < -0.0f) h = zs[t] - hb;
+    else h = fmx(fmx(zs[i], zs[t]) - hb, p.eps);
+    f.hv[i] = fmx(h, 0.0f);
+    if (p.zb_accum == 2) f.zb64[i] = (double)zb[i];
+}
+
+// FLOWDT (Wave_kernel.cu:161-170) block minima; non-finite values raise a flag.
+__global__ __launch_bounds__(256) void k_cfl_flow(KP p, Fld f, float cfl) {
+    __shared__ float red[256];
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    float v = 3.0e38f;
+    if (i < p.N) {
+        float d = cfl * p.dx / (sqrtf(9.81f * f.hh[i]));
+        if (!(d == d) || d == INFINITY || d == 0.0f) atomicOr(f.red_bad, 1);
+        v = d;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] = fmn(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) f.red_flow[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_cfl_final(Fld f, int nflow, int nwave) {
+    __shared__ float a[256], b[256];
+    float fm = 3.0e38f, wm = 9999999.9f;
+    for (int j = threadIdx.x; j < nflow; j += 256) fm = fmn(fm, f.red_flow[j]);
+    for (int j = threadIdx.x; j < nwave; j += 256) wm = fmn(wm, f.red_wave[j]);
+    a[threadIdx.x] = fm;
+    b[threadIdx.x] = wm;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            a[threadIdx.x] = fmn(a[threadIdx.x], a[threadIdx.x + st]);
+            b[threadIdx.x] = fmn(b[threadIdx.x], b[threadIdx.x + st]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        f.red_out[0] = a[0];
+        f.red_out[1] = b[0];
+        f.red_out[2] = (float)(*f.red_bad);
+        *f.red_bad = 0;
+    }
+}
+
+// addavg_var / max_var (Wave_kernel.cu:85-159) for the output means.
+__global__ __launch_bounds__(256) void k_means(KP p, Fld f, float* Hm, float* um, float* vm, float* hm, float* zm,
+                                               float* zmax, float* Cm) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.N) return;
+    Hm[i] = Hm[i] + f.H[i];
+    um[i] = um[i] + f.uu[i];
+    vm[i] = vm[i] + f.vv[i];
+    hm[i] = hm[i] + f.hh[i];
+    zm[i] = zm[i] + f.zs[i];
+    zmax[i] = fmx(zmax[i], f.zs[i]);
+    Cm[i] = Cm[i] + f.C[i];
+}
+
+__global__ __launch_bounds__(256) void k_scale(float* a, size_t n, float div) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = a[i] / div;
+}
+
+__global__ __launch_bounds__(256) void k_fill(float* a, size_t n, float v) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// FP64 compensated block sums of hh and zb for the mass balance.
+__global__ __launch_bounds__(256) void k_sum2(const float* a, const float* b, size_t n, double* out) {
+    __shared__ double sa[256], sb[256];
+    double xa = 0.0, xb = 0.0, ca = 0.0, cb = 0.0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        double y = (double)a[i] - ca;
+        double t = xa + y;
+        ca = (t - xa) - y;
+        xa = t;
+        y = (double)b[i] - cb;
+        t = xb + y;
+        cb = (t - xb) - y;
+        xb = t;
+    }
+    sa[threadIdx.x] = xa;
+    sb[threadIdx.x] = xb;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            sa[threadIdx.x] += sa[threadIdx.x + st];
+            sb[threadIdx.x] += sb[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sa[0];
+        out[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Context and C-ABI
+// ===========================================================================
+
+namespace {
+thread_local std::string g_create_error;
+
+struct FieldDesc {
+    float** slot;   // device pointer slot inside the context
+    size_t count;
+};
+}  // namespace
+
+struct xbg_ctx {
+    xbg_params P;
+    KP kp;
+    hipStream_t stream = nullptr;
+    size_t N = 0, ND = 0;
+    std::vector<void*> allocs;
+    Fld f;
+    float* ee[2] = {nullptr, nullptr};
+    float* rr[2] = {nullptr, nullptr};
+    float* Cb[2] = {nullptr, nullptr};
+    int cur = 0, ccur = 0;
+    float* St = nullptr;
+    float* qbo = nullptr;
+    float* qbn = nullptr;
+    float* Stfile = nullptr;
+    float* qfile = nullptr;
+    int nrec = 0;
+    float *Hmean = nullptr, *uumean = nullptr, *vvmean = nullptr, *hhmean = nullptr, *zsmean = nullptr,
+          *zsmax = nullptr, *Cmean = nullptr;
+    float Trep = 10.0f, tsl = 0.0f, tnext = 1.0f, zsbndi = 0.0f, zsbndn = 0.0f, windu = 0.0f, windv = 0.0f;
+    int nblk_wave = 0, nblk_flow = 0, nblk_sum = 0;
+    float* host_red = nullptr;
+    double* host_sum = nullptr;
+    std::string err;
+    std::map<std::string, FieldDesc> fields;
+    // profiling
+    bool prof = false;
+    struct Ev {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> evpool;
+    size_t evused = 0;
+    std::map<std::string, std::pair<long, double>> prof_acc;
+};
+
+namespace {
+
+int fail(xbg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPC(ctx, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ctx, XBG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+dim3 grid2(const KP& p) { return dim3((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY, 1); }
+const dim3 blk2(BX, BY, 1);
+
+void prof_begin(xbg_ctx* c, const char* name) {
+    if (!c->prof) return;
+    if (c->evused == c->evpool.size()) {
+        xbg_ctx::Ev e;
+        e.name = name;
+        hipEventCreate(&e.a);
+        hipEventCreate(&e.b);
+        c->evpool.push_back(e);
+    }
+    c->evpool[c->evused].name = name;
+    hipEventRecord(c->evpool[c->evused].a, c->stream);
+}
+void prof_end(xbg_ctx* c) {
+    if (!c->prof) return;
+    hipEventRecord(c->evpool[c->evused].b, c->stream);
+    c->evused++;
+}
+void prof_collect(xbg_ctx* c) {
+    if (!c->prof || c->evused == 0) return;
+    hipStreamSynchronize(c->stream);
+    for (size_t j = 0; j < c->evused; j++) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, c->evpool[j].a, c->evpool[j].b);
+        auto& acc = c->prof_acc[c->evpool[j].name];
+        acc.first += 1;
+        acc.second += ms;
+    }
+    c->evused = 0;
+}
+
+#define LAUNCH(ctx, name, kern, grid, block, ...)                              \
+    do {                                                                       \
+        prof_begin(ctx, name);                                                 \
+        kern<<<grid, block, 0, (ctx)->stream>>>(__VA_ARGS__);                  \
+        prof_end(ctx);                                                         \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess)                                                  \
+            return fail(ctx, XBG_E_HIP, std::string(name) + ": " + hipGetErrorString(e_)); \
+        if ((ctx)->prof && (ctx)->evused >= 4096) prof_collect(ctx);           \
+    } while (0)
+
+template <class T>
+int dalloc(xbg_ctx* c, T** p, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T) > 0 ? n * sizeof(T) : 16);
+    if (e != hipSuccess) return fail(c, XBG_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    hipMemset(q, 0, n * sizeof(T));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+void make_kp(xbg_ctx* c) {
+    const xbg_params& P = c->P;
+    KP& k = c->kp;
+    std::memset(&k, 0, sizeof(k));
+    k.nx = P.nx; k.ny = P.ny; k.nt = P.ntheta; k.N = (uint32_t)c->N;
+    k.dx = (float)P.dx; k.g = (float)P.g; k.rho = (float)P.rho; k.eps = (float)P.eps;
+    k.nuh = (float)P.nuh; k.nuhfac = (float)P.nuhfac; k.fc = (float)P.fc; k.Cd = (float)P.Cd;
+    k.wci = (float)P.wci; k.hwci = (float)P.hwci; k.gammaa = (float)P.gammaa; k.nexp = (float)P.n;
+    k.alpha = (float)P.alpha; k.gammax = (float)P.gammax; k.beta = (float)P.beta;
+    k.dtheta = (float)P.dtheta; k.D50 = (float)P.D50; k.D90 = (float)P.D90; k.rhosed = (float)P.rhosed;
+    k.wws = (float)P.wws; k.drydzmax = (float)P.drydzmax; k.wetdzmax = (float)P.wetdzmax;
+    k.maxslpchg = (float)P.maxslpchg; k.por = (float)P.por; k.morfac = (float)P.morfac;
+    k.sus = (float)P.sus; k.bed = (float)P.bed; k.facsk = (float)P.facsk; k.facas = (float)P.facas;
+    k.cfsand = (float)P.cfsand; k.cfreef = (float)P.cfreef; k.fwsand = (float)P.fwsand;
+    k.fwreef = (float)P.fwreef; k.CFL = (float)P.CFL;
+    // Wave_gpu.cu:95-98
+    float phi = (1.0f + sqrtf(5.0f)) / 2;
+    k.aphi = 1 / (phi + 1);
+    k.bphi = phi / (phi + 1);
+    k.twopi = 8 * atanf(1.0f);
+    k.usesmago = P.usesmago; k.breakmodel = P.breakmodel; k.roller = P.roller;
+    k.zb_accum = P.zb_accum; k.swave = P.swave;
+    xbg_theta_tables(&P, k.theta, k.cxsth, k.sxnth);
+}
+
+SP make_sp(xbg_ctx* c, double dt, double totaltime) {
+    SP s;
+    s.dt = (float)dt;
+    s.totaltime = (float)totaltime;
+    s.Trep = c->Trep;
+    s.sigm = (float)(2 * PI_ / (double)c->Trep);
+    // wave_CPU.cpp:29: taper = min(totaltime/100.0, 1.0) with DECNUM totaltime
+    double tp = (double)s.totaltime / 100.0;
+    s.taper = (float)(tp < 1.0 ? tp : 1.0);
+    s.tsl = c->tsl; s.tnext = c->tnext; s.zsbndi = c->zsbndi; s.zsbndn = c->zsbndn;
+    s.windu = c->windu; s.windv = c->windv;
+    s.signwu = ((double)c->windu > 0.0) - ((double)c->windu < 0.0);
+    s.signwv = ((double)c->windv > 0.0) - ((double)c->windv < 0.0);
+    return s;
+}
+
+void sync_ptrs(xbg_ctx* c) {
+    c->f.ee_in = c->ee[c->cur];
+    c->f.ee_out = c->ee[1 - c->cur];
+    c->f.rr_in = c->rr[c->cur];
+    c->f.rr_out = c->rr[1 - c->cur];
+    c->f.C = c->Cb[c->ccur];
+    c->f.C_out = c->Cb[1 - c->ccur];
+    c->f.St = c->St;
+    c->f.qbo = c->qbo;
+    c->f.qbn = c->qbn;
+}
+
+void register_fields(xbg_ctx* c) {
+    Fld& f = c->f;
+    size_t N = c->N, ND = c->ND;
+    auto R = [&](const char* n, float** slot, size_t cnt) { c->fields[n] = FieldDesc{slot, cnt}; };
+    R("ee", &c->ee[0], ND);  // resolved through cur
+    R("rr", &c->rr[0], ND);
+    R("C", &c->Cb[0], N);
+    R("zs", &f.zs, N); R("uu", &f.uu, N); R("vv", &f.vv, N); R("hh", &f.hh, N); R("zb", &f.zb, N);
+    R("stdep", &f.stdep, N); R("hu", &f.hu, N); R("hv", &f.hv, N); R("ueu", &f.ueu, N);
+    R("vev", &f.vev, N); R("vmageu", &f.vmageu, N); R("vmagev", &f.vmagev, N); R("vu", &f.vu, N);
+    R("uv", &f.uv, N); R("qx", &f.qx, N); R("qy", &f.qy, N); R("uuold", &f.uuold, N);
+    R("vvold", &f.vvold, N); R("zsold", &f.zsold, N); R("dzsdt", &f.dzsdt, N);
+    R("rolthick", &f.rolthick, N); R("dzb", &f.dzb, N); R("k", &f.k, N); R("c", &f.c, N);
+    R("cg", &f.cg, N); R("H", &f.H, N); R("E", &f.E, N); R("DR", &f.DR, N); R("R", &f.R, N);
+    R("urms", &f.urms, N); R("ust", &f.ust, N); R("thetamean", &f.thetamean, N); R("Sxx", &f.Sxx, N);
+    R("Sxy", &f.Sxy, N); R("Syy", &f.Syy, N); R("Fx", &f.Fx, N); R("Fy", &f.Fy, N); R("nuh", &f.nuh, N);
+    R("Sus", &f.Sus, N); R("Svs", &f.Svs, N); R("ceqsg", &f.ceqsg, N);
+    R("umeanbnd", &f.umean, (size_t)c->P.ny);
+    R("Hmean", &c->Hmean, N); R("uumean", &c->uumean, N); R("vvmean", &c->vvmean, N);
+    R("hhmean", &c->hhmean, N); R("zsmean", &c->zsmean, N); R("zsmax", &c->zsmax, N);
+    R("Cmean", &c->Cmean, N);
+}
+
+float* resolve(xbg_ctx* c, const std::string& name, size_t* cnt) {
+    auto it = c->fields.find(name);
+    if (it == c->fields.end()) return nullptr;
+    *cnt = it->second.count;
+    if (name == "ee") return c->ee[c->cur];
+    if (name == "rr") return c->rr[c->cur];
+    if (name == "C") return c->Cb[c->ccur];
+    return *it->second.slot;
+}
+
+const char kFieldNames[] =
+    "ee\0rr\0C\0zs\0uu\0vv\0hh\0zb\0stdep\0hu\0hv\0ueu\0vev\0vmageu\0vmagev\0vu\0uv\0qx\0qy\0uuold\0"
+    "vvold\0zsold\0dzsdt\0rolthick\0dzb\0k\0c\0cg\0H\0E\0DR\0R\0urms\0ust\0thetamean\0Sxx\0Sxy\0Syy\0"
+    "Fx\0Fy\0nuh\0Sus\0Svs\0ceqsg\0umeanbnd\0Hmean\0uumean\0vvmean\0hhmean\0zsmean\0zsmax\0Cmean\0";
+
+int launch_wave(xbg_ctx* c, const SP& s) {
+    KP& p = c->kp;
+    dim3 g = grid2(p);
+    LAUNCH(c, "k_dispersion", k_dispersion, g, blk2, p, s, c->f);
+    switch (p.nt) {
+        case 12: LAUNCH(c, "k_wave", k_wave<12>, g, blk2, p, s, c->f); break;
+        case 24: LAUNCH(c, "k_wave", k_wave<24>, g, blk2, p, s, c->f); break;
+        default: LAUNCH(c, "k_wave", k_wave<0>, g, blk2, p, s, c->f); break;
+    }
+    LAUNCH(c, "k_wave_force", k_wave_force, g, blk2, p, s, c->f);
+    c->cur = 1 - c->cur;
+    sync_ptrs(c);
+    return 0;
+}
+
+int launch_flow(xbg_ctx* c, const SP& s) {
+    KP& p = c->kp;
+    dim3 g = grid2(p);
+    LAUNCH(c, "k_flow_nuh", k_flow_nuh, g, blk2, p, c->f);
+    LAUNCH(c, "k_flow_u", k_flow_u, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_flow_v", k_flow_v, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_flow_wrk", k_flow_wrk, g, blk2, p, c->f);
+    LAUNCH(c, "k_flow_corr", k_flow_corr, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_flow_cont", k_flow_cont, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_flow_wrkzs", k_flow_wrkzs, g, blk2, p, c->f);
+    LAUNCH(c, "k_flow_zs", k_flow_zs, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_flow_uvvu", k_flow_uvvu, g, blk2, p, c->f);
+    return 0;
+}
+
+int launch_sed(xbg_ctx* c, const SP& s) {
+    KP& p = c->kp;
+    dim3 g = grid2(p);
+    int morph = p.morfac > 0.0f ? 1 : 0;
+    LAUNCH(c, "k_sed_flux", k_sed_flux, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_sed_conc", k_sed_conc, g, blk2, p, s, c->f, morph);
+    c->ccur = 1 - c->ccur;
+    sync_ptrs(c);
+    if (morph) {
+        LAUNCH(c, "k_sed_aval", k_sed_aval, g, blk2, p, s, c->f);
+        LAUNCH(c, "k_sed_update", k_sed_update, g, blk2, p, c->f);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xbg_last_error(const xbg_ctx* ctx) {
+    if (!ctx) return g_create_error.c_str();
+    return ctx->err.c_str();
+}
+
+const char* xbg_field_names(void) { return kFieldNames; }
+
+int xbg_create(const xbg_params* pin, xbg_ctx** out) {
+    if (!pin || !out) return fail(nullptr, XBG_E_ARG, "xbg_create: NULL argument");
+    *out = nullptr;
+    xbg_params P = *pin;
+    if (P.nx < 4 || P.ny < 4) return fail(nullptr, XBG_E_ARG, "xbg_create: nx and ny must be >= 4");
+    if (P.ntheta < 1 || P.ntheta > MAXT)
+        return fail(nullptr, XBG_E_ARG, "xbg_create: ntheta must be in [1, 64]");
+    if ((double)P.nx * P.ny * P.ntheta >= 2147483647.0)
+        return fail(nullptr, XBG_E_ARG, "xbg_create: nx*ny*ntheta must be < 2^31");
+    if (P.zb_accum < 0 || P.zb_accum > 2) return fail(nullptr, XBG_E_ARG, "xbg_create: zb_accum must be 0, 1 or 2");
+    xbg_ctx* c = new xbg_ctx();
+    c->P = P;
+    c->N = (size_t)P.nx * P.ny;
+    c->ND = c->N * P.ntheta;
+    make_kp(c);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete c;
+        return XBG_E_HIP;
+    }
+    Fld& f = c->f;
+    size_t N = c->N, ND = c->ND, ny = P.ny;
+    int rc = 0;
+#define A(ptr, n) if ((rc = dalloc(c, ptr, n)) != 0) { g_create_error = c->err; xbg_destroy(c); return rc; }
+    A(&c->ee[0], ND); A(&c->ee[1], ND); A(&c->rr[0], ND); A(&c->rr[1], ND);
+    A(&c->Cb[0], N); A(&c->Cb[1], N);
+    A(&c->St, ny * P.ntheta); A(&c->qbo, 4 * ny); A(&c->qbn, 4 * ny);
+    float** two[] = {&f.zs, &f.uu, &f.vv, &f.hh, &f.zb, &f.stdep, &f.hu, &f.hv, &f.ueu, &f.vev, &f.vmageu,
+                     &f.vmagev, &f.vu, &f.uv, &f.qx, &f.qy, &f.uuold, &f.vvold, &f.zsold, &f.dzsdt,
+                     &f.rolthick, &f.dzb, &f.k, &f.c, &f.cg, &f.H, &f.E, &f.Dw, &f.DR, &f.R, &f.urms,
+                     &f.ust_raw, &f.ust, &f.thetamean, &f.Sxx, &f.Sxy, &f.Syy, &f.Fx, &f.Fy, &f.nuh,
+                     &f.uu_b, &f.vv_b, &f.w1u, &f.w2u, &f.w1v, &f.w2v, &f.zsn, &f.w1z, &f.w2z, &f.Sus,
+                     &f.Svs, &f.hh_b, &f.zb_b, &f.stdep_b, &f.dzbx, &f.dzby, &f.ceqsg, &f.zbc, &f.zbc_b,
+                     &c->Hmean, &c->uumean, &c->vvmean, &c->hhmean, &c->zsmean, &c->zsmax, &c->Cmean};
+    for (auto** q : two) { A(q, N); }
+    A(&f.umean, ny);
+    A(&f.wet, N);
+    if (P.zb_accum == 2) { A(&f.zb64, N); A(&f.zb64_b, N); }
+    c->nblk_wave = ((P.nx + BX - 1) / BX) * ((P.ny + BY - 1) / BY);
+    c->nblk_flow = (int)((N + 255) / 256);
+    c->nblk_sum = 256;
+    A(&f.red_wave, c->nblk_wave); A(&f.red_flow, c->nblk_flow); A(&f.red_bad, 4); A(&f.red_out, 4);
+    A(&f.red_sum, 2 * c->nblk_sum);
+#undef A
+    e = hipHostMalloc((void**)&c->host_red, 4 * sizeof(float), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_sum, 2 * c->nblk_sum * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipHostMalloc: ") + hipGetErrorString(e);
+        xbg_destroy(c);
+        return XBG_E_HIP;
+    }
+    sync_ptrs(c);
+    register_fields(c);
+    // sigm-free initial wave dt partials
+    hipDeviceSynchronize();
+    *out = c;
+    return XBG_OK;
+}
+
+int xbg_destroy(xbg_ctx* c) {
+    if (!c) return XBG_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void* q : c->allocs) hipFree(q);
+    for (auto& e : c->evpool) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    if (c->host_red) hipHostFree(c->host_red);
+    if (c->host_sum) hipHostFree(c->host_sum);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return XBG_OK;
+}
+
+size_t xbg_field_size(const xbg_ctx* c, const char* name) {
+    if (!c || !name) return 0;
+    auto it = c->fields.find(name);
+    return it == c->fields.end() ? 0 : it->second.count;
+}
+
+int xbg_set_field(xbg_ctx* c, const char* name, const float* host, size_t n) {
+    if (!c || !name || !host) return fail(c, XBG_E_ARG, "xbg_set_field: NULL argument");
+    size_t cnt = 0;
+    float* d = resolve(c, name, &cnt);
+    if (!d) return fail(c, XBG_E_ARG, std::string("xbg_set_field: unknown field ") + name);
+    if (n != cnt) return fail(c, XBG_E_ARG, std::string("xbg_set_field: size mismatch for ") + name);
+    HIPC(c, hipMemcpyAsync(d, host, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return XBG_OK;
+}
+
+int xbg_get_field(xbg_ctx* c, const char* name, float* host, size_t n) {
+    if (!c || !name || !host) return fail(c, XBG_E_ARG, "xbg_get_field: NULL argument");
+    size_t cnt = 0;
+    float* d = resolve(c, name, &cnt);
+    if (!d) return fail(c, XBG_E_ARG, std::string("xbg_get_field: unknown field ") + name);
+    if (n != cnt) return fail(c, XBG_E_ARG, std::string("xbg_get_field: size mismatch for ") + name);
+    HIPC(c, hipMemcpyAsync(host, d, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return XBG_OK;
+}
+
+int xbg_init_state(xbg_ctx* c, double Trep) {
+    if (!c) return XBG_E_ARG;
+    c->Trep = (float)Trep;
+    SP s = make_sp(c, 0.0, 0.0);
+    (void)s;
+    LAUNCH(c, "k_init_depth", k_init_depth, grid2(c->kp), blk2, c->kp, c->f);
+    // the first step's wave CFL minimum is not used (totaltime == 0)
+    dim3 g1((c->nblk_wave + 255) / 256);
+    LAUNCH(c, "k_fill", k_fill, g1, dim3(256), c->f.red_wave, (size_t)c->nblk_wave, 9999999.9f);
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return XBG_OK;
+}
+
+int xbg_set_wave_forcing(xbg_ctx* c, const float* St, double Trep) {
+    if (!c || !St) return fail(c, XBG_E_ARG, "xbg_set_wave_forcing: NULL argument");
+    HIPC(c, hipMemcpyAsync(c->St, St, (size_t)c->P.ny * c->P.ntheta * sizeof(float), hipMemcpyHostToDevice,
+                           c->stream));
+    c->Trep = (float)Trep;
+    return XBG_OK;
+}
+
+int xbg_set_flow_forcing(xbg_ctx* c, const float* qbndold, const float* qbndnew, double timesincelast,
+                         double timenext, double zsbndi, double zsbndn, double windu, double windv) {
+    if (!c) return XBG_E_ARG;
+    size_t n = 4 * (size_t)c->P.ny * sizeof(float);
+    if (qbndold) HIPC(c, hipMemcpyAsync(c->qbo, qbndold, n, hipMemcpyHostToDevice, c->stream));
+    if (qbndnew) HIPC(c, hipMemcpyAsync(c->qbn, qbndnew, n, hipMemcpyHostToDevice, c->stream));
+    c->tsl = (float)timesincelast;
+    c->tnext = (float)timenext;
+    c->zsbndi = (float)zsbndi;
+    c->zsbndn = (float)zsbndn;
+    c->windu = (float)windu;
+    c->windv = (float)windv;
+    return XBG_OK;
+}
+
+int xbg_set_bnd_series(xbg_ctx* c, const float* Stfile, const float* qfile, int nrec) {
+    if (!c || !Stfile || !qfile || nrec < 2) return fail(c, XBG_E_ARG, "xbg_set_bnd_series: bad argument");
+    size_t ns = (size_t)nrec * c->P.ny * c->P.ntheta, nq = (size_t)nrec * 4 * c->P.ny;
+    if (c->Stfile) hipFree(c->Stfile);
+    if (c->qfile) hipFree(c->qfile);
+    c->Stfile = nullptr;
+    c->qfile = nullptr;
+    HIPC(c, hipMalloc((void**)&c->Stfile, ns * sizeof(float)));
+    HIPC(c, hipMalloc((void**)&c->qfile, nq * sizeof(float)));
+    HIPC(c, hipMemcpy(c->Stfile, Stfile, ns * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->qfile, qfile, nq * sizeof(float), hipMemcpyHostToDevice));
+    c->nrec = nrec;
+    return XBG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// St = interptime(Stnew, Stold, timenext, timesincelast) in double
+// (Wavestep.cu:740-745 with tools.cpp:22-25) from the resident records.
+__global__ __launch_bounds__(256) void k_bnd_interp(const float* Stfile, const float* qfile, int rec, int nst,
+                                                    int nq, double tsl, double tnext, float* St, float* qo,
+                                                    float* qn) {
+    int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < nst) {
+        double so = (double)Stfile[(size_t)rec * nst + j];
+        double sn = (double)Stfile[(size_t)(rec + 1) * nst + j];
+        St[j] = (float)(so + (tsl) / (tnext) * (sn - so));
+    }
+    if (j < nq) {
+        qo[j] = qfile[(size_t)rec * nq + j];
+        qn[j] = qfile[(size_t)(rec + 1) * nq + j];
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int xbg_use_bnd_record(xbg_ctx* c, int rec, double timesincelast, double timenext, double Trep, double zsbndi,
+                       double zsbndn, double windu, double windv) {
+    if (!c) return XBG_E_ARG;
+    if (!c->Stfile || rec < 0 || rec + 1 >= c->nrec)
+        return fail(c, XBG_E_STATE, "xbg_use_bnd_record: record out of range or no series");
+    int nst = c->P.ny * c->P.ntheta, nq = 4 * c->P.ny;
+    int n = nst > nq ? nst : nq;
+    LAUNCH(c, "k_bnd_interp", k_bnd_interp, dim3((n + 255) / 256), dim3(256), c->Stfile, c->qfile, rec, nst, nq,
+           timesincelast, timenext, c->St, c->qbo, c->qbn);
+    c->Trep = (float)Trep;
+    c->tsl = (float)timesincelast;
+    c->tnext = (float)timenext;
+    c->zsbndi = (float)zsbndi;
+    c->zsbndn = (float)zsbndn;
+    c->windu = (float)windu;
+    c->windv = (float)windv;
+    return XBG_OK;
+}
+
+int xbg_flow_bnd(xbg_ctx* c, double dt, double totaltime) {
+    if (!c) return XBG_E_ARG;
+    SP s = make_sp(c, dt, totaltime);
+    LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(256), c->kp, s, c->f, c->P.swave == 1 ? 1 : 0);
+    return XBG_OK;
+}
+
+int xbg_wave_step(xbg_ctx* c, double dt, double totaltime) {
+    if (!c) return XBG_E_ARG;
+    return launch_wave(c, make_sp(c, dt, totaltime));
+}
+
+int xbg_flow_step(xbg_ctx* c, double dt) {
+    if (!c) return XBG_E_ARG;
+    return launch_flow(c, make_sp(c, dt, 0.0));
+}
+
+int xbg_sed_step(xbg_ctx* c, double dt) {
+    if (!c) return XBG_E_ARG;
+    return launch_sed(c, make_sp(c, dt, 0.0));
+}
+
+int xbg_step(xbg_ctx* c, double dt, double totaltime) {
+    if (!c) return XBG_E_ARG;
+    SP s = make_sp(c, dt, totaltime);
+    int rc;
+    if (c->P.flow == 1) {
+        LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(256), c->kp, s, c->f, c->P.swave == 1 ? 1 : 0);
+    }
+    if (c->P.swave == 1 && (rc = launch_wave(c, s)) != 0) return rc;
+    if (c->P.flow == 1 && (rc = launch_flow(c, s)) != 0) return rc;
+    if (c->P.sedtrans == 1 && totaltime >= c->P.sedstart && (rc = launch_sed(c, s)) != 0) return rc;
+    return XBG_OK;
+}
+
+int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
+    if (!c) return XBG_E_ARG;
+    float cfl = initial ? 0.5f * c->kp.CFL : c->kp.CFL;
+    LAUNCH(c, "k_cfl_flow", k_cfl_flow, dim3(c->nblk_flow), dim3(256), c->kp, c->f, cfl);
+    LAUNCH(c, "k_cfl_final", k_cfl_final, dim3(1), dim3(256), c->f, c->nblk_flow, c->nblk_wave);
+    HIPC(c, hipMemcpyAsync(c->host_red, c->f.red_out, 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double fm = (double)c->host_red[0];
+    if (c->host_red[2] != 0.0f) fm = NAN;
+    if (dt_flow) *dt_flow = fm;
+    if (dt_wave) *dt_wave = (double)c->host_red[1];
+    if (!(fm == fm) || fm == 0.0 || std::isinf(fm))
+        return fail(c, XBG_E_CRASH, "Model crashed: non-finite or zero flow time step");
+    return XBG_OK;
+}
+
+int xbg_accumulate_means(xbg_ctx* c) {
+    if (!c) return XBG_E_ARG;
+    LAUNCH(c, "k_means", k_means, dim3((unsigned)((c->N + 255) / 256)), dim3(256), c->kp, c->f, c->Hmean,
+           c->uumean, c->vvmean, c->hhmean, c->zsmean, c->zsmax, c->Cmean);
+    return XBG_OK;
+}
+
+int xbg_reset_means(xbg_ctx* c) {
+    if (!c) return XBG_E_ARG;
+    float* a[] = {c->Hmean, c->uumean, c->vvmean, c->hhmean, c->zsmean, c->zsmax, c->Cmean};
+    for (float* q : a) LAUNCH(c, "k_fill", k_fill, dim3((unsigned)((c->N + 255) / 256)), dim3(256), q, c->N, 0.0f);
+    return XBG_OK;
+}
+
+int xbg_divide_means(xbg_ctx* c, double nstep) {
+    if (!c) return XBG_E_ARG;
+    float* a[] = {c->Hmean, c->uumean, c->vvmean, c->hhmean, c->zsmean, c->Cmean};
+    for (float* q : a)
+        LAUNCH(c, "k_scale", k_scale, dim3((unsigned)((c->N + 255) / 256)), dim3(256), q, c->N, (float)nstep);
+    return XBG_OK;
+}
+
+int xbg_mass_balance(xbg_ctx* c, double* water, double* bed) {
+    if (!c) return XBG_E_ARG;
+    LAUNCH(c, "k_sum2", k_sum2, dim3(c->nblk_sum), dim3(256), c->f.hh, c->f.zb, c->N, c->f.red_sum);
+    HIPC(c, hipMemcpyAsync(c->host_sum, c->f.red_sum, 2 * c->nblk_sum * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double a = 0.0, b = 0.0, ca = 0.0, cb = 0.0;
+    for (int j = 0; j < c->nblk_sum; j++) {
+        double y = c->host_sum[2 * j] - ca, t = a + y;
+        ca = (t - a) - y;
+        a = t;
+        y = c->host_sum[2 * j + 1] - cb;
+        t = b + y;
+        cb = (t - b) - y;
+        b = t;
+    }
+    double area = c->P.dx * c->P.dx;
+    if (water) *water = a * area;
+    if (bed) *bed = b * area;
+    return XBG_OK;
+}
+
+int xbg_sync(xbg_ctx* c) {
+    if (!c) return XBG_E_ARG;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return XBG_OK;
+}
+
+void* xbg_stream(xbg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int xbg_profile(xbg_ctx* c, int enable) {
+    if (!c) return XBG_E_ARG;
+    if (!enable) prof_collect(c);
+    c->prof = enable != 0;
+    return XBG_OK;
+}
+
+int xbg_profile_report(xbg_ctx* c, char* buf, size_t n) {
+    if (!c || !buf || n == 0) return XBG_E_ARG;
+    prof_collect(c);
+    std::string out;
+    char line[256];
+    for (auto& kv : c->prof_acc) {
+        snprintf(line, sizeof(line), "%s %ld %.6f %.3f\n", kv.first.c_str(), kv.second.first, kv.second.second,
+                 kv.second.first ? 1000.0 * kv.second.second / kv.second.first : 0.0);
+        out += line;
+    }
+    c->prof_acc.clear();
+    size_t m = out.size() < n - 1 ? out.size() : n - 1;
+    std::memcpy(buf, out.data(), m);
+    buf[m] = 0;
+    return XBG_OK;
+}
+
+}  // extern "C"
