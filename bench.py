@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain FP32 surf-beat step on one MI355X.
+
+Workload: the large single domain, 2048 x 1024 cells, dx = 2 m, 24
+direction bins, barred rip-channel beach, JONSWAP-scale wave-group forcing
+(Hm0 3 m, Tp 12 s, 10 deg), FP64 bed-level accumulation.  One step is the
+reference main-loop step (Wave_gpu.cu:621-728): CFL time-step reductions,
+boundary record selection/interpolation, ubnd1D, the wave action balance,
+shallow-water flow, sediment transport and bed update.  All inputs are
+resident in HBM before the timed region.
+
+Prints one JSON line (metric cell-steps/s).  The roofline object prices the
+dominant kernel (k_wave) by its algorithmic bytes per launch over its average
+duration measured with HIP events on the library stream; cpu_baseline times
+the CPU oracle on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def build_domain(p, seed=1234):
+    from xbeach_gpu_amd import Domain, setups
+
+    elev = setups.barred_bathy(p.nx, p.ny, p.dx)
+    st = setups.spun_up_state(p, elev, seed=seed, Hrms=3.0 / math.sqrt(2.0))
+    frc = setups.GroupForcing(p, 3.0, 12.0, dir_deg=10.0, s=20.0, seed=0)
+    return st, frc
+
+
+def alg_bytes(p, kernel="step"):
+    """Algorithmic bytes per cell.  Step: compulsory read+write of the carried
+    state once per step, 16*ntheta (ee, rr) + 13 2D fields * 8 B, + 16 B for
+    the FP64 bed-level accumulator.  k_wave: ee and rr read and written once
+    (16*ntheta) plus the 2D inputs hh, uu, vv (12 B)."""
+    if kernel == "k_wave":
+        return 16 * p.ntheta + 12
+    return 16 * p.ntheta + 104 + (16 if p.zb_accum == 2 else 0)
+
+
+def run_gpu(p, steps, warmup, t0=300.0):
+    from xbeach_gpu_amd import Domain, next_dt
+
+    st, frc = build_domain(p)
+    d = Domain(p)
+    for k, v in st.items():
+        d.set(k, v)
+    d.init_state(frc.Trep)
+    nrec = int(math.ceil((steps + warmup) * 0.2 / frc.dtbc)) + 8
+    S, Q = frc.records(t0, nrec)
+    d.set_bnd_series(S, Q)
+
+    state = {"t": t0, "dt": 0.05}
+
+    def one_step():
+        dtf, dtw = d.cfl(False)
+        dt = next_dt(dtf, dtw, 1, state["dt"], state["t"], 1e30, 1e30)
+        t = state["t"] + dt
+        rec = int((t - t0) // frc.dtbc)
+        tsl = t - (t0 + rec * frc.dtbc)
+        d.use_bnd_record(rec, tsl, frc.dtbc, frc.Trep)
+        d.step(dt, t)
+        state["t"], state["dt"] = t, dt
+
+    for _ in range(warmup):
+        one_step()
+    d.sync()
+    tsim0 = state["t"]
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        one_step()
+    d.sync()
+    wall = time.perf_counter() - w0
+    sim = state["t"] - tsim0
+
+    # kernel timing pass (HIP events on the library stream)
+    d.profile(True)
+    for _ in range(min(steps, 10)):
+        one_step()
+    d.sync()
+    prof = d.profile_report()
+    d.profile(False)
+    fin = {k: bool(np.isfinite(d.get(k)).all()) for k in ("zs", "uu", "H", "zb")}
+    d.close()
+    return wall, sim, prof, fin
+
+
+def cpu_baseline(p, budget_s=20.0):
+    """CPU oracle (FP32, single thread) on the same workload: as many whole
+    steps of the large domain as fit in about budget_s seconds (at least 1)."""
+    from oracle.pyorc import Oracle
+
+    st, frc = build_domain(p)
+    o = Oracle(p, 4)
+    for k, v in st.items():
+        o.set(k, v)
+    o.init_state(frc.Trep)
+    t, dt, n = 300.0, 0.05, 0
+    w0 = time.perf_counter()
+    while True:
+        t += dt
+        o.set_forcing(frc.St(t), frc.qbnd(t), frc.qbnd(t + 1), frc.Trep, 0.5, 1.0)
+        o.step(dt, t)
+        n += 1
+        if time.perf_counter() - w0 > budget_s or n >= 50:
+            break
+    wall = time.perf_counter() - w0
+    o.close()
+    return {"value": p.nx * p.ny * n / wall, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} whole steps of the {p.nx}x{p.ny}x{p.ntheta} domain, oracle FP32, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="large", choices=["large", "small"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    from xbeach_gpu_amd import setups
+
+    p = setups.large_params() if a.workload == "large" else setups.small_params()
+    wall, sim, prof, fin = run_gpu(p, a.steps, a.warmup)
+    N = p.nx * p.ny
+    cps = N * a.steps / wall
+    kw = prof.get("k_wave", (0, 0.0, 0.0))
+    kw_us = kw[2]
+    b_kw = alg_bytes(p, "k_wave") * N
+    achieved = b_kw / (kw_us * 1e-6) / 1e9 if kw_us > 0 else 0.0
+    out = {
+        "metric": "cell-steps/s",
+        "value": cps,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "sim_h_per_wall_h": sim / wall,
+        "config": {"workload": f"{a.workload} single domain {p.nx}x{p.ny}, dx={p.dx} m, ntheta={p.ntheta}, "
+                               f"wave+flow+sediment+bed, zb_accum={p.zb_accum}",
+                   "nx": p.nx, "ny": p.ny, "ntheta": p.ntheta},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_wave",
+                     "kernel_avg_us": kw_us, "alg_bytes_per_cell": alg_bytes(p, "k_wave")},
+        "step_roofline_frac": alg_bytes(p) * cps / (HBM_PEAK_GBS * 1e9),
+        "kernels_us": {k: round(v[2], 2) for k, v in sorted(prof.items())},
+        "finite": all(fin.values()),
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, a.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

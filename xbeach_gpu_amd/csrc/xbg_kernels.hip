@@ -66,6 +66,7 @@ struct Fld {
     // wave products of the step
     float *k, *c, *cg, *H, *E, *Dw, *DR, *R, *urms, *ust_raw, *ust, *thetamean;
     float *Sxx, *Sxy, *Syy, *Fx, *Fy;
+    float *bnd_e1, *bnd_r1;  // post-Euler ee/rr of rows 1 and ny-2
     // flow intermediates
     float* nuh;
     uint8_t* wet;  // bit0 wetu, bit1 wetv (udepthmomcont / vdepthmomcont)
@@ -77,7 +78,7 @@ struct Fld {
     // per-block partial minima of the CFL reductions
     float *red_wave, *red_flow;
     int* red_bad;
-    float* red_out;  // [0] flow min, [1] wave min, [2] flag
+    unsigned* red_acc;  // two halves of {flow min bits, wave min bits, flag}
     double* red_sum;
 };
 
@@ -98,8 +99,11 @@ __global__ __launch_bounds__(256) void k_bnd_offshore(KP p, SP s, Fld f, int wav
         int cnt = min(1024, ny - base);
         for (int t = threadIdx.x; t < cnt; t += blockDim.x) col[t] = f.uu[(size_t)(base + t) * nx];
         __syncthreads();
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
+            // sequential row-order sum; LDS reads unrolled ahead of the adds
+#pragma unroll 16
             for (int t = 0; t < cnt; t++) acc = acc + col[t];
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) avg = acc / (float)ny;
@@ -151,10 +155,12 @@ XDEV float act_rr(const KP& p, const Fld& f, int x, int y, int th) {
     return f.rr_in[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
 }
 
-// propagthetaCPU (wave_CPU.cpp:259-287) with slopesCPU (209-257) at (x, y).
-template <int NT>
-XDEV void ctheta_cell(const KP& p, const SP& s, const Fld& f, int x, int y, float* ct) {
-    const int nt = NT > 0 ? NT : p.nt;
+// Refraction speed of propagthetaCPU (wave_CPU.cpp:259-287), with the
+// slopes of slopesCPU (209-257) evaluated once per cell.
+struct Refr {
+    float sig, dhdx, dhdy, dudx, dudy, dvdx, dvdy, w, lo, hi;
+};
+XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     size_t i = (size_t)x + (size_t)y * nx;
@@ -162,104 +168,108 @@ XDEV void ctheta_cell(const KP& p, const SP& s, const Fld& f, int x, int y, floa
     size_t ib = (size_t)x + (size_t)ym * nx, it = (size_t)x + (size_t)yp * nx;
     float ddx = (float)(unsigned)(xp - xm) * p.dx;
     float ddy = (float)(unsigned)(yp - ym) * p.dx;
-    float dhdx = (f.hh[ir] - f.hh[il]) / ddx;
-    float dudx = (f.uu[ir] - f.uu[il]) / ddx;
-    float dvdx = (f.vv[ir] - f.vv[il]) / ddx;
-    float dhdy = (f.hh[it] - f.hh[ib]) / ddy;
-    float dudy = (f.uu[it] - f.uu[ib]) / ddy;
-    float dvdy = (f.vv[it] - f.vv[ib]) / ddy;
+    Refr r;
+    r.dhdx = (f.hh[ir] - f.hh[il]) / ddx;
+    r.dudx = (f.uu[ir] - f.uu[il]) / ddx;
+    r.dvdx = (f.vv[ir] - f.vv[il]) / ddx;
+    r.dhdy = (f.hh[it] - f.hh[ib]) / ddy;
+    r.dudy = (f.uu[it] - f.uu[ib]) / ddy;
+    r.dvdy = (f.vv[it] - f.vv[ib]) / ddy;
     float h = f.hh[i];
     float kh = fmn(f.k[i] * h, 10.0f);
-    float sig = s.sigm / sinhf(2.0f * kh);
-    float w = p.wci * fmn(h / p.hwci, 1.0f);
-#pragma unroll
-    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
-        if (th >= nt) break;
-        float cs = p.cxsth[th], sn = p.sxnth[th];
-        float ctht = sig * (dhdx * sn - dhdy * cs) + w * (cs * (sn * dudx - cs * dudy) + sn * (sn * dvdx - cs * dvdy));
-        ct[th] = fmn(fmx(ctht, -0.25f * s.sigm), 0.25f * s.sigm);
-    }
+    r.sig = s.sigm / sinhf(2.0f * kh);
+    r.w = p.wci * fmn(h / p.hwci, 1.0f);
+    r.lo = -0.25f * s.sigm;
+    r.hi = 0.25f * s.sigm;
+    return r;
+}
+XDEV float ctheta_of(const KP& p, const Refr& r, int th) {
+    float cs = p.cxsth[th], sn = p.sxnth[th];
+    float ctht = r.sig * (r.dhdx * sn - r.dhdy * cs) + r.w * (cs * (sn * r.dudx - cs * r.dudy) + sn * (sn * r.dvdx - cs * r.dvdy));
+    return fmn(fmx(ctht, r.lo), r.hi);
 }
 
-// x/y upwind advection (xadvecupwind2CPU, yadvecupwind2CPU,
-// wave_CPU.cpp:367-564) + theta advection (thetaadvecuw2hoCPU, 706-808) +
-// Euler update (eulerupwindCPU, 810-828) of ee (IS_EE, speed cg) or rr
-// (speed c) at cell (x, y).  out[th] = the field after the Euler update.
-template <int NT, bool IS_EE>
-XDEV void euler_cell(const KP& p, const SP& s, const Fld& f, int x, int y, const float* ct, float* out) {
-    const int nt = NT > 0 ? NT : p.nt;
+// Face speeds of the x/y upwind advection at one cell
+// (xadvecupwind2CPU / yadvecupwind2CPU, wave_CPU.cpp:367-564): group (ee)
+// or phase (rr) speed at the cell and its +-1 neighbours, current times the
+// cell's wci.
+struct AdvC {
+    float sc, sxm, sxp, sym, syp, ui, uxm, uxp, vi, vym, vyp;
+};
+XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
     const int nx = p.nx, ny = p.ny;
-    const size_t N = p.N;
-    int xm = m1(x), xp = p1(x, nx), xm2 = m2(x), xp2 = p2(x, nx);
-    int ym = m1(y), yp = p1(y, ny), ym2 = m2(y), yp2 = p2(y, ny);
+    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     size_t row = (size_t)y * nx;
-    const float* S = IS_EE ? f.cg : f.c;
-    float sc = S[x + row], sxm = S[xm + row], sxp = S[xp + row];
-    float sym = S[x + (size_t)ym * nx], syp = S[x + (size_t)yp * nx];
+    AdvC a;
+    a.sc = S[x + row];
+    a.sxm = S[xm + row];
+    a.sxp = S[xp + row];
+    a.sym = S[x + (size_t)ym * nx];
+    a.syp = S[x + (size_t)yp * nx];
     float w = p.wci * fmn(f.hh[x + row] / p.hwci, 1.0f);
-    float ui = f.uu[x + row] * w, uxm = f.uu[xm + row] * w, uxp = f.uu[xp + row] * w;
-    float vi = w * f.vv[x + row], vym = w * f.vv[x + (size_t)ym * nx], vyp = w * f.vv[x + (size_t)yp * nx];
-    float dxi = 1.0f / p.dx;
-    float e0[NT > 0 ? NT : MAXT], adv[NT > 0 ? NT : MAXT];
-#pragma unroll
-    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
-        if (th >= nt) break;
-        float ec, exm, exm2, exp_, exp2, eym, eym2, eyp, eyp2;
-        if (IS_EE) {
-            ec = act_ee(p, s, f, x, y, th);
-            exm = act_ee(p, s, f, xm, y, th);
-            exm2 = act_ee(p, s, f, xm2, y, th);
-            exp_ = act_ee(p, s, f, xp, y, th);
-            exp2 = act_ee(p, s, f, xp2, y, th);
-            eym = act_ee(p, s, f, x, ym, th);
-            eym2 = act_ee(p, s, f, x, ym2, th);
-            eyp = act_ee(p, s, f, x, yp, th);
-            eyp2 = act_ee(p, s, f, x, yp2, th);
-        } else {
-            const float* r = f.rr_in + (size_t)th * N;
-            ec = r[x + row];
-            exm = r[xm + row];
-            exm2 = r[xm2 + row];
-            exp_ = r[xp + row];
-            exp2 = r[xp2 + row];
-            eym = r[x + (size_t)ym * nx];
-            eym2 = r[x + (size_t)ym2 * nx];
-            eyp = r[x + (size_t)yp * nx];
-            eyp2 = r[x + (size_t)yp2 * nx];
-        }
-        float cs = p.cxsth[th], sn = p.sxnth[th];
-        float cgx = 0.5f * (sc * cs + ui + sxp * cs + uxp);
-        float cgxmin = 0.5f * (sc * cs + ui + sxm * cs + uxm);
-        float ain = cgx > 0.0f ? upw2d(ec, exm) * cgx : upw2d(exp_, exp2) * cgx;
-        float amin = cgxmin > 0.0f ? upw2d(exm, exm2) * cgxmin : upw2d(ec, exp_) * cgxmin;
-        float xa = (ain - amin) * dxi;
-        float cgy = 0.5f * (sc * sn + vi + syp * sn + vyp);
-        float cgymin = 0.5f * (sc * sn + vi + sym * sn + vym);
-        ain = cgy > 0.0f ? upw2d(ec, eym) * cgy : upw2d(eyp, eyp2) * cgy;
-        amin = cgymin > 0.0f ? upw2d(eym, eym2) * cgymin : upw2d(ec, eyp) * cgymin;
-        float ya = (ain - amin) * dxi;
-        e0[th] = ec;
-        adv[th] = xa + ya;
+    a.ui = f.uu[x + row] * w;
+    a.uxm = f.uu[xm + row] * w;
+    a.uxp = f.uu[xp + row] * w;
+    a.vi = w * f.vv[x + row];
+    a.vym = w * f.vv[x + (size_t)ym * nx];
+    a.vyp = w * f.vv[x + (size_t)yp * nx];
+    return a;
+}
+
+// x + y flux divergence of one bin from its 9-point cross stencil
+// (values at the clamped neighbour indices).
+XDEV float adv_xy(const KP& p, const AdvC& a, float dxi, int th, float ec, float exm, float exm2, float exp_,
+                  float exp2, float eym, float eym2, float eyp, float eyp2) {
+    float cs = p.cxsth[th], sn = p.sxnth[th];
+    // upwind operands are selected first so each face evaluates one
+    // 2nd-order value (branch-free code would otherwise evaluate both)
+    float cgx = 0.5f * (a.sc * cs + a.ui + a.sxp * cs + a.uxp);
+    float cgxmin = 0.5f * (a.sc * cs + a.ui + a.sxm * cs + a.uxm);
+    bool px = cgx > 0.0f, pxm = cgxmin > 0.0f;
+    float ain = upw2d(px ? ec : exp_, px ? exm : exp2) * cgx;
+    float amin = upw2d(pxm ? exm : ec, pxm ? exm2 : exp_) * cgxmin;
+    float xa = (ain - amin) * dxi;
+    float cgy = 0.5f * (a.sc * sn + a.vi + a.syp * sn + a.vyp);
+    float cgymin = 0.5f * (a.sc * sn + a.vi + a.sym * sn + a.vym);
+    bool py = cgy > 0.0f, pym = cgymin > 0.0f;
+    ain = upw2d(py ? ec : eyp, py ? eym : eyp2) * cgy;
+    amin = upw2d(pym ? eym : ec, pym ? eym2 : eyp) * cgymin;
+    float ya = (ain - amin) * dxi;
+    return xa + ya;
+}
+
+// theta advection (thetaadvecuw2hoCPU, wave_CPU.cpp:706-808) and the Euler
+// update (eulerupwindCPU, 810-828): ea holds x+y advection on entry and the
+// updated field on exit.  The reference forms the flux through the face
+// between bins th and th+1 twice, as arrint of th and as arrmint of th+1,
+// from identical operands; here it is formed once per face (same value).
+// Zero flux through the two end faces.
+template <int NT>
+XDEV void theta_euler(const KP& p, const SP& s, const Refr& rf, const float* e0, float* ea) {
+    const int nt = NT > 0 ? NT : p.nt;
+    if (nt <= 1) {
+        ea[0] = e0[0] - s.dt * (ea[0] + 0.0f);
+        return;
     }
+    float ctl = ctheta_of(p, rf, 0);
+    float fl = 0.0f;  // flux through the face below the current bin
 #pragma unroll
     for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
         if (th >= nt) break;
         float ta;
-        if (nt > 1) {
-            int tm = m1(th), tp = p1(th, nt), tm2 = m2(th), tp2 = p2(th, nt);
-            float cb = 0.5f * (ct[th] + ct[tp]);
+        if (th < nt - 1) {
+            int tm = m1(th), tp = th + 1, tp2 = p2(th, nt);
+            float ctr = ctheta_of(p, rf, tp);
+            float cb = 0.5f * (ctl + ctr);
             float eu = cb > 0.0f ? upw2f(e0[th], e0[tm]) : upw2f(e0[tp], e0[tp2]);
-            float arrint = cb * eu;
-            cb = 0.5f * (ct[tm] + ct[th]);
-            eu = cb > 0.0f ? upw2f(e0[tm], e0[tm2]) : upw2f(e0[th], e0[tp]);
-            float arrmint = cb * eu;
-            ta = (arrint - arrmint) / p.dtheta;
-            if (th == nt - 1) ta = (0.0f - arrmint) / p.dtheta;
-            if (th == 0) ta = (arrint - 0.0f) / p.dtheta;
+            float fr = cb * eu;
+            ta = th == 0 ? (fr - 0.0f) / p.dtheta : (fr - fl) / p.dtheta;
+            fl = fr;
+            ctl = ctr;
         } else {
-            ta = 0.0f;
+            ta = (0.0f - fl) / p.dtheta;
         }
-        out[th] = e0[th] - s.dt * (adv[th] + ta);
+        ea[th] = e0[th] - s.dt * (ea[th] + ta);
     }
 }
 
@@ -343,132 +353,262 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     DRout = drr * p.dtheta;
 }
 
-// The fused wave step.  Thread = cell.  Rows 0 and ny-1 take the
-// rollerlatbnd copies (wave_CPU.cpp:830-871) from rows 1 and ny-2, so their
-// threads also evaluate the pipeline of that source row.
+// meandirCPU (wave_CPU.cpp:1092-1116), radstressCPU (1118-1157) and
+// breakerdelayCPU (1334-1374, called with R) at cell i on the final ee/rr;
+// stores the directional state of the cell.
 template <int NT>
-__global__ __launch_bounds__(256) void k_wave(KP p, SP s, Fld f) {
+XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const float* efin, const float* r, float Rv,
+                      float DRv) {
+    const int nt = NT > 0 ? NT : p.nt;
+    const size_t N = p.N;
+    float cgi = f.cg[i], ci = f.c[i];
+    float nn = cgi / ci;
+    float sume = 0.0f, sumethet = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f, rxx = 0.0f, ryy = 0.0f, rxy = 0.0f;
+#pragma unroll
+    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
+        if (th >= nt) break;
+        float ev = efin[th], rv = r[th];
+        float cs = p.cxsth[th], sn = p.sxnth[th];
+        sume = sume + ev;
+        sumethet = sumethet + ev * p.theta[th];
+        txx = txx + ((1.0f + cs * cs) * ev);
+        tyy = tyy + ((1.0f + sn * sn) * ev);
+        txy = txy + (sn * cs * ev);
+        rxx = rxx + ((cs * cs) * rv);
+        ryy = ryy + ((sn * sn) * rv);
+        rxy = rxy + (sn * cs * rv);
+        f.ee_out[i + (size_t)th * N] = ev;
+        f.rr_out[i + (size_t)th * N] = rv;
+    }
+    f.Sxx[i] = (nn * txx - 0.5f * sume) * p.dtheta + rxx * p.dtheta;
+    f.Syy[i] = (nn * tyy - 0.5f * sume) * p.dtheta + ryy * p.dtheta;
+    f.Sxy[i] = nn * txy * p.dtheta + rxy * p.dtheta;
+    float sm = fmx(sume, 0.00001f);
+    f.thetamean[i] = (sumethet / (float)nt) / (sm / (float)nt);
+    float Em = sm * p.dtheta;
+    float Hm = sqrtf(sm * p.dtheta / (p.rho * p.g / 8));
+    f.E[i] = Em;
+    f.H[i] = Hm;
+    f.R[i] = Rv;
+    f.DR[i] = DRv;
+    float hmin = 0.2f;
+    float hhi = fmx(f.hh[i], hmin);
+    float cib = fmx(ci, sqrtf(hmin * p.g));
+    float uorb = (float)(PI_ * (double)Hm / (double)s.Trep / (double)sinhf(fmn(fmx(f.k[i], 0.01f) * hhi, 10.0f)));
+    f.urms[i] = uorb / (sqrtf(2.0f));
+    float ustw = Em / cib / p.rho / hhi;
+    float ustr = 2.0f * Rv / cib / p.rho / hhi;
+    f.ust_raw[i] = ustw + ustr;
+}
+
+// Wave tile: 64 cells along x (one wavefront per row) by WTY rows; each
+// directional plane of the tile plus a 2-cell halo is staged through LDS.
+constexpr int WTY = 8;
+constexpr int PW = BX + 4;
+constexpr int PH = WTY + 4;
+constexpr int PN = PW * PH;
+constexpr int WTHREADS = BX * WTY;
+
+// Fill one directional plane of the tile (+halo) into LDS.  Halo cells hold
+// the values at the clamped neighbour indices (mminus/pplus/mminus2/pplus2
+// equal clamping), so the stencil reads are plain offsets.
+template <bool IS_EE>
+XDEV void fill_plane(float* pl, const KP& p, const SP& s, const Fld& f, int x0, int y0, int th, int tid) {
+    const int nx = p.nx, ny = p.ny;
+    // opaque bin index: keeps the compiler from precomputing (and holding
+    // in registers) the plane addresses of every bin of the unrolled loop
+    int thv = th;
+    asm volatile("" : "+s"(thv));
+    const uint32_t pbase = (uint32_t)thv * p.N;
+    th = thv;
+#pragma unroll
+    for (int it = 0; it < (PN + WTHREADS - 1) / WTHREADS; it++) {
+        int l = tid + it * WTHREADS;
+        if (l < PN) {
+            int r = l / PW, c = l - r * PW;
+            int gxx = min(max(x0 + c - 2, 0), nx - 1);
+            int gyy = min(max(y0 + r - 2, 0), ny - 1);
+            float v;
+            if (IS_EE) {
+                v = gxx == 0 ? f.St[gyy + th * ny] * s.taper : f.ee_in[pbase + (uint32_t)gxx + (uint32_t)gyy * nx];
+                v = fmx(v, 0.000f) / s.sigm;  // sanity + action (wave_CPU.cpp:58-82, 289-309)
+            } else {
+                v = f.rr_in[pbase + (uint32_t)gxx + (uint32_t)gyy * nx];
+            }
+            pl[l] = v;
+        }
+    }
+}
+
+// x/y advection, theta advection and Euler update of all bins of one field
+// for the thread's cell.  Planes are staged through two LDS buffers (the
+// load of bin th+1 overlaps bin th).  Bin th-2 is finished as soon as the
+// action of bin th is known (its theta stencil reaches th-2..th+2), so only
+// a 5-bin window of pre-update values is live.  ea[] holds the updated
+// field on exit (eulerupwindCPU, wave_CPU.cpp:810-828).
+template <int NT, bool IS_EE>
+XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f, const AdvC& a, const Refr& rf,
+                        int x0, int y0, int tid, int cx, int cy, float* ea) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
-    __shared__ float wred[256];
-    int x = gx(), y = gy();
-    float wmin = 9999999.9f;
-    if (x < p.nx && y < p.ny) {
-        const int nx = p.nx, ny = p.ny;
-        const size_t N = p.N;
-        size_t i = (size_t)x + (size_t)y * nx;
-        int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
-        size_t is = (size_t)x + (size_t)ys * nx;
-        float wet = f.hh[i] > p.eps ? 1.0f : 0.0f;
-        float ct[A], e[A], r[A];
-        // WAVEDT (Wave_kernel.cu:232-249) uses this cell's ctheta.
-        ctheta_cell<NT>(p, s, f, x, y, ct);
-#pragma unroll
-        for (int th = 0; th < A; th++) {
-            if (th >= nt) break;
-            float v = p.CFL * p.dtheta / fabsf(ct[th]);
-            wmin = (v < wmin) ? v : wmin;  // fminf: a NaN speed is ignored
-        }
-        if (ys != y) ctheta_cell<NT>(p, s, f, x, ys, ct);
-        euler_cell<NT, true>(p, s, f, x, ys, ct, e);
-        if (p.roller == 1) {
-            euler_cell<NT, false>(p, s, f, x, ys, ct, r);
+    const float dxi = 1.0f / p.dx;
+    float e0[A];
+    float ctl = ctheta_of(p, rf, 0), fl = 0.0f;
+    // finish bin b (0 <= b < nt) once e0[b-1..b+2] are known; theta flux
+    // through the face between b and b+1 formed once (thetaadvecuw2hoCPU,
+    // wave_CPU.cpp:706-808, forms it twice from identical operands)
+    auto finish = [&](int b) {
+        float ta;
+        if (nt <= 1) {
+            ta = 0.0f;
+        } else if (b < nt - 1) {
+            int tm = m1(b), tp = b + 1, tp2 = p2(b, nt);
+            float ctr = ctheta_of(p, rf, tp);
+            float cb = 0.5f * (ctl + ctr);
+            bool pc = cb > 0.0f;
+            float eu = upw2f(pc ? e0[b] : e0[tp], pc ? e0[tm] : e0[tp2]);
+            float fr = cb * eu;
+            ta = b == 0 ? (fr - 0.0f) / p.dtheta : (fr - fl) / p.dtheta;
+            fl = fr;
+            ctl = ctr;
         } else {
-#pragma unroll
-            for (int th = 0; th < A; th++) {
-                if (th >= nt) break;
-                r[th] = act_rr(p, f, x, y, th);
-            }
+            ta = (0.0f - fl) / p.dtheta;
         }
-        float Rv, DRv;
-        float efin[A];
-        if (ys != y) {
-            // source row pipeline for the second rollerlatbnd(ee)
-            float es[A], rs[A];
-#pragma unroll
-            for (int th = 0; th < A; th++) {
-                if (th >= nt) break;
-                es[th] = e[th];
-                rs[th] = r[th];
-                e[th] = e[th] * wet;
-                if (p.roller == 1) r[th] = r[th] * wet;
-            }
-            if (p.roller != 1) {
-#pragma unroll
-                for (int th = 0; th < A; th++) {
-                    if (th >= nt) break;
-                    rs[th] = act_rr(p, f, x, ys, th);
-                }
-            }
-            float Rs, DRs;
-            dissip_cell<NT>(p, s, f, is, es, rs, Rs, DRs);
-            dissip_cell<NT>(p, s, f, i, e, r, Rv, DRv);
-#pragma unroll
-            for (int th = 0; th < A; th++) {
-                if (th >= nt) break;
-                efin[th] = es[th] * wet;
-            }
-        } else {
-            dissip_cell<NT>(p, s, f, i, e, r, Rv, DRv);
-#pragma unroll
-            for (int th = 0; th < A; th++) {
-                if (th >= nt) break;
-                efin[th] = e[th];
-            }
-        }
-        // meandirCPU (wave_CPU.cpp:1092-1116) and radstressCPU (1118-1157)
-        float cgi = f.cg[i], ci = f.c[i];
-        float nn = cgi / ci;
-        float sume = 0.0f, sumethet = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f, rxx = 0.0f, ryy = 0.0f, rxy = 0.0f;
-#pragma unroll
-        for (int th = 0; th < A; th++) {
-            if (th >= nt) break;
-            float ev = efin[th], rv = r[th];
-            float cs = p.cxsth[th], sn = p.sxnth[th];
-            sume = sume + ev;
-            sumethet = sumethet + ev * p.theta[th];
-            txx = txx + ((1.0f + cs * cs) * ev);
-            tyy = tyy + ((1.0f + sn * sn) * ev);
-            txy = txy + (sn * cs * ev);
-            rxx = rxx + ((cs * cs) * rv);
-            ryy = ryy + ((sn * sn) * rv);
-            rxy = rxy + (sn * cs * rv);
-            f.ee_out[i + (size_t)th * N] = ev;
-            f.rr_out[i + (size_t)th * N] = rv;
-        }
-        float sumr = txx;  // keep radstress sums in their own registers
-        (void)sumr;
-        f.Sxx[i] = (nn * txx - 0.5f * sume) * p.dtheta + rxx * p.dtheta;
-        f.Syy[i] = (nn * tyy - 0.5f * sume) * p.dtheta + ryy * p.dtheta;
-        f.Sxy[i] = nn * txy * p.dtheta + rxy * p.dtheta;
-        float sm = fmx(sume, 0.00001f);
-        f.thetamean[i] = (sumethet / (float)nt) / (sm / (float)nt);
-        float Em = sm * p.dtheta;
-        float Hm = sqrtf(sm * p.dtheta / (p.rho * p.g / 8));
-        f.E[i] = Em;
-        f.H[i] = Hm;
-        f.R[i] = Rv;
-        f.DR[i] = DRv;
-        // breakerdelayCPU (wave_CPU.cpp:1334-1374), called with R
-        float hmin = 0.2f;
-        float hhi = fmx(f.hh[i], hmin);
-        float cib = fmx(ci, sqrtf(hmin * p.g));
-        float uorb = (float)(PI_ * (double)Hm / (double)s.Trep /
-                             (double)sinhf(fmn(fmx(f.k[i], 0.01f) * hhi, 10.0f)));
-        f.urms[i] = uorb / (sqrtf(2.0f));
-        float ustw = Em / cib / p.rho / hhi;
-        float ustr = 2.0f * Rv / cib / p.rho / hhi;
-        f.ust_raw[i] = ustw + ustr;
-    }
-    // block minimum of the wave CFL limit
-    int t = threadIdx.x + threadIdx.y * BX;
-    wred[t] = wmin;
+        ea[b] = e0[b] - s.dt * (ea[b] + ta);
+    };
+    fill_plane<IS_EE>(pl[0], p, s, f, x0, y0, 0, tid);
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (t < st) wred[t] = fmn(wred[t], wred[t + st]);
+    const int c0 = (cy + 2) * PW + (cx + 2);
+#pragma unroll
+    for (int th = 0; th < A; th++) {
+        if (th >= nt) break;
+        if (th + 1 < nt) fill_plane<IS_EE>(pl[(th + 1) & 1], p, s, f, x0, y0, th + 1, tid);
+        const float* q = pl[th & 1];
+        float ec = q[c0];
+        e0[th] = ec;
+        ea[th] = adv_xy(p, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
+                        q[c0 + PW], q[c0 + 2 * PW]);
+        if (th >= 2) finish(th - 2);
+        // materialise this bin's results before the barrier: otherwise the
+        // arithmetic is sunk past all barriers and the nine stencil values of
+        // every bin stay live in registers
+        asm volatile("" : "+v"(ea[th]));
+        if (th >= 2) asm volatile("" : "+v"(ea[th - 2]));
         __syncthreads();
     }
-    if (t == 0) f.red_wave[blockIdx.x + blockIdx.y * gridDim.x] = wred[0];
+#pragma unroll
+    for (int b = 0; b < A; b++) {
+        if (b >= nt) break;
+        if (b >= nt - 2) finish(b);
+    }
+}
+
+// The fused wave step, one thread per cell of a 64 x WTY tile:
+// offshorebndWav, sanity, action, x/y/theta advection + Euler of ee and rr,
+// energy, energint, roelvink/baldock, dissipation, meandir, radstress,
+// breakerdelay (Wavestep.cu:781-1125 order).  Rows 0 and ny-1 are finished
+// by k_wave_rows (rollerlatbnd copies from rows 1 and ny-2, whose
+// post-Euler bins this kernel saves).  Also the block minimum of WAVEDT.
+template <int NT>
+__global__ __launch_bounds__(WTHREADS) void k_wave(KP p, SP s, Fld f) {
+    constexpr int A = NT > 0 ? NT : MAXT;
+    const int nt = NT > 0 ? NT : p.nt;
+    __shared__ float pl[2][PN];
+    __shared__ float wred[WTHREADS / 64];
+    const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * WTY;
+    const int nx = p.nx, ny = p.ny;
+    const int x = min(x0 + cx, nx - 1), y = min(y0 + cy, ny - 1);
+    const bool valid = (x0 + cx < nx) && (y0 + cy < ny);
+    const uint32_t i = (uint32_t)x + (uint32_t)y * nx;
+    float ea[A];
+    const Refr rf = refr_cell(p, s, f, x, y);
+    {
+        AdvC a = advc_cell(p, f, f.cg, x, y);
+        advect_planes<NT, true>(pl, p, s, f, a, rf, x0, y0, tid, cx, cy, ea);
+    }
+    // WAVEDT (Wave_kernel.cu:232-249): min over bins of CFL*dtheta/|ctheta|.
+    // Rounded division is monotonic in the divisor, so this equals one
+    // division by the largest |ctheta| (NaN speeds ignored as fminf does).
+    float cmax = 0.0f;
+#pragma unroll
+    for (int th = 0; th < A; th++) {
+        if (th >= nt) break;
+        float v = fabsf(ctheta_of(p, rf, th));
+        cmax = v > cmax ? v : cmax;
+    }
+    float wmin = fmn(9999999.9f, p.CFL * p.dtheta / cmax);
+    const int side = (y == 1) ? 0 : (y == ny - 2 ? 1 : -1);
+    if (valid && side >= 0) {
+#pragma unroll
+        for (int th = 0; th < A; th++) {
+            if (th >= nt) break;
+            f.bnd_e1[((uint32_t)side * nt + th) * nx + x] = ea[th];
+        }
+    }
+    float ra[A];
+    if (p.roller == 1) {
+        AdvC a = advc_cell(p, f, f.c, x, y);
+        advect_planes<NT, false>(pl, p, s, f, a, rf, x0, y0, tid, cx, cy, ra);
+        if (valid && side >= 0) {
+#pragma unroll
+            for (int th = 0; th < A; th++) {
+                if (th >= nt) break;
+                f.bnd_r1[((uint32_t)side * nt + th) * nx + x] = ra[th];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int th = 0; th < A; th++) {
+            if (th >= nt) break;
+            ra[th] = f.rr_in[i + (uint32_t)th * p.N];
+        }
+    }
+    if (valid) {
+        float Rv, DRv;
+        dissip_cell<NT>(p, s, f, i, ea, ra, Rv, DRv);
+        finish_cell<NT>(p, s, f, i, ea, ra, Rv, DRv);
+    } else {
+        wmin = 9999999.9f;
+    }
+    // block minimum: wavefront reduction, then across the WTY/... waves
+    for (int o = 32; o > 0; o >>= 1) wmin = fmn(wmin, __shfl_xor(wmin, o, 64));
+    if ((tid & 63) == 0) wred[tid >> 6] = wmin;
+    __syncthreads();
+    if (tid == 0) {
+        float m = wred[0];
+        for (int w = 1; w < WTHREADS / 64; w++) m = fmn(m, wred[w]);
+        f.red_wave[blockIdx.x + blockIdx.y * gridDim.x] = m;
+    }
+}
+
+// Rows 0 and ny-1 (rollerlatbnd, wave_CPU.cpp:830-871): the post-Euler
+// ee/rr of rows 1 / ny-2 times the cell's wet flag, then the cell's own
+// energy..dissipation, and finally ee after dissipation copied again from
+// rows 1 / ny-2.
+template <int NT>
+__global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
+    constexpr int A = NT > 0 ? NT : MAXT;
+    const int nt = NT > 0 ? NT : p.nt;
+    int x = blockIdx.x * 256 + threadIdx.x;
+    int side = blockIdx.y;
+    if (x >= p.nx) return;
+    const int nx = p.nx, ny = p.ny;
+    const size_t N = p.N;
+    int y = side ? ny - 1 : 0, ys = side ? ny - 2 : 1;
+    size_t i = (size_t)x + (size_t)y * nx, is = (size_t)x + (size_t)ys * nx;
+    float wet = f.hh[i] > p.eps ? 1.0f : 0.0f;
+    float e[A], r[A], efin[A];
+#pragma unroll
+    for (int th = 0; th < A; th++) {
+        if (th >= nt) break;
+        e[th] = f.bnd_e1[((size_t)side * nt + th) * nx + x] * wet;
+        r[th] = p.roller == 1 ? f.bnd_r1[((size_t)side * nt + th) * nx + x] * wet : act_rr(p, f, x, y, th);
+        efin[th] = f.ee_out[is + (size_t)th * N] * wet;
+    }
+    float Rv, DRv;
+    dissip_cell<NT>(p, s, f, i, e, r, Rv, DRv);
+    finish_cell<NT>(p, s, f, i, efin, r, Rv, DRv);
 }
 
 // wavforceCPU (wave_CPU.cpp:1159-1223) at (x, y)
@@ -1219,45 +1359,44 @@ __global__ __launch_bounds__(256) void k_init_depth(KP p, Fld f) {
     if (p.zb_accum == 2) f.zb64[i] = (double)zb[i];
 }
 
-// FLOWDT (Wave_kernel.cu:161-170) block minima; non-finite values raise a flag.
-__global__ __launch_bounds__(256) void k_cfl_flow(KP p, Fld f, float cfl) {
-    __shared__ float red[256];
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    float v = 3.0e38f;
-    if (i < p.N) {
+// FLOWDT (Wave_kernel.cu:161-170) and the WAVEDT block partials reduced to
+// global minima in one pass.  Both limits are positive floats, whose IEEE
+// bit patterns order like the values, so the global minimum is an unsigned
+// atomicMin.  Non-finite or zero flow limits raise a flag (the reference's
+// crash check, Wave_gpu.cu:642-647).  acc[0..2] of the active half were
+// reset by the previous call, which also resets the other half.
+__global__ __launch_bounds__(256) void k_cfl(KP p, Fld f, float cfl, int nwave, unsigned* acc, unsigned* other) {
+    __shared__ float rf[4], rw[4];
+    const size_t i0 = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+    float v = 3.0e38f, w = 9999999.9f;
+    int bad = 0;
+    for (size_t i = i0; i < p.N; i += stride) {
         float d = cfl * p.dx / (sqrtf(9.81f * f.hh[i]));
-        if (!(d == d) || d == INFINITY || d == 0.0f) atomicOr(f.red_bad, 1);
-        v = d;
+        if (!(d == d) || d == INFINITY || d == 0.0f) bad = 1;
+        else v = fmn(v, d);
     }
-    red[threadIdx.x] = v;
+    for (size_t j = i0; j < (size_t)nwave; j += stride) w = fmn(w, f.red_wave[j]);
+    for (int o = 32; o > 0; o >>= 1) {
+        v = fmn(v, __shfl_xor(v, o, 64));
+        w = fmn(w, __shfl_xor(w, o, 64));
+    }
+    bad = __any(bad);
+    if ((threadIdx.x & 63) == 0) {
+        rf[threadIdx.x >> 6] = v;
+        rw[threadIdx.x >> 6] = w;
+        if (bad) atomicOr(&acc[2], 1u);
+    }
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) red[threadIdx.x] = fmn(red[threadIdx.x], red[threadIdx.x + st]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) f.red_flow[blockIdx.x] = red[0];
-}
-
-__global__ __launch_bounds__(256) void k_cfl_final(Fld f, int nflow, int nwave) {
-    __shared__ float a[256], b[256];
-    float fm = 3.0e38f, wm = 9999999.9f;
-    for (int j = threadIdx.x; j < nflow; j += 256) fm = fmn(fm, f.red_flow[j]);
-    for (int j = threadIdx.x; j < nwave; j += 256) wm = fmn(wm, f.red_wave[j]);
-    a[threadIdx.x] = fm;
-    b[threadIdx.x] = wm;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
-            a[threadIdx.x] = fmn(a[threadIdx.x], a[threadIdx.x + st]);
-            b[threadIdx.x] = fmn(b[threadIdx.x], b[threadIdx.x + st]);
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        f.red_out[0] = a[0];
-        f.red_out[1] = b[0];
-        f.red_out[2] = (float)(*f.red_bad);
-        *f.red_bad = 0;
+        v = fmn(fmn(rf[0], rf[1]), fmn(rf[2], rf[3]));
+        w = fmn(fmn(rw[0], rw[1]), fmn(rw[2], rw[3]));
+        atomicMin(&acc[0], __float_as_uint(v));
+        atomicMin(&acc[1], __float_as_uint(w));
+        if (blockIdx.x == 0) {
+            other[0] = 0x7f7fffffu;
+            other[1] = 0x7f7fffffu;
+            other[2] = 0u;
+        }
     }
 }
 
@@ -1350,7 +1489,7 @@ struct xbg_ctx {
     float *Hmean = nullptr, *uumean = nullptr, *vvmean = nullptr, *hhmean = nullptr, *zsmean = nullptr,
           *zsmax = nullptr, *Cmean = nullptr;
     float Trep = 10.0f, tsl = 0.0f, tnext = 1.0f, zsbndi = 0.0f, zsbndn = 0.0f, windu = 0.0f, windv = 0.0f;
-    int nblk_wave = 0, nblk_flow = 0, nblk_sum = 0;
+    int nblk_wave = 0, nblk_flow = 0, nblk_sum = 0, acc_cur = 0;
     float* host_red = nullptr;
     double* host_sum = nullptr;
     std::string err;
@@ -1531,10 +1670,21 @@ int launch_wave(xbg_ctx* c, const SP& s) {
     KP& p = c->kp;
     dim3 g = grid2(p);
     LAUNCH(c, "k_dispersion", k_dispersion, g, blk2, p, s, c->f);
+    dim3 gr((p.nx + 255) / 256, 2, 1);
+    dim3 gw((p.nx + BX - 1) / BX, (p.ny + WTY - 1) / WTY, 1), bw(BX, WTY, 1);
     switch (p.nt) {
-        case 12: LAUNCH(c, "k_wave", k_wave<12>, g, blk2, p, s, c->f); break;
-        case 24: LAUNCH(c, "k_wave", k_wave<24>, g, blk2, p, s, c->f); break;
-        default: LAUNCH(c, "k_wave", k_wave<0>, g, blk2, p, s, c->f); break;
+        case 12:
+            LAUNCH(c, "k_wave", k_wave<12>, gw, bw, p, s, c->f);
+            LAUNCH(c, "k_wave_rows", k_wave_rows<12>, gr, dim3(256), p, s, c->f);
+            break;
+        case 24:
+            LAUNCH(c, "k_wave", k_wave<24>, gw, bw, p, s, c->f);
+            LAUNCH(c, "k_wave_rows", k_wave_rows<24>, gr, dim3(256), p, s, c->f);
+            break;
+        default:
+            LAUNCH(c, "k_wave", k_wave<0>, gw, bw, p, s, c->f);
+            LAUNCH(c, "k_wave_rows", k_wave_rows<0>, gr, dim3(256), p, s, c->f);
+            break;
     }
     LAUNCH(c, "k_wave_force", k_wave_force, g, blk2, p, s, c->f);
     c->cur = 1 - c->cur;
@@ -1620,12 +1770,13 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
                      &c->Hmean, &c->uumean, &c->vvmean, &c->hhmean, &c->zsmean, &c->zsmax, &c->Cmean};
     for (auto** q : two) { A(q, N); }
     A(&f.umean, ny);
+    A(&f.bnd_e1, 2 * (size_t)P.ntheta * P.nx); A(&f.bnd_r1, 2 * (size_t)P.ntheta * P.nx);
     A(&f.wet, N);
     if (P.zb_accum == 2) { A(&f.zb64, N); A(&f.zb64_b, N); }
-    c->nblk_wave = ((P.nx + BX - 1) / BX) * ((P.ny + BY - 1) / BY);
+    c->nblk_wave = ((P.nx + BX - 1) / BX) * ((P.ny + WTY - 1) / WTY);
     c->nblk_flow = (int)((N + 255) / 256);
     c->nblk_sum = 256;
-    A(&f.red_wave, c->nblk_wave); A(&f.red_flow, c->nblk_flow); A(&f.red_bad, 4); A(&f.red_out, 4);
+    A(&f.red_wave, c->nblk_wave); A(&f.red_flow, c->nblk_flow); A(&f.red_bad, 4); A(&f.red_acc, 8);
     A(&f.red_sum, 2 * c->nblk_sum);
 #undef A
     e = hipHostMalloc((void**)&c->host_red, 4 * sizeof(float), hipHostMallocDefault);
@@ -1695,6 +1846,11 @@ int xbg_init_state(xbg_ctx* c, double Trep) {
     // the first step's wave CFL minimum is not used (totaltime == 0)
     dim3 g1((c->nblk_wave + 255) / 256);
     LAUNCH(c, "k_fill", k_fill, g1, dim3(256), c->f.red_wave, (size_t)c->nblk_wave, 9999999.9f);
+    {
+        unsigned h[8] = {0x7f7fffffu, 0x7f7fffffu, 0u, 0u, 0x7f7fffffu, 0x7f7fffffu, 0u, 0u};
+        HIPC(c, hipMemcpyAsync(c->f.red_acc, h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+    }
     HIPC(c, hipStreamSynchronize(c->stream));
     return XBG_OK;
 }
@@ -1817,14 +1973,21 @@ int xbg_step(xbg_ctx* c, double dt, double totaltime) {
 int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
     if (!c) return XBG_E_ARG;
     float cfl = initial ? 0.5f * c->kp.CFL : c->kp.CFL;
-    LAUNCH(c, "k_cfl_flow", k_cfl_flow, dim3(c->nblk_flow), dim3(256), c->kp, c->f, cfl);
-    LAUNCH(c, "k_cfl_final", k_cfl_final, dim3(1), dim3(256), c->f, c->nblk_flow, c->nblk_wave);
-    HIPC(c, hipMemcpyAsync(c->host_red, c->f.red_out, 3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    unsigned* acc = c->f.red_acc + 4 * c->acc_cur;
+    unsigned* other = c->f.red_acc + 4 * (1 - c->acc_cur);
+    int nb = c->nblk_flow < 512 ? c->nblk_flow : 512;
+    LAUNCH(c, "k_cfl", k_cfl, dim3(nb), dim3(256), c->kp, c->f, cfl, c->nblk_wave, acc, other);
+    HIPC(c, hipMemcpyAsync(c->host_red, acc, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    double fm = (double)c->host_red[0];
-    if (c->host_red[2] != 0.0f) fm = NAN;
+    c->acc_cur = 1 - c->acc_cur;
+    unsigned* hb = (unsigned*)c->host_red;
+    float fmv, wmv;
+    std::memcpy(&fmv, &hb[0], 4);
+    std::memcpy(&wmv, &hb[1], 4);
+    double fm = (double)fmv;
+    if (hb[2] != 0u) fm = NAN;
     if (dt_flow) *dt_flow = fm;
-    if (dt_wave) *dt_wave = (double)c->host_red[1];
+    if (dt_wave) *dt_wave = (double)wmv;
     if (!(fm == fm) || fm == 0.0 || std::isinf(fm))
         return fail(c, XBG_E_CRASH, "Model crashed: non-finite or zero flow time step");
     return XBG_OK;
