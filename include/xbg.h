@@ -26,8 +26,7 @@
  * One context is used by one host thread.  Each context owns one HIP stream
  * and all of its device memory, allocated once in xbg_create; no call after
  * xbg_create allocates device memory, except xbg_set_bnd_series (which sizes
- * the resident boundary record buffers) and xbg_graph_step (which builds its
- * HIP graph once per dt/time signature change, not per step).
+ * the resident boundary record buffers once per series).
  */
 #ifndef XBG_H
 #define XBG_H

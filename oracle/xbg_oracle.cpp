@@ -58,6 +58,7 @@
 #include <vector>
 
 #include "../include/xbg.h"
+#include "../xbeach_gpu_amd/csrc/xbg_math.h"
 
 namespace {
 
@@ -74,31 +75,36 @@ inline int pp1(int i, int n) { return i >= n - 1 ? n - 1 : i + 1; }
 inline int mm2(int i, int n) { (void)n; return i <= 1 ? 0 : i - 2; }
 inline int pp2(int i, int n) { return i >= n - 2 ? n - 1 : i + 2; }
 
-// Math in the working precision: float calls the f-suffixed libm routine
-// the reference calls through the float overloads of <cmath>.
+// Math in the working precision.  The reference calls the host libm on its
+// CPU path (float overloads of <cmath>) and the CUDA device library on its
+// GPU path, which disagree in the last bit; the oracle and the HIP kernels
+// both use the deterministic functions of xbg_math.h (double-evaluated,
+// within 2 float ulps of glibc, see tests/test_xmath.py) so that the HIP
+// step can be checked bit for bit.  sqrt, division and the arithmetic are
+// IEEE correctly rounded on both sides.
 template <class T> struct Fm;
 template <> struct Fm<float> {
     static float sqrt(float x) { return sqrtf(x); }
-    static float exp(float x) { return expf(x); }
-    static float log(float x) { return logf(x); }
-    static float log10(float x) { return log10f(x); }
-    static float pow(float x, float y) { return powf(x, y); }
-    static float sinh(float x) { return sinhf(x); }
-    static float tanh(float x) { return tanhf(x); }
-    static float cos(float x) { return cosf(x); }
-    static float sin(float x) { return sinf(x); }
+    static float exp(float x) { return xm::expf(x); }
+    static float log(float x) { return xm::logf(x); }
+    static float log10(float x) { return xm::log10f(x); }
+    static float pow(float x, float y) { return xm::powf(x, y); }
+    static float sinh(float x) { return xm::sinhf(x); }
+    static float tanh(float x) { return xm::tanhf(x); }
+    static float cos(float x) { return xm::cosf(x); }
+    static float sin(float x) { return xm::sinf(x); }
     static float fabs(float x) { return fabsf(x); }
 };
 template <> struct Fm<double> {
     static double sqrt(double x) { return std::sqrt(x); }
-    static double exp(double x) { return std::exp(x); }
-    static double log(double x) { return std::log(x); }
-    static double log10(double x) { return std::log10(x); }
-    static double pow(double x, double y) { return std::pow(x, y); }
-    static double sinh(double x) { return std::sinh(x); }
-    static double tanh(double x) { return std::tanh(x); }
-    static double cos(double x) { return std::cos(x); }
-    static double sin(double x) { return std::sin(x); }
+    static double exp(double x) { return xm::exp(x); }
+    static double log(double x) { return xm::log(x); }
+    static double log10(double x) { return xm::log10(x); }
+    static double pow(double x, double y) { return xm::pow(x, y); }
+    static double sinh(double x) { return xm::sinh(x); }
+    static double tanh(double x) { return xm::tanh(x); }
+    static double cos(double x) { return xm::cos(x); }
+    static double sin(double x) { return xm::sin(x); }
     static double fabs(double x) { return std::fabs(x); }
 };
 
@@ -249,7 +255,7 @@ template <class T> struct Dom {
             T L0 = twopi * g / (sigm[i] * sigm[i]);
             T L1 = L0, L2, err;
             for (int n = 1; n < 200; n++) {
-                L2 = (T)(D(L0) * std::tanh(2 * PI_ * D(hh[i]) / D(L1)));
+                L2 = (T)(D(L0) * xm::tanh(2 * PI_ * D(hh[i]) / D(L1)));
                 err = F::fabs(L2 - L1);
                 L1 = (L1 * aphi + L2 * bphi);
                 if (err <= T(0.0001f)) break;
@@ -475,7 +481,7 @@ template <class T> struct Dom {
             T h = hh[i];
             T et = E[i];
             T Ht = F::sqrt(et * fac);
-            T Qb = (T)(1.0 - std::exp(tmax(-1.0 * D(F::pow(Ht / gammaa / h, nexp)), -100.0)));
+            T Qb = (T)(1.0 - xm::exp(tmax(-1.0 * D(F::pow(Ht / gammaa / h, nexp)), -100.0)));
             Qb = tmin(Qb, T(1.0f));
             Dd[i] = (T)(D(Qb) * 2.0 * D(alpha) / D(Trep) * D(et) * D(Ht) / D(h));
             T uorb = (T)(PI_ * D(Ht) / D(Trep * F::sinh(tmin(k[i] * h, T(10.0f)))));
@@ -1697,3 +1703,37 @@ int orc_cfl(void* hv, int initial, double* dtflow, double* dtwave) {
 }
 
 }  // extern "C"
+
+// Elementary functions of xbg_math.h over an array, for the tests that pin
+// them against the host libm.
+extern "C" int orc_math(const char* name, int is_float, const double* x, const double* y, double* out, int n) {
+    std::string s(name);
+    for (int i = 0; i < n; i++) {
+        double a = x[i], b = y ? y[i] : 0.0, r;
+        if (is_float) {
+            float af = (float)a, bf = (float)b;
+            if (s == "exp") r = xm::expf(af);
+            else if (s == "log") r = xm::logf(af);
+            else if (s == "log10") r = xm::log10f(af);
+            else if (s == "pow") r = xm::powf(af, bf);
+            else if (s == "sinh") r = xm::sinhf(af);
+            else if (s == "tanh") r = xm::tanhf(af);
+            else if (s == "sin") r = xm::sinf(af);
+            else if (s == "cos") r = xm::cosf(af);
+            else return -1;
+        } else {
+            if (s == "exp") r = xm::exp(a);
+            else if (s == "log") r = xm::log(a);
+            else if (s == "log10") r = xm::log10(a);
+            else if (s == "pow") r = xm::pow(a, b);
+            else if (s == "sinh") r = xm::sinh(a);
+            else if (s == "tanh") r = xm::tanh(a);
+            else if (s == "sin") r = xm::sin(a);
+            else if (s == "cos") r = xm::cos(a);
+            else if (s == "expm1") r = xm::expm1(a);
+            else return -1;
+        }
+        out[i] = r;
+    }
+    return 0;
+}

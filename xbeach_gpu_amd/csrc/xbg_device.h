@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "xbg_math.h"
+
 #define XDEV __device__ __forceinline__
 
 namespace xbg {
@@ -56,12 +58,12 @@ XDEV void dispersion_cell(const KP& p, float sig, float h, float& k, float& c, f
     float L0 = p.twopi * p.g / (sig * sig);
     float L1 = L0, L2, err;
     for (int n = 1; n < 200; n++) {
-        L2 = (float)((double)L0 * tanh(2 * PI_ * (double)h / (double)L1));
+        L2 = (float)((double)L0 * xm::tanh(2 * PI_ * (double)h / (double)L1));
         err = fabsf(L2 - L1);
         L1 = (L1 * p.aphi + L2 * p.bphi);
         if (err <= 0.0001f) break;
         if (n == 199) {
-            L1 = L0 * powf(tanhf(powf(sig * sig * h / p.g, 3.0f / 4.0f)), 2.0f / 3.0f);
+            L1 = L0 * xm::powf(xm::tanhf(xm::powf(sig * sig * h / p.g, 3.0f / 4.0f)), 2.0f / 3.0f);
             break;
         }
     }
@@ -70,7 +72,7 @@ XDEV void dispersion_cell(const KP& p, float sig, float h, float& k, float& c, f
     float cc = sig / kk;
     c = cc;
     float kkhh = fmn(kk * h, 10.0f);
-    float s2kh = sinhf(2.0f * kkhh);
+    float s2kh = xm::sinhf(2.0f * kkhh);
     cg = cc * (0.5f + kkhh / s2kh);
 }
 

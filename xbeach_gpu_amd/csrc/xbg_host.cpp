@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "../../include/xbg.h"
+#include "xbg_math.h"
 
 namespace {
 const double PI_ = 3.14159265;  // XBeachGPU.h:4
@@ -61,8 +62,8 @@ extern "C" void xbg_theta_tables(const xbg_params* p, float* theta, float* cxsth
     for (int i = 0; i < p->ntheta; i++) {
         float t = (float)(i * (p->dtheta) + p->thetamin + 0.5f * p->dtheta);
         if (theta) theta[i] = t;
-        if (cxsth) cxsth[i] = cosf(t);
-        if (sxnth) sxnth[i] = sinf(t);
+        if (cxsth) cxsth[i] = xm::cosf(t);
+        if (sxnth) sxnth[i] = xm::sinf(t);
     }
 }
 

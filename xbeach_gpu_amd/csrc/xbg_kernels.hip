@@ -177,7 +177,7 @@ XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
     r.dvdy = (f.vv[it] - f.vv[ib]) / ddy;
     float h = f.hh[i];
     float kh = fmn(f.k[i] * h, 10.0f);
-    r.sig = s.sigm / sinhf(2.0f * kh);
+    r.sig = s.sigm / xm::sinhf(2.0f * kh);
     r.w = p.wci * fmn(h / p.hwci, 1.0f);
     r.lo = -0.25f * s.sigm;
     r.hi = 0.25f * s.sigm;
@@ -277,10 +277,10 @@ XDEV void theta_euler(const KP& p, const SP& s, const Refr& rf, const float* e0,
 XDEV float roelvink_cell(const KP& p, float Trep, float E, float h, float k, float fw) {
     float fac = (float)(8.0 / (double)p.rho / (double)p.g);
     float Ht = sqrtf(E * fac);
-    float Qb = (float)(1.0 - exp(dmx(-1.0 * (double)powf(Ht / p.gammaa / h, p.nexp), -100.0)));
+    float Qb = (float)(1.0 - xm::exp(dmx(-1.0 * (double)xm::powf(Ht / p.gammaa / h, p.nexp), -100.0)));
     Qb = fmn(Qb, 1.0f);
     float Dv = (float)((double)Qb * 2.0 * (double)p.alpha / (double)Trep * (double)E * (double)Ht / (double)h);
-    float uorb = (float)(PI_ * (double)Ht / (double)(Trep * sinhf(fmn(k * h, 10.0f))));
+    float uorb = (float)(PI_ * (double)Ht / (double)(Trep * xm::sinhf(fmn(k * h, 10.0f))));
     return (float)((double)Dv + (double)0.66666666f / PI_ * (double)p.rho * (double)fw * (double)uorb *
                    (double)uorb * (double)uorb);
 }
@@ -288,11 +288,11 @@ XDEV float roelvink_cell(const KP& p, float Trep, float E, float h, float k, flo
 // baldockCPU (wave_CPU.cpp:987-1027), breakmodel != 1.
 XDEV float baldock_cell(const KP& p, float Trep, float Ht, float h, float k, float fw) {
     float khh = k * h;
-    float Hb = (0.88f / k) * tanhf(p.gammaa * khh / 0.88f);
+    float Hb = (0.88f / k) * xm::tanhf(p.gammaa * khh / 0.88f);
     float Rr = Hb / fmx(Ht, 0.00001f);
-    float Qb = expf(-1.0f * Rr * Rr);
+    float Qb = xm::expf(-1.0f * Rr * Rr);
     float Dv = 0.25f * p.alpha * Qb * p.rho * (1.0f / Trep) * p.g * (Hb * Hb + Ht * Ht);
-    float uorb = (float)(PI_ * (double)Ht / (double)(Trep * sinhf(fmn(k * h, 10.0f))));
+    float uorb = (float)(PI_ * (double)Ht / (double)(Trep * xm::sinhf(fmn(k * h, 10.0f))));
     return (float)((double)Dv + (double)0.66666666f / PI_ * (double)p.rho * (double)fw * (double)uorb *
                    (double)uorb * (double)uorb);
 }
@@ -314,7 +314,7 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     }
     Et = Et * p.dtheta;
     float Ht = sqrtf(Et / (p.rho * p.g / 8.0f));
-    float idiv = fmx(1.0f, powf(Ht / (p.gammax * h), 2.0f));
+    float idiv = fmx(1.0f, xm::powf(Ht / (p.gammax * h), 2.0f));
     Ht = fmn(Ht, p.gammax * h);
     float E = (p.rho * p.g * Ht * Ht) / 8.0f;
     float kk = f.k[i];
@@ -394,7 +394,7 @@ XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const fl
     float hmin = 0.2f;
     float hhi = fmx(f.hh[i], hmin);
     float cib = fmx(ci, sqrtf(hmin * p.g));
-    float uorb = (float)(PI_ * (double)Hm / (double)s.Trep / (double)sinhf(fmn(fmx(f.k[i], 0.01f) * hhi, 10.0f)));
+    float uorb = (float)(PI_ * (double)Hm / (double)s.Trep / (double)xm::sinhf(fmn(fmx(f.k[i], 0.01f) * hhi, 10.0f)));
     f.urms[i] = uorb / (sqrtf(2.0f));
     float ustw = Em / cib / p.rho / hhi;
     float ustr = 2.0f * Rv / cib / p.rho / hhi;
@@ -685,7 +685,7 @@ __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f) {
     if (f.hu[i] > p.eps && humi > p.eps) m |= 1;
     if (f.hv[i] > p.eps && hvmi > p.eps) m |= 2;
     f.wet[i] = m;
-    float turb = p.nuhfac * h * powf(f.DR[i] / p.rho, 1.0f / 3.0f);
+    float turb = p.nuhfac * h * xm::powf(f.DR[i] / p.rho, 1.0f / 3.0f);
     if (p.usesmago == 1) {
         const float* uu = f.uu;
         const float* vv = f.vv;
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f) {
         float dudy = 0.50f * (uu[ix2(p, x, yp)] - uu[ix2(p, x, ym)] + uu[ix2(p, xm, yp)] - uu[ix2(p, xm, ym)]) / p.dx;
         float dvdy = (vv[i] - vv[ix2(p, x, ym)]) / p.dx;
         float dvdx = 0.50f * (vv[ix2(p, xp, y)] - vv[ix2(p, xm, y)] + vv[ix2(p, xp, ym)] - vv[ix2(p, xm, ym)]) / p.dx;
-        float tau = sqrtf(2.0f * dudx * dudx + 2.0f * dvdy * dvdy + powf(dudy + dvdx, 2.0f));
+        float tau = sqrtf(2.0f * dudx * dudx + 2.0f * dvdy * dvdy + xm::powf(dudy + dvdx, 2.0f));
         f.nuh[i] = fmx(p.nuh * p.nuh * p.dx * p.dx * tau, turb);
     } else {
         f.nuh[i] = fmx(p.nuh, turb);
@@ -1042,7 +1042,7 @@ XDEV void calcuvvu_at(const KP& p, const Fld& f, int x, int y, float* o) {
     float vu = 0.25f * (vv[ix2(p, x, ym)] + vv[i] + vv[ix2(p, xp, ym)] + vv[r]) * wu;
     float uv = 0.25f * (uu[ix2(p, xm, y)] + uu[i] + uu[ix2(p, xm, yp)] + uu[t]) * wv;
     float tmi = f.thetamean[i], tmr = f.thetamean[r], tmt = f.thetamean[t];
-    float si = sinf(tmi), ci = cosf(tmi), sr = sinf(tmr), cr = cosf(tmr), st = sinf(tmt), ctt = cosf(tmt);
+    float si = xm::sinf(tmi), ci = xm::cosf(tmi), sr = xm::sinf(tmr), cr = xm::cosf(tmr), st = xm::sinf(tmt), ctt = xm::cosf(tmt);
     float usi = f.ust[i], usr = f.ust[r], ust_t = f.ust[t];
     float vsu = 0.5f * (usi * si + usr * sr) * wu;
     float usu = 0.5f * (usi * ci + usr * cr) * wu;
@@ -1085,15 +1085,15 @@ __global__ __launch_bounds__(256) void k_flow_uvvu(KP p, Fld f) {
 // Rvr (Sediment_kernel.cu:477-510) at cell i
 XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
     const float m1c = 0.0f, m2c = 0.7939f, m3c = -0.6065f, m4c = 0.3539f, m5c = 0.6373f, m6c = 0.5995f;
-    float alph = -1.0f * log10f(expf(1.0f)) / m4c;
-    float bet = expf(m3c / m4c);
+    float alph = -1.0f * xm::log10f(xm::expf(1.0f)) / m4c;
+    float bet = xm::expf(m3c / m4c);
     float kk = (float)(2 * PI_ / (double)(f.c[i] * s.Trep));
-    float Ur = 3.0f / 8.0f * sqrtf(2.0f) * f.H[i] * kk / powf(kk * f.hh[i], 3.0f);
+    float Ur = 3.0f / 8.0f * sqrtf(2.0f) * f.H[i] * kk / xm::powf(kk * f.hh[i], 3.0f);
     Ur = fmx(Ur, 0.00001f);
-    float Bm = m1c + (m2c - m1c) / (1.0f + bet * powf(Ur, alph));
-    float B1 = (float)((double)(-90.0f + 90.0f * tanhf(m5c / powf(Ur, m6c))) * PI_ / 180.0);
-    float Sk = Bm * cosf(B1);
-    float As = Bm * sinf(B1);
+    float Bm = m1c + (m2c - m1c) / (1.0f + bet * xm::powf(Ur, alph));
+    float B1 = (float)((double)(-90.0f + 90.0f * xm::tanhf(m5c / xm::powf(Ur, m6c))) * PI_ / 180.0);
+    float Sk = Bm * xm::cosf(B1);
+    float As = Bm * xm::sinf(B1);
     return (p.facsk * Sk - p.facas * As) * f.urms[i];
 }
 
@@ -1111,15 +1111,15 @@ __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f) {
     float uai = rvr_at(p, s, f, i), uar = rvr_at(p, s, f, r), uat = rvr_at(p, s, f, t);
     float cci = f.C[i], ccr = f.C[r], cct = f.C[t];
     float tm = f.thetamean[i];
-    float ctm = cosf(tm), stm = sinf(tm);
+    float ctm = xm::cosf(tm), stm = xm::sinf(tm);
     float uau = (float)(0.5 * (double)ctm * (double)(uai + uar));
     float uav = (float)(0.5 * (double)stm * (double)(uar + uai));
     float urep = f.ueu[i] + uau;
-    float vmagu = sqrtf(powf((f.uu[i] + uau), 2.0f) + powf((f.vu[i] + uav), 2.0f));
+    float vmagu = sqrtf(xm::powf((f.uu[i] + uau), 2.0f) + xm::powf((f.vu[i] + uav), 2.0f));
     uau = (float)(0.5 * (double)ctm * (double)(uai + uat));
     uav = (float)(0.5 * (double)stm * (double)(uat + uai));
     float vrep = f.vev[i] + uav;
-    float vmagv = sqrtf(powf(f.uv[i] + uau, 2.0f) + powf(f.vv[i] + uav, 2.0f));
+    float vmagv = sqrtf(xm::powf(f.uv[i] + uau, 2.0f) + xm::powf(f.vv[i] + uav, 2.0f));
     float dzbdx = -1.0f * (f.zb[r] - f.zb[i]) / p.dx;
     float dzbdy = -1.0f * (f.zb[t] - f.zb[i]) / p.dx;
     float hu = f.hu[i], hv = f.hv[i];
@@ -1133,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f) {
     else if (vrep < 0.0f) cv = cct;
     else cv = 0.50f * (cci + cct);
     float dcsdy = (cct - cci) / p.dx;
-    float Dc = p.nuh + p.nuhfac * f.hh[i] * powf(f.DR[i] / p.rho, 1.0f / 3.0f);
+    float Dc = p.nuh + p.nuhfac * f.hh[i] * xm::powf(f.DR[i] / p.rho, 1.0f / 3.0f);
     const float facsl = 1.6f;
     f.Sus[i] = p.sus * (cu * urep * hu - Dc * hu * dcsdx - facsl * cu * vmagu * hu * dzbdx) * wu;
     f.Svs[i] = p.sus * (cv * vrep * hv - Dc * hv * dcsdy - facsl * cv * vmagv * hv * dzbdy) * wv;
@@ -1148,31 +1148,31 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     float h = f.hh[i];
     float hhi = fmx(h, 0.01f);
     float drho = (p.rhosed - p.rho) / p.rho;
-    float dester = powf(drho * p.g, 1.0f / 3.0f) / 0.0001f * p.D50;
+    float dester = xm::powf(drho * p.g, 1.0f / 3.0f) / 0.0001f * p.D50;
     float ue = (float)(0.5 * (double)(f.ueu[ix2(p, m1(x), y)] + f.ueu[i]));
     float ve = (float)(0.5 * (double)(f.vev[ix2(p, x, m1(y))] + f.vev[i]));
     float vmags = ue * ue + ve * ve;
     float ML = fmx(fmn(sqrtf(2.0f * f.R[i] * s.Trep / (p.rho * f.c[i])), hhi), 0.01f);
-    float dcfin = expf(fmn(hhi, 100.0f) / ML);
+    float dcfin = xm::expf(fmn(hhi, 100.0f) / ML);
     float dcf = fmn(1 / (dcfin - 1), 1.0f);
-    float kb = p.nuhfac * powf(f.DR[i] / p.rho, 0.66666666667f) * dcf;
+    float kb = p.nuhfac * xm::powf(f.DR[i] / p.rho, 0.66666666667f) * dcf;
     float Urms2 = f.urms[i] * f.urms[i] + 1.45f * (kb + 0.0f);
     float Tsg = fmx(0.1f * hhi / p.wws, 0.5f);
     float Ucr;
-    if (p.D50 <= 0.0005f) Ucr = 0.19f * powf(p.D50, 0.1f) * log10f(4.0f * hhi / p.D90);
-    else Ucr = 8.50f * powf(p.D50, 0.6f) * log10f(4.0f * hhi / p.D90);
+    if (p.D50 <= 0.0005f) Ucr = 0.19f * xm::powf(p.D50, 0.1f) * xm::log10f(4.0f * hhi / p.D90);
+    else Ucr = 8.50f * xm::powf(p.D50, 0.6f) * xm::log10f(4.0f * hhi / p.D90);
     float hdrag = fmx(hhi, 10.0f * zo);
-    float Cdd = 0.4f / (logf(hdrag / zo) - 1.0f);
+    float Cdd = 0.4f / (xm::logf(hdrag / zo) - 1.0f);
     Cdd = Cdd * Cdd;
-    float Asb = 0.005f * hhi * powf(1 / hhi / (drho * p.g), 1.2f);
-    float Ass = 0.012f * p.D50 * powf(dester, -0.6f) / (powf(drho * p.g * p.D50, 1.2f));
+    float Asb = 0.005f * hhi * xm::powf(1 / hhi / (drho * p.g), 1.2f);
+    float Ass = 0.012f * p.D50 * xm::powf(dester, -0.6f) / (xm::powf(drho * p.g * p.D50, 1.2f));
     float cfm = cfm_of(p, f.stdep[i]);
     float T1 = vmags + 0.018f / Cdd * Urms2;
     T1 = fmn(T1, 100000.0f * p.g / cfm * p.D50 * drho);
     T1 = sqrtf(T1);
     float wetz = hhi > p.eps ? 1.0f : 0.0f;
     float T2 = 0.0f;
-    if (T1 > Ucr && hhi > p.eps) T2 = powf(T1 - Ucr, 2.4f);
+    if (T1 > Ucr && hhi > p.eps) T2 = xm::powf(T1 - Ucr, 2.4f);
     float ceqb = Asb * T2;
     ceqb = fmn(ceqb / hhi, 0.05f);
     float ceqbg = (1 - bulk) * ceqb * sedcal * wetz;
