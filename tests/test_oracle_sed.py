@@ -168,3 +168,95 @@ def test_longturb_roller_thickness():
     c = math.sqrt(G * 3.0)
     assert np.allclose(o.get("rolthick"), max(0.5 * (0.4 - p.beta * c), 0.0))
     assert (o.get("kturb") == 0).all()
+
+
+def susp_case(p, o, **fields):
+    N = (p.ny, p.nx)
+    base = dict(C=np.full(N, 2e-3), ceqbg=np.zeros(N), ua=np.zeros(N), thetamean=np.zeros(N),
+                ueu=np.zeros(N), vev=np.zeros(N), uu=np.zeros(N), vv=np.zeros(N), vu=np.zeros(N),
+                uv=np.zeros(N), hu=np.full(N, 2.0), hv=np.full(N, 2.5), DR=np.zeros(N),
+                zb=np.full(N, 3.0))
+    base.update(fields)
+    for k, v in base.items():
+        o.set(k, v)
+    o.stage("Susp")
+    return o.get("Sus"), o.get("Svs")
+
+
+def test_susp_advective_diffusive_and_slope_fluxes():
+    """Sediment_kernel.cu:534-717: S = C_up u h - Dc h dC/dx
+    - 1.6 C_up |u| h dzb/dx (with dzb/dx = -(zb_{i+1} - zb_i)/dx), per face."""
+    p, o = sed_domain(nx=16, ny=8)
+    N = (p.ny, p.nx)
+    y, x = np.mgrid[0:p.ny, 0:p.nx].astype(np.float64) * p.dx
+    # advection: uniform C, u, v
+    Sus, Svs = susp_case(p, o, ueu=np.full(N, 0.4), vev=np.full(N, -0.3))
+    assert np.allclose(Sus, p.sus * 2e-3 * 0.4 * 2.0, rtol=1e-12)
+    assert np.allclose(Svs, p.sus * 2e-3 * -0.3 * 2.5, rtol=1e-12)
+    # upwinding: a step in C, the face value comes from the upstream cell
+    C = np.where(x < 5 * p.dx, 1e-3, 4e-3)
+    for u0, cface in ((0.4, 1e-3), (-0.4, 4e-3)):
+        Sus, _ = susp_case(p, o, C=C, ueu=np.full(N, u0))
+        # face between ix = 4 and 5; diffusion adds -Dc h dC/dx, Dc = nuh
+        want = p.sus * (cface * u0 * 2.0 - p.nuh * 2.0 * 3e-3 / p.dx)
+        assert np.allclose(Sus[:, 4], want, rtol=1e-12)
+    # diffusion with the breaking-induced part: Dc = nuh + nuhfac h (DR/rho)^(1/3)
+    Cl = 1e-3 + 1e-4 * x
+    DR = np.full(N, 50.0)
+    Sus, _ = susp_case(p, o, C=Cl, DR=DR)
+    Dc = p.nuh + p.nuhfac * 3.0 * (50.0 / RHO) ** (1 / 3)
+    assert np.allclose(Sus[:, :-1], -p.sus * Dc * 2.0 * 1e-4, rtol=1e-9)
+    # bed slope term: zb rising with x (bed deepening seaward is negative dzb/dx)
+    s = 0.01
+    Sus, _ = susp_case(p, o, zb=3.0 + s * x, uu=np.full(N, -0.5))
+    assert np.allclose(Sus[:, :-1], -p.sus * 1.6 * 2e-3 * 0.5 * 2.0 * (-s), rtol=1e-9)
+    # dry faces carry nothing
+    hu = np.full(N, 2.0)
+    hu[:, 3] = p.eps / 2
+    Sus, _ = susp_case(p, o, ueu=np.full(N, 0.4), hu=hu)
+    assert (Sus[:, 3] == 0).all() and (Sus[:, 2] != 0).all()
+
+
+def test_conc_equilibrium_is_a_fixed_point_and_flux_divergence():
+    """Sediment_kernel.cu:721-779 with facero = 1: C = ceq is stationary
+    (ero = h ceq / Ts balances depo); a flux through one face changes cs by
+    -dt Ts/(dt+Ts) dS/dx."""
+    p, o = sed_domain(nx=12, ny=6)
+    N = (p.ny, p.nx)
+    dt, h, ceq, Ts = 0.4, 3.0, 3e-3, 2.0
+    for k, v in dict(C=np.full(N, ceq), ceqsg=np.full(N, ceq), Tsg=np.full(N, Ts), facero=np.ones(N),
+                     Sus=np.zeros(N), Svs=np.zeros(N)).items():
+        o.set(k, v)
+    o.stage("Conc", dt=dt)
+    assert np.allclose(o.get("C"), ceq, rtol=1e-12)
+    assert np.allclose(o.get("ero"), h * ceq / Ts, rtol=1e-12)
+    assert np.allclose(o.get("depo"), h * ceq / Ts, rtol=1e-12)
+    Sus = np.zeros(N)
+    Sus[:, 6:] = 1e-4              # divergence only at ix = 6
+    for k, v in dict(C=np.full(N, ceq), Sus=Sus).items():
+        o.set(k, v)
+    o.stage("Conc", dt=dt)
+    C = o.get("C")
+    assert np.allclose(C[:, 6], ceq - dt * Ts / (dt + Ts) * 1e-4 / p.dx / h, rtol=1e-12)
+    C[:, 6] = ceq
+    assert np.allclose(C, ceq, rtol=1e-12)
+    # clamps: cs in [0, 0.1 h]
+    o.set("C", np.full(N, 0.5))
+    o.set("Sus", np.zeros(N))
+    o.stage("Conc", dt=dt)
+    assert np.allclose(o.get("C"), 0.1, rtol=1e-12)
+
+
+def test_erosus_limits_erosion_to_the_available_sediment():
+    """Sediment_kernel.cu:512-531: facero = min(1, stdep / expected
+    erosion), expected = morfac dt/(1-por) h ceq/Ts."""
+    p, o = sed_domain(nx=8, ny=4, morfac=10.0)
+    N = (p.ny, p.nx)
+    dt, h, ceq, Ts = 0.5, 3.0, 0.02, 1.0
+    st = np.broadcast_to(np.linspace(0.0, 0.5, p.nx), N).copy()
+    for k, v in dict(ceqsg=np.full(N, ceq), ceqbg=np.zeros(N), Tsg=np.full(N, Ts), stdep=st).items():
+        o.set(k, v)
+    o.stage("Erosus", dt=dt)
+    exp_ero = 10.0 * dt / (1 - p.por) * h * ceq / Ts
+    assert np.allclose(o.get("facero"), np.minimum(1.0, st / exp_ero), rtol=1e-12)
+    assert (o.get("facero")[:, -1] == 1).all() and (o.get("facero")[:, 0] == 0).all()
