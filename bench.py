@@ -122,6 +122,19 @@ def cpu_baseline(p, budget_s=20.0):
             "sample": f"{n} whole steps of the {p.nx}x{p.ny}x{p.ntheta} domain, oracle FP32, 1 thread"}
 
 
+def copy_calibration(nbytes=1 << 30):
+    """A plain copy kernel of known size: each launch reads and writes
+    nbytes once."""
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,7 +143,13 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "small"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (the reference "
+                         "reading for the FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
               file=sys.stderr)

@@ -172,10 +172,12 @@ int xbg_set_flow_forcing(xbg_ctx* ctx, const float* qbndold, const float* qbndne
  * Resident boundary records (no counterpart in the reference, which copies
  * St and qbnd host->device every step, Wavestep.cu:749-779): upload a whole
  * series of nrec records once (Stfile: nrec*ny*ntheta, qfile: nrec*4*ny,
- * record-major as the reference Stfile/qfile, Wavestep.cu:722-737), then
- * select and time-interpolate on the device per step.
+ * record-major and in double as the reference Stfile/qfile,
+ * Wavestep.cu:61-68, 722-737), then select and time-interpolate on the
+ * device per step: St in double rounded to float, qbnd rounded to float, as
+ * wavebnd does on the host.
  */
-int xbg_set_bnd_series(xbg_ctx* ctx, const float* Stfile, const float* qfile, int nrec);
+int xbg_set_bnd_series(xbg_ctx* ctx, const double* Stfile, const double* qfile, int nrec);
 int xbg_use_bnd_record(xbg_ctx* ctx, int rec, double timesincelast, double timenext,
                        double Trep, double zsbndi, double zsbndn, double windu, double windv);
 
@@ -213,6 +215,67 @@ int xbg_divide_means(xbg_ctx* ctx, double nstep);
  * (compensated device reduction; no counterpart in the reference).
  */
 int xbg_mass_balance(xbg_ctx* ctx, double* water_volume, double* bed_volume);
+
+/* ------------------------------------------------------------------------
+ * Offshore wave boundary generation (host code; makjonswap.cpp)
+ * ---------------------------------------------------------------------- */
+
+/*
+ * One boundary record of the wave-boundary list (Wavebndparam,
+ * XBeachGPU.h:198-210): time, significant height Hs (m), peak period Tp (s),
+ * main direction Dp (radians in the grid convention, i.e. already converted
+ * from nautical degrees as read_input.cpp:516-522 does:
+ * Dp = (1.5 pi - grdalpha) - Dp_naut * pi/180), directional spreading s and
+ * peak enhancement gamma.
+ */
+typedef struct xbg_wavebnd {
+    double time, Hs, Tp, Dp, s, gamma;
+} xbg_wavebnd;
+
+/* Sizes of the JONSWAP spectrum of makjonswap (makjonswap.cpp:79-98, 167):
+ * nfreq frequencies (fp/20 spacing up to 3 fp), 90 directions. */
+int xbg_jonswap_size(const xbg_wavebnd* w, int* nfreq, int* ndir);
+
+/* makjonswap (makjonswap.cpp:73-233): freq[nfreq], dir[ndir] (full circle
+ * from -pi), spec[f + d*nfreq] in m2/Hz/rad with 4 sqrt(m0) = Hs and a
+ * cos^2s directional distribution normalised over the circle. */
+int xbg_jonswap(const xbg_wavebnd* w, double* freq, double* dir, double* spec);
+
+/* Records per generated boundary block, ceil(rtlength/dtbc) + 1
+ * (makjonswap.cpp:470). */
+int xbg_bnd_nrec(const xbg_params* p);
+
+/*
+ * GenWGnLBW (makjonswap.cpp:235-1419): wave groups and bound long waves
+ * from a 2D spectrum, for one block of rtlength seconds at dtbc spacing.
+ * Random phases and directions come from std::default_random_engine(seed)
+ * as in the reference (seed 0 when random == 0, makjonswap.cpp:363-383).
+ * offdepth is the offshore depth (Wave_gpu.cu:2017).  Outputs, in the
+ * reference layouts and in double:
+ *   Trep    representative period (rounded to float as the reference holds it)
+ *   Stfile  nrec*ntheta*ny   [j + itheta*ny + m*ny*ntheta]
+ *   qfile   nrec*4*ny        [j + k*ny + m*ny*4] (k = 0 qx, 1 qy, 2-3 zero)
+ * with nrec = xbg_bnd_nrec(p).  The inverse transforms are evaluated without
+ * FFTW (see xbg_bnd.cpp); results agree with an exact DFT to ~1e-13.
+ */
+int xbg_wave_groups(const xbg_params* p, double offdepth, int nfreq, int ndir, const double* freq,
+                    const double* dir, const double* spec, unsigned seed, double* Trep, double* Stfile,
+                    double* qfile);
+
+/* GenCstWave (makjonswap.cpp:24-71): one record per list entry, uniform
+ * along the boundary, cos^2s over the bins (theta: bin centres), zero
+ * bound wave.  Tpfile may be NULL. */
+int xbg_constant_waves(const xbg_params* p, int nrec, const xbg_wavebnd* w, const float* theta, double* Stfile,
+                       double* qfile, double* Tpfile);
+
+/* The generator's uniform deviates and the Hilbert analytic signal of
+ * tools.cpp:259-323 (exposed for tests; no counterpart in the reference). */
+int xbg_bnd_uniforms(unsigned seed, int n, double* out);
+int xbg_hilbert(int m, const double* x, double* re, double* im);
+/* Real inverse DFT (FFTW_BACKWARD convention, unnormalised) of a length-N
+ * Hermitian spectrum from its non-zero coefficients at 0 < idx < N/2
+ * (exposed for tests). */
+int xbg_bnd_synth(int N, int nk, const int* idx, const double* gre, const double* gim, double* out);
 
 int xbg_sync(xbg_ctx* ctx);
 void* xbg_stream(xbg_ctx* ctx);   /* the context's hipStream_t */

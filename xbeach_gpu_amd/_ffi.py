@@ -82,7 +82,7 @@ PROTOTYPES = {
     "xbg_set_wave_forcing": (C.c_int, [_CTX, _F, C.c_double]),
     "xbg_set_flow_forcing": (C.c_int, [_CTX, _F, _F, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_double]),
-    "xbg_set_bnd_series": (C.c_int, [_CTX, _F, _F, C.c_int]),
+    "xbg_set_bnd_series": (C.c_int, [_CTX, _D, _D, C.c_int]),
     "xbg_use_bnd_record": (C.c_int, [_CTX, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double]),
     "xbg_flow_bnd": (C.c_int, [_CTX, C.c_double, C.c_double]),
@@ -99,7 +99,22 @@ PROTOTYPES = {
     "xbg_stream": (C.c_void_p, [_CTX]),
     "xbg_profile": (C.c_int, [_CTX, C.c_int]),
     "xbg_profile_report": (C.c_int, [_CTX, C.c_char_p, C.c_size_t]),
+    "xbg_jonswap_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "xbg_jonswap": (C.c_int, [C.c_void_p, _D, _D, _D]),
+    "xbg_bnd_nrec": (C.c_int, [_P]),
+    "xbg_wave_groups": (C.c_int, [_P, C.c_double, C.c_int, C.c_int, _D, _D, _D, C.c_uint, _D, _D, _D]),
+    "xbg_constant_waves": (C.c_int, [_P, C.c_int, C.c_void_p, _F, _D, _D, _D]),
+    "xbg_bnd_uniforms": (C.c_int, [C.c_uint, C.c_int, _D]),
+    "xbg_hilbert": (C.c_int, [C.c_int, _D, _D, _D]),
+    "xbg_bnd_synth": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), _D, _D, _D]),
 }
+
+
+class XbgWavebnd(C.Structure):
+    """Mirror of ``xbg_wavebnd`` (Wavebndparam, XBeachGPU.h:198-210)."""
+
+    _fields_ = [("time", C.c_double), ("Hs", C.c_double), ("Tp", C.c_double), ("Dp", C.c_double),
+                ("s", C.c_double), ("gamma", C.c_double)]
 
 _lib = None
 

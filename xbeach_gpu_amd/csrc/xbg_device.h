@@ -36,6 +36,9 @@ struct KP {
     float D50, D90, rhosed, wws, drydzmax, wetdzmax, maxslpchg, por, morfac, sus, bed;
     float facsk, facas, cfsand, cfreef, fwsand, fwreef, CFL;
     float twopi, aphi, bphi;
+    // uniform sediment constants, formed on the host with the same float
+    // operations and elementary functions as the per-cell code
+    float sed_drho, sed_dester, sed_Ass, sed_ucrfac, rvr_alph, rvr_bet;
     int usesmago, breakmodel, roller, zb_accum, swave;
     float theta[MAXT], cxsth[MAXT], sxnth[MAXT];
 };

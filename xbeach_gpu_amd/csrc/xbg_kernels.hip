@@ -51,6 +51,7 @@ constexpr int BX = 64;  // block: 64 cells along x (one wavefront per row)
 constexpr int BY = 4;   // 4 rows -> 256 threads
 
 struct Fld {
+    const float* dirs;  // direction tables, 8 x MAXT (see tab())
     const float* St;   // ny*nt wave boundary of this step
     const float* qbo;  // 4*ny
     const float* qbn;
@@ -84,6 +85,19 @@ struct Fld {
 
 XDEV int gx() { return blockIdx.x * BX + threadIdx.x; }
 XDEV int gy() { return blockIdx.y * BY + threadIdx.y; }
+
+// Direction-table entry read
+// through an opaque scalar index: a scalar load at each use instead of
+// nTheta x 8 values held in scalar registers across the unrolled bin loop.
+// dirs[k * MAXT + th], k = 0..7: theta, cos, sin, 1 + cos^2, 1 + sin^2,
+// sin*cos, cos^2, sin^2 (FP32 products as radstressCPU forms them,
+// wave_CPU.cpp:1118-1157).
+typedef const __attribute__((address_space(4))) float* cfloat_ptr;  // constant (read-only) memory
+XDEV float tab(const float* t, int th) {
+    int k = th;
+    asm volatile("" : "+s"(k));
+    return ((cfloat_ptr)t)[k];
+}
 
 // ---------------------------------------------------------------------------
 // Flow boundary: ubnd1D (Flow_kernel.cu:167-242) / ubnd1Dnowaves (332-403).
@@ -183,8 +197,8 @@ XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
     r.hi = 0.25f * s.sigm;
     return r;
 }
-XDEV float ctheta_of(const KP& p, const Refr& r, int th) {
-    float cs = p.cxsth[th], sn = p.sxnth[th];
+XDEV float ctheta_of(const KP& p, const float* dirs, const Refr& r, int th) {
+    float cs = tab(dirs + MAXT, th), sn = tab(dirs + 2 * MAXT, th);
     float ctht = r.sig * (r.dhdx * sn - r.dhdy * cs) + r.w * (cs * (sn * r.dudx - cs * r.dudy) + sn * (sn * r.dvdx - cs * r.dvdy));
     return fmn(fmx(ctht, r.lo), r.hi);
 }
@@ -218,9 +232,9 @@ XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
 
 // x + y flux divergence of one bin from its 9-point cross stencil
 // (values at the clamped neighbour indices).
-XDEV float adv_xy(const KP& p, const AdvC& a, float dxi, int th, float ec, float exm, float exm2, float exp_,
+XDEV float adv_xy(const KP& p, const float* dirs, const AdvC& a, float dxi, int th, float ec, float exm, float exm2, float exp_,
                   float exp2, float eym, float eym2, float eyp, float eyp2) {
-    float cs = p.cxsth[th], sn = p.sxnth[th];
+    float cs = tab(dirs + MAXT, th), sn = tab(dirs + 2 * MAXT, th);
     // upwind operands are selected first so each face evaluates one
     // 2nd-order value (branch-free code would otherwise evaluate both)
     float cgx = 0.5f * (a.sc * cs + a.ui + a.sxp * cs + a.uxp);
@@ -236,41 +250,6 @@ XDEV float adv_xy(const KP& p, const AdvC& a, float dxi, int th, float ec, float
     amin = upw2d(pym ? eym : ec, pym ? eym2 : eyp) * cgymin;
     float ya = (ain - amin) * dxi;
     return xa + ya;
-}
-
-// theta advection (thetaadvecuw2hoCPU, wave_CPU.cpp:706-808) and the Euler
-// update (eulerupwindCPU, 810-828): ea holds x+y advection on entry and the
-// updated field on exit.  The reference forms the flux through the face
-// between bins th and th+1 twice, as arrint of th and as arrmint of th+1,
-// from identical operands; here it is formed once per face (same value).
-// Zero flux through the two end faces.
-template <int NT>
-XDEV void theta_euler(const KP& p, const SP& s, const Refr& rf, const float* e0, float* ea) {
-    const int nt = NT > 0 ? NT : p.nt;
-    if (nt <= 1) {
-        ea[0] = e0[0] - s.dt * (ea[0] + 0.0f);
-        return;
-    }
-    float ctl = ctheta_of(p, rf, 0);
-    float fl = 0.0f;  // flux through the face below the current bin
-#pragma unroll
-    for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
-        if (th >= nt) break;
-        float ta;
-        if (th < nt - 1) {
-            int tm = m1(th), tp = th + 1, tp2 = p2(th, nt);
-            float ctr = ctheta_of(p, rf, tp);
-            float cb = 0.5f * (ctl + ctr);
-            float eu = cb > 0.0f ? upw2f(e0[th], e0[tm]) : upw2f(e0[tp], e0[tp2]);
-            float fr = cb * eu;
-            ta = th == 0 ? (fr - 0.0f) / p.dtheta : (fr - fl) / p.dtheta;
-            fl = fr;
-            ctl = ctr;
-        } else {
-            ta = (0.0f - fl) / p.dtheta;
-        }
-        ea[th] = e0[th] - s.dt * (ea[th] + ta);
-    }
 }
 
 // roelvinkCPU (wave_CPU.cpp:932-985): Qb and D in double as written.
@@ -326,12 +305,20 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     float RR = 0.0f, drr = 0.0f;
     bool wetc = h > p.eps;
     float Emx = fmx(E, 0.0000001f);
+    // x / 1 == x exactly: the division is needed only where energint capped H
+    if (idiv != 1.0f) {
+#pragma unroll
+        for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
+            if (th >= nt) break;
+            e[th] = e[th] / idiv;
+        }
+    }
 #pragma unroll
     for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
         if (th >= nt) break;
-        float eet = e[th] / idiv;
-        float cx = cc * p.cxsth[th] + uui;
-        float cy = cc * p.sxnth[th] + vvi;
+        float eet = e[th];
+        float cx = cc * tab(f.dirs + MAXT, th) + uui;
+        float cy = cc * tab(f.dirs + 2 * MAXT, th) + vvi;
         float dd = eet * Dv / Emx;
         float rrt;
         if (wetc) {
@@ -368,15 +355,14 @@ XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const fl
     for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
         if (th >= nt) break;
         float ev = efin[th], rv = r[th];
-        float cs = p.cxsth[th], sn = p.sxnth[th];
         sume = sume + ev;
-        sumethet = sumethet + ev * p.theta[th];
-        txx = txx + ((1.0f + cs * cs) * ev);
-        tyy = tyy + ((1.0f + sn * sn) * ev);
-        txy = txy + (sn * cs * ev);
-        rxx = rxx + ((cs * cs) * rv);
-        ryy = ryy + ((sn * sn) * rv);
-        rxy = rxy + (sn * cs * rv);
+        sumethet = sumethet + ev * tab(f.dirs, th);
+        txx = txx + (tab(f.dirs + 3 * MAXT, th) * ev);   // (1 + cs*cs) * ev
+        tyy = tyy + (tab(f.dirs + 4 * MAXT, th) * ev);   // (1 + sn*sn) * ev
+        txy = txy + (tab(f.dirs + 5 * MAXT, th) * ev);   // sn*cs*ev
+        rxx = rxx + (tab(f.dirs + 6 * MAXT, th) * rv);   // (cs*cs) * rv
+        ryy = ryy + (tab(f.dirs + 7 * MAXT, th) * rv);   // (sn*sn) * rv
+        rxy = rxy + (tab(f.dirs + 5 * MAXT, th) * rv);   // sn*cs*rv
         f.ee_out[i + (size_t)th * N] = ev;
         f.rr_out[i + (size_t)th * N] = rv;
     }
@@ -409,31 +395,57 @@ constexpr int PH = WTY + 4;
 constexpr int PN = PW * PH;
 constexpr int WTHREADS = BX * WTY;
 
-// Fill one directional plane of the tile (+halo) into LDS.  Halo cells hold
-// the values at the clamped neighbour indices (mminus/pplus/mminus2/pplus2
-// equal clamping), so the stencil reads are plain offsets.
+// Staging of one directional plane of the tile (+halo) into LDS.  Halo
+// cells hold the values at the clamped neighbour indices
+// (mminus/pplus/mminus2/pplus2 equal clamping), so the stencil reads are
+// plain offsets.  The element a thread stages is the same in every plane:
+// its in-plane offset is computed once (Slots) and each plane adds only its
+// uniform base.  Offshore-column elements (gxx == 0) of ee come from the
+// boundary spectrum instead (offshorebndWav); they are coded as -(gyy+1).
+constexpr int NSLOT = (PN + WTHREADS - 1) / WTHREADS;
+struct Slots {
+    int code[NSLOT];
+    bool bnd;  // tile touches the offshore column (x0 == 0): uniform per tile
+};
+XDEV Slots make_slots(const KP& p, int x0, int y0, int tid) {
+    Slots sl;
+    sl.bnd = x0 == 0;
+#pragma unroll
+    for (int it = 0; it < NSLOT; it++) {
+        int l = tid + it * WTHREADS;
+        int r = l / PW, c = l - r * PW;
+        int gxx = min(max(x0 + c - 2, 0), p.nx - 1);
+        int gyy = min(max(y0 + r - 2, 0), p.ny - 1);
+        sl.code[it] = gxx == 0 ? -(gyy + 1) : gxx + gyy * p.nx;
+    }
+    return sl;
+}
+
 template <bool IS_EE>
-XDEV void fill_plane(float* pl, const KP& p, const SP& s, const Fld& f, int x0, int y0, int th, int tid) {
-    const int nx = p.nx, ny = p.ny;
+XDEV void fill_plane(float* pl, const KP& p, const SP& s, const Fld& f, const Slots& sl, int th, int tid) {
     // opaque bin index: keeps the compiler from precomputing (and holding
     // in registers) the plane addresses of every bin of the unrolled loop
     int thv = th;
     asm volatile("" : "+s"(thv));
     const uint32_t pbase = (uint32_t)thv * p.N;
-    th = thv;
+    const float* plane = (IS_EE ? f.ee_in : f.rr_in) + pbase;
 #pragma unroll
-    for (int it = 0; it < (PN + WTHREADS - 1) / WTHREADS; it++) {
+    for (int it = 0; it < NSLOT; it++) {
         int l = tid + it * WTHREADS;
         if (l < PN) {
-            int r = l / PW, c = l - r * PW;
-            int gxx = min(max(x0 + c - 2, 0), nx - 1);
-            int gyy = min(max(y0 + r - 2, 0), ny - 1);
+            int code = sl.code[it];
             float v;
             if (IS_EE) {
-                v = gxx == 0 ? f.St[gyy + th * ny] * s.taper : f.ee_in[pbase + (uint32_t)gxx + (uint32_t)gyy * nx];
+                // only the tiles of the first column hold offshore cells
+                if (!sl.bnd || code >= 0) {
+                    v = plane[(uint32_t)code];
+                } else {
+                    v = f.St[(uint32_t)(-code - 1) + (uint32_t)thv * p.ny] * s.taper;
+                }
                 v = fmx(v, 0.000f) / s.sigm;  // sanity + action (wave_CPU.cpp:58-82, 289-309)
             } else {
-                v = f.rr_in[pbase + (uint32_t)gxx + (uint32_t)gyy * nx];
+                // rr: the offshore column carries its own value
+                v = plane[code >= 0 ? (uint32_t)code : (uint32_t)(-code - 1) * p.nx];
             }
             pl[l] = v;
         }
@@ -445,15 +457,22 @@ XDEV void fill_plane(float* pl, const KP& p, const SP& s, const Fld& f, int x0, 
 // load of bin th+1 overlaps bin th).  Bin th-2 is finished as soon as the
 // action of bin th is known (its theta stencil reaches th-2..th+2), so only
 // a 5-bin window of pre-update values is live.  ea[] holds the updated
-// field on exit (eulerupwindCPU, wave_CPU.cpp:810-828).
+// field on exit (eulerupwindCPU, wave_CPU.cpp:810-828).  With WAVEDT the
+// largest |ctheta| over the bins, in bin order, is returned in *cmax: the
+// theta fluxes evaluate ctheta of every bin once.
 template <int NT, bool IS_EE>
 XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f, const AdvC& a, const Refr& rf,
-                        int x0, int y0, int tid, int cx, int cy, float* ea) {
+                        const Slots& sl, int tid, int cx, int cy, float* ea, float* cmax) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
     const float dxi = 1.0f / p.dx;
     float e0[A];
-    float ctl = ctheta_of(p, rf, 0), fl = 0.0f;
+    float ctl = ctheta_of(p, f.dirs, rf, 0), fl = 0.0f;
+    float cm = 0.0f;
+    if (IS_EE) {
+        float v = fabsf(ctl);
+        cm = v > cm ? v : cm;
+    }
     // finish bin b (0 <= b < nt) once e0[b-1..b+2] are known; theta flux
     // through the face between b and b+1 formed once (thetaadvecuw2hoCPU,
     // wave_CPU.cpp:706-808, forms it twice from identical operands)
@@ -463,7 +482,11 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
             ta = 0.0f;
         } else if (b < nt - 1) {
             int tm = m1(b), tp = b + 1, tp2 = p2(b, nt);
-            float ctr = ctheta_of(p, rf, tp);
+            float ctr = ctheta_of(p, f.dirs, rf, tp);
+            if (IS_EE) {
+                float v = fabsf(ctr);
+                cm = v > cm ? v : cm;
+            }
             float cb = 0.5f * (ctl + ctr);
             bool pc = cb > 0.0f;
             float eu = upw2f(pc ? e0[b] : e0[tp], pc ? e0[tm] : e0[tp2]);
@@ -476,17 +499,17 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
         }
         ea[b] = e0[b] - s.dt * (ea[b] + ta);
     };
-    fill_plane<IS_EE>(pl[0], p, s, f, x0, y0, 0, tid);
+    fill_plane<IS_EE>(pl[0], p, s, f, sl, 0, tid);
     __syncthreads();
     const int c0 = (cy + 2) * PW + (cx + 2);
 #pragma unroll
     for (int th = 0; th < A; th++) {
         if (th >= nt) break;
-        if (th + 1 < nt) fill_plane<IS_EE>(pl[(th + 1) & 1], p, s, f, x0, y0, th + 1, tid);
+        if (th + 1 < nt) fill_plane<IS_EE>(pl[(th + 1) & 1], p, s, f, sl, th + 1, tid);
         const float* q = pl[th & 1];
         float ec = q[c0];
         e0[th] = ec;
-        ea[th] = adv_xy(p, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
+        ea[th] = adv_xy(p, f.dirs, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
                         q[c0 + PW], q[c0 + 2 * PW]);
         if (th >= 2) finish(th - 2);
         // materialise this bin's results before the barrier: otherwise the
@@ -501,6 +524,7 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
         if (b >= nt) break;
         if (b >= nt - 2) finish(b);
     }
+    if (IS_EE) *cmax = cm;
 }
 
 // The fused wave step, one thread per cell of a 64 x WTY tile:
@@ -509,12 +533,14 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
 // breakerdelay (Wavestep.cu:781-1125 order).  Rows 0 and ny-1 are finished
 // by k_wave_rows (rollerlatbnd copies from rows 1 and ny-2, whose
 // post-Euler bins this kernel saves).  Also the block minimum of WAVEDT.
+// Four waves per SIMD (<= 128 VGPRs): two workgroups per CU.
 template <int NT>
-__global__ __launch_bounds__(WTHREADS) void k_wave(KP p, SP s, Fld f) {
+__global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
     __shared__ float pl[2][PN];
     __shared__ float wred[WTHREADS / 64];
+    __shared__ float stash[NT > 0 ? NT : 1][WTHREADS];
     const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
     const int x0 = blockIdx.x * BX, y0 = blockIdx.y * WTY;
     const int nx = p.nx, ny = p.ny;
@@ -523,19 +549,14 @@ __global__ __launch_bounds__(WTHREADS) void k_wave(KP p, SP s, Fld f) {
     const uint32_t i = (uint32_t)x + (uint32_t)y * nx;
     float ea[A];
     const Refr rf = refr_cell(p, s, f, x, y);
-    {
-        AdvC a = advc_cell(p, f, f.cg, x, y);
-        advect_planes<NT, true>(pl, p, s, f, a, rf, x0, y0, tid, cx, cy, ea);
-    }
+    const Slots sl = make_slots(p, x0, y0, tid);
     // WAVEDT (Wave_kernel.cu:232-249): min over bins of CFL*dtheta/|ctheta|.
     // Rounded division is monotonic in the divisor, so this equals one
     // division by the largest |ctheta| (NaN speeds ignored as fminf does).
-    float cmax = 0.0f;
-#pragma unroll
-    for (int th = 0; th < A; th++) {
-        if (th >= nt) break;
-        float v = fabsf(ctheta_of(p, rf, th));
-        cmax = v > cmax ? v : cmax;
+    float cmax;
+    {
+        AdvC a = advc_cell(p, f, f.cg, x, y);
+        advect_planes<NT, true>(pl, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
     }
     float wmin = fmn(9999999.9f, p.CFL * p.dtheta / cmax);
     const int side = (y == 1) ? 0 : (y == ny - 2 ? 1 : -1);
@@ -546,10 +567,16 @@ __global__ __launch_bounds__(WTHREADS) void k_wave(KP p, SP s, Fld f) {
             f.bnd_e1[((uint32_t)side * nt + th) * nx + x] = ea[th];
         }
     }
+    // the post-Euler ee bins wait in LDS while rr is advected (they would
+    // otherwise hold nTheta registers through the roller pass)
+    if (NT > 0) {
+#pragma unroll
+        for (int th = 0; th < A; th++) stash[th][tid] = ea[th];
+    }
     float ra[A];
     if (p.roller == 1) {
         AdvC a = advc_cell(p, f, f.c, x, y);
-        advect_planes<NT, false>(pl, p, s, f, a, rf, x0, y0, tid, cx, cy, ra);
+        advect_planes<NT, false>(pl, p, s, f, a, rf, sl, tid, cx, cy, ra, nullptr);
         if (valid && side >= 0) {
 #pragma unroll
             for (int th = 0; th < A; th++) {
@@ -563,6 +590,10 @@ __global__ __launch_bounds__(WTHREADS) void k_wave(KP p, SP s, Fld f) {
             if (th >= nt) break;
             ra[th] = f.rr_in[i + (uint32_t)th * p.N];
         }
+    }
+    if (NT > 0) {
+#pragma unroll
+        for (int th = 0; th < A; th++) ea[th] = stash[th][tid];
     }
     if (valid) {
         float Rv, DRv;
@@ -1084,9 +1115,9 @@ __global__ __launch_bounds__(256) void k_flow_uvvu(KP p, Fld f) {
 
 // Rvr (Sediment_kernel.cu:477-510) at cell i
 XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
-    const float m1c = 0.0f, m2c = 0.7939f, m3c = -0.6065f, m4c = 0.3539f, m5c = 0.6373f, m6c = 0.5995f;
-    float alph = -1.0f * xm::log10f(xm::expf(1.0f)) / m4c;
-    float bet = xm::expf(m3c / m4c);
+    const float m1c = 0.0f, m2c = 0.7939f, m5c = 0.6373f, m6c = 0.5995f;
+    // alph = -log10(e)/m4, bet = exp(m3/m4) with m3 = -0.6065, m4 = 0.3539
+    const float alph = p.rvr_alph, bet = p.rvr_bet;
     float kk = (float)(2 * PI_ / (double)(f.c[i] * s.Trep));
     float Ur = 3.0f / 8.0f * sqrtf(2.0f) * f.H[i] * kk / xm::powf(kk * f.hh[i], 3.0f);
     Ur = fmx(Ur, 0.00001f);
@@ -1102,13 +1133,35 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
 // bulk = 1 (Sediment_kernel.cu:196, 320), so the bed-load fluxes Sub/Svb
 // are zero and only Sus/Svs are formed.
 __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
+    // Rvr's ua at the cell and at its +x / +y neighbours: evaluated once per
+    // cell into an LDS tile (+1 column, +1 row) at clamped coordinates, so
+    // tile(cx+1, cy) is ua at pplus(x) and tile(cx, cy+1) at pplus(y)
+    __shared__ float uat_[BY + 1][BX + 1];
     const int nx = p.nx, ny = p.ny;
+    const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    {
+        int xc = min(x0 + cx, nx - 1), yc = min(y0 + cy, ny - 1);
+        uat_[cy][cx] = rvr_at(p, s, f, ix2(p, xc, yc));
+        if (tid < BX + BY) {
+            int hx, hy, tx, ty;
+            if (tid < BY) {
+                tx = BX; ty = tid;
+            } else {
+                tx = tid - BY; ty = BY;
+            }
+            hx = min(x0 + tx, nx - 1);
+            hy = min(y0 + ty, ny - 1);
+            uat_[ty][tx] = rvr_at(p, s, f, ix2(p, hx, hy));
+        }
+    }
+    __syncthreads();
+    int x = x0 + cx, y = y0 + cy;
+    if (x >= nx || y >= ny) return;
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, nx), y), t = ix2(p, x, p1(y, ny));
     float rol = f.rolthick[i] + s.dt * (fabsf(f.dzsdt[i]) - p.beta * f.c[i]);
     f.rolthick[i] = fmx(rol, 0.0f);
-    float uai = rvr_at(p, s, f, i), uar = rvr_at(p, s, f, r), uat = rvr_at(p, s, f, t);
+    float uai = uat_[cy][cx], uar = uat_[cy][cx + 1], uat = uat_[cy + 1][cx];
     float cci = f.C[i], ccr = f.C[r], cct = f.C[t];
     float tm = f.thetamean[i];
     float ctm = xm::cosf(tm), stm = xm::sinf(tm);
@@ -1147,8 +1200,7 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     const float zo = 0.006f, sedcal = 1.0f, bulk = 1.0f;
     float h = f.hh[i];
     float hhi = fmx(h, 0.01f);
-    float drho = (p.rhosed - p.rho) / p.rho;
-    float dester = xm::powf(drho * p.g, 1.0f / 3.0f) / 0.0001f * p.D50;
+    const float drho = p.sed_drho;
     float ue = (float)(0.5 * (double)(f.ueu[ix2(p, m1(x), y)] + f.ueu[i]));
     float ve = (float)(0.5 * (double)(f.vev[ix2(p, x, m1(y))] + f.vev[i]));
     float vmags = ue * ue + ve * ve;
@@ -1158,14 +1210,12 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     float kb = p.nuhfac * xm::powf(f.DR[i] / p.rho, 0.66666666667f) * dcf;
     float Urms2 = f.urms[i] * f.urms[i] + 1.45f * (kb + 0.0f);
     float Tsg = fmx(0.1f * hhi / p.wws, 0.5f);
-    float Ucr;
-    if (p.D50 <= 0.0005f) Ucr = 0.19f * xm::powf(p.D50, 0.1f) * xm::log10f(4.0f * hhi / p.D90);
-    else Ucr = 8.50f * xm::powf(p.D50, 0.6f) * xm::log10f(4.0f * hhi / p.D90);
+    float Ucr = p.sed_ucrfac * xm::log10f(4.0f * hhi / p.D90);
     float hdrag = fmx(hhi, 10.0f * zo);
     float Cdd = 0.4f / (xm::logf(hdrag / zo) - 1.0f);
     Cdd = Cdd * Cdd;
     float Asb = 0.005f * hhi * xm::powf(1 / hhi / (drho * p.g), 1.2f);
-    float Ass = 0.012f * p.D50 * xm::powf(dester, -0.6f) / (xm::powf(drho * p.g * p.D50, 1.2f));
+    const float Ass = p.sed_Ass;
     float cfm = cfm_of(p, f.stdep[i]);
     float T1 = vmags + 0.018f / Cdd * Urms2;
     T1 = fmn(T1, 100000.0f * p.g / cfm * p.D50 * drho);
@@ -1481,10 +1531,11 @@ struct xbg_ctx {
     float* Cb[2] = {nullptr, nullptr};
     int cur = 0, ccur = 0;
     float* St = nullptr;
+    float* dirs = nullptr;
     float* qbo = nullptr;
     float* qbn = nullptr;
-    float* Stfile = nullptr;
-    float* qfile = nullptr;
+    double* Stfile = nullptr;
+    double* qfile = nullptr;
     int nrec = 0;
     float *Hmean = nullptr, *uumean = nullptr, *vvmean = nullptr, *hhmean = nullptr, *zsmean = nullptr,
           *zsmax = nullptr, *Cmean = nullptr;
@@ -1595,6 +1646,14 @@ void make_kp(xbg_ctx* c) {
     k.aphi = 1 / (phi + 1);
     k.bphi = phi / (phi + 1);
     k.twopi = 8 * atanf(1.0f);
+    // Sbvr (Sediment_kernel.cu:167-326) and Rvr (477-510) terms that depend
+    // on parameters only
+    k.sed_drho = (k.rhosed - k.rho) / k.rho;
+    k.sed_dester = xm::powf(k.sed_drho * k.g, 1.0f / 3.0f) / 0.0001f * k.D50;
+    k.sed_Ass = 0.012f * k.D50 * xm::powf(k.sed_dester, -0.6f) / (xm::powf(k.sed_drho * k.g * k.D50, 1.2f));
+    k.sed_ucrfac = k.D50 <= 0.0005f ? 0.19f * xm::powf(k.D50, 0.1f) : 8.50f * xm::powf(k.D50, 0.6f);
+    k.rvr_alph = -1.0f * xm::log10f(xm::expf(1.0f)) / 0.3539f;
+    k.rvr_bet = xm::expf(-0.6065f / 0.3539f);
     k.usesmago = P.usesmago; k.breakmodel = P.breakmodel; k.roller = P.roller;
     k.zb_accum = P.zb_accum; k.swave = P.swave;
     xbg_theta_tables(&P, k.theta, k.cxsth, k.sxnth);
@@ -1624,6 +1683,7 @@ void sync_ptrs(xbg_ctx* c) {
     c->f.C = c->Cb[c->ccur];
     c->f.C_out = c->Cb[1 - c->ccur];
     c->f.St = c->St;
+    c->f.dirs = c->dirs;
     c->f.qbo = c->qbo;
     c->f.qbn = c->qbn;
 }
@@ -1761,6 +1821,7 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
     A(&c->ee[0], ND); A(&c->ee[1], ND); A(&c->rr[0], ND); A(&c->rr[1], ND);
     A(&c->Cb[0], N); A(&c->Cb[1], N);
     A(&c->St, ny * P.ntheta); A(&c->qbo, 4 * ny); A(&c->qbn, 4 * ny);
+    A(&c->dirs, 8 * MAXT);
     float** two[] = {&f.zs, &f.uu, &f.vv, &f.hh, &f.zb, &f.stdep, &f.hu, &f.hv, &f.ueu, &f.vev, &f.vmageu,
                      &f.vmagev, &f.vu, &f.uv, &f.qx, &f.qy, &f.uuold, &f.vvold, &f.zsold, &f.dzsdt,
                      &f.rolthick, &f.dzb, &f.k, &f.c, &f.cg, &f.H, &f.E, &f.Dw, &f.DR, &f.R, &f.urms,
@@ -1785,6 +1846,26 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
         g_create_error = std::string("hipHostMalloc: ") + hipGetErrorString(e);
         xbg_destroy(c);
         return XBG_E_HIP;
+    }
+    {
+        float tabs[8 * MAXT] = {};
+        for (int t = 0; t < P.ntheta; t++) {
+            float cs = c->kp.cxsth[t], sn = c->kp.sxnth[t];
+            tabs[t] = c->kp.theta[t];
+            tabs[MAXT + t] = cs;
+            tabs[2 * MAXT + t] = sn;
+            tabs[3 * MAXT + t] = 1.0f + cs * cs;
+            tabs[4 * MAXT + t] = 1.0f + sn * sn;
+            tabs[5 * MAXT + t] = sn * cs;
+            tabs[6 * MAXT + t] = cs * cs;
+            tabs[7 * MAXT + t] = sn * sn;
+        }
+        e = hipMemcpy(c->dirs, tabs, sizeof(tabs), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            g_create_error = std::string("hipMemcpy: ") + hipGetErrorString(e);
+            xbg_destroy(c);
+            return XBG_E_HIP;
+        }
     }
     sync_ptrs(c);
     register_fields(c);
@@ -1878,17 +1959,17 @@ int xbg_set_flow_forcing(xbg_ctx* c, const float* qbndold, const float* qbndnew,
     return XBG_OK;
 }
 
-int xbg_set_bnd_series(xbg_ctx* c, const float* Stfile, const float* qfile, int nrec) {
+int xbg_set_bnd_series(xbg_ctx* c, const double* Stfile, const double* qfile, int nrec) {
     if (!c || !Stfile || !qfile || nrec < 2) return fail(c, XBG_E_ARG, "xbg_set_bnd_series: bad argument");
     size_t ns = (size_t)nrec * c->P.ny * c->P.ntheta, nq = (size_t)nrec * 4 * c->P.ny;
     if (c->Stfile) hipFree(c->Stfile);
     if (c->qfile) hipFree(c->qfile);
     c->Stfile = nullptr;
     c->qfile = nullptr;
-    HIPC(c, hipMalloc((void**)&c->Stfile, ns * sizeof(float)));
-    HIPC(c, hipMalloc((void**)&c->qfile, nq * sizeof(float)));
-    HIPC(c, hipMemcpy(c->Stfile, Stfile, ns * sizeof(float), hipMemcpyHostToDevice));
-    HIPC(c, hipMemcpy(c->qfile, qfile, nq * sizeof(float), hipMemcpyHostToDevice));
+    HIPC(c, hipMalloc((void**)&c->Stfile, ns * sizeof(double)));
+    HIPC(c, hipMalloc((void**)&c->qfile, nq * sizeof(double)));
+    HIPC(c, hipMemcpy(c->Stfile, Stfile, ns * sizeof(double), hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->qfile, qfile, nq * sizeof(double), hipMemcpyHostToDevice));
     c->nrec = nrec;
     return XBG_OK;
 }
@@ -1898,18 +1979,19 @@ int xbg_set_bnd_series(xbg_ctx* c, const float* Stfile, const float* qfile, int 
 namespace {
 // St = interptime(Stnew, Stold, timenext, timesincelast) in double
 // (Wavestep.cu:740-745 with tools.cpp:22-25) from the resident records.
-__global__ __launch_bounds__(256) void k_bnd_interp(const float* Stfile, const float* qfile, int rec, int nst,
+__global__ __launch_bounds__(256) void k_bnd_interp(const double* Stfile, const double* qfile, int rec, int nst,
                                                     int nq, double tsl, double tnext, float* St, float* qo,
                                                     float* qn) {
     int j = blockIdx.x * 256 + threadIdx.x;
     if (j < nst) {
-        double so = (double)Stfile[(size_t)rec * nst + j];
-        double sn = (double)Stfile[(size_t)(rec + 1) * nst + j];
+        double so = Stfile[(size_t)rec * nst + j];
+        double sn = Stfile[(size_t)(rec + 1) * nst + j];
         St[j] = (float)(so + (tsl) / (tnext) * (sn - so));
     }
     if (j < nq) {
-        qo[j] = qfile[(size_t)rec * nq + j];
-        qn[j] = qfile[(size_t)(rec + 1) * nq + j];
+        // qbndold / qbndnew are DECNUM copies of the double records
+        qo[j] = (float)qfile[(size_t)rec * nq + j];
+        qn[j] = (float)qfile[(size_t)(rec + 1) * nq + j];
     }
 }
 }  // namespace

@@ -34,6 +34,10 @@ def _fp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
 class Domain:
     def __init__(self, params: XbgParams):
         self.L = _ffi.lib()
@@ -108,11 +112,11 @@ class Domain:
                                               float(windu), float(windv)))
 
     def set_bnd_series(self, Stfile, qfile) -> None:
-        s = np.ascontiguousarray(np.asarray(Stfile, dtype=np.float32))
-        q = np.ascontiguousarray(np.asarray(qfile, dtype=np.float32))
+        s = np.ascontiguousarray(np.asarray(Stfile, dtype=np.float64))
+        q = np.ascontiguousarray(np.asarray(qfile, dtype=np.float64))
         nrec = s.shape[0]
         assert s.size == nrec * self.nt * self.ny and q.size == nrec * 4 * self.ny
-        self._chk(self.L.xbg_set_bnd_series(self._ctx, _fp(s), _fp(q), int(nrec)))
+        self._chk(self.L.xbg_set_bnd_series(self._ctx, _dp(s), _dp(q), int(nrec)))
 
     def use_bnd_record(self, rec, timesincelast, timenext, Trep, zsbndi=0.0, zsbndn=0.0, windu=0.0,
                        windv=0.0) -> None:

@@ -158,7 +158,8 @@ class GroupForcing:
         return q
 
     def records(self, t0: float, n: int):
-        """Stfile (n, ntheta, ny) and qfile (n, 4, ny) at t0 + m*dtbc."""
+        """Stfile (n, ntheta, ny) and qfile (n, 4, ny) at t0 + m*dtbc, in
+        double as the reference holds them."""
         S = np.stack([self.St(t0 + m * self.dtbc) for m in range(n)])
         Q = np.stack([self.qbnd(t0 + m * self.dtbc) for m in range(n)])
-        return S.astype(np.float32), Q.astype(np.float32)
+        return S.astype(np.float64), Q.astype(np.float64)
