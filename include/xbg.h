@@ -277,6 +277,25 @@ int xbg_hilbert(int m, const double* x, double* re, double* im);
  * (exposed for tests). */
 int xbg_bnd_synth(int N, int nk, const int* idx, const double* gre, const double* gim, double* out);
 
+/* ------------------------------------------------------------------------
+ * netCDF output (host code, no netCDF library; writenetcdf.cpp:32-239)
+ * ---------------------------------------------------------------------- */
+typedef struct xbg_nc xbg_nc;
+
+/* creatncfileUD (writenetcdf.cpp:32-106): dims xx, yy, theta, time
+ * (unlimited); coordinate variables xx = i*dx, yy = j*dx, theta = bin
+ * centres, and time[0] = totaltime.  NULL on failure. */
+xbg_nc* xbg_nc_create(const char* path, int nx, int ny, int ntheta, double dx, double dtheta, double thetamin,
+                      double totaltime);
+/* defncvar (writenetcdf.cpp:108-174): vdim 2 -> (yy, xx), 3 -> (time, yy,
+ * xx), 4 -> (time, theta, yy, xx); var holds the first record / the values. */
+int xbg_nc_def_var(xbg_nc* nc, const char* name, int vdim, const float* var);
+/* writenctimestep (writenetcdf.cpp:176-191): append a record with its time. */
+int xbg_nc_write_time(xbg_nc* nc, double totaltime);
+/* writencvarstep (writenetcdf.cpp:193-239): the variable in the last record. */
+int xbg_nc_write_var_step(xbg_nc* nc, const char* name, const float* var);
+int xbg_nc_close(xbg_nc* nc);
+
 int xbg_sync(xbg_ctx* ctx);
 void* xbg_stream(xbg_ctx* ctx);   /* the context's hipStream_t */
 

@@ -107,6 +107,12 @@ PROTOTYPES = {
     "xbg_bnd_uniforms": (C.c_int, [C.c_uint, C.c_int, _D]),
     "xbg_hilbert": (C.c_int, [C.c_int, _D, _D, _D]),
     "xbg_bnd_synth": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), _D, _D, _D]),
+    "xbg_nc_create": (C.c_void_p, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_double]),
+    "xbg_nc_def_var": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, _F]),
+    "xbg_nc_write_time": (C.c_int, [C.c_void_p, C.c_double]),
+    "xbg_nc_write_var_step": (C.c_int, [C.c_void_p, C.c_char_p, _F]),
+    "xbg_nc_close": (C.c_int, [C.c_void_p]),
 }
 
 
