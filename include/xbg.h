@@ -296,6 +296,33 @@ int xbg_nc_write_time(xbg_nc* nc, double totaltime);
 int xbg_nc_write_var_step(xbg_nc* nc, const char* name, const float* var);
 int xbg_nc_close(xbg_nc* nc);
 
+/* ------------------------------------------------------------------------
+ * Host driver (input files, boundary bookkeeping, main loop, output):
+ * the reference's main program and mainloopGPU (Wave_gpu.cu:610-937,
+ * 1639-2990) over this C-ABI.
+ * ---------------------------------------------------------------------- */
+
+/* The non-numeric part of XBGPUParam (XBeachGPU.h:116-139). */
+typedef struct xbg_inputs {
+    char bathy[512], outfile[512], sedthk[512], wavebndfile[512], slbndfile[512], windfile[512];
+    int wavebndtype;       /* 1 constant (ReadCstBnd), 4 JONSWAP (ReadJSWPBnd) */
+    double dt;             /* fixed time step key (read, not used by the GPU loop) */
+    char outvars[2048];    /* comma-separated output variable names */
+} xbg_inputs;
+
+/* Defaults of the non-numeric part (outfile XBGPU_output.nc, wavebndtype 2). */
+void xbg_default_inputs(xbg_inputs* in);
+
+/* readparamstr (read_input.cpp:595-1225) for one "key = value;" line of
+ * XBG_param.txt.  thetamin/thetamax/dtheta are converted to radians. */
+int xbg_param_line(const char* line, xbg_params* p, xbg_inputs* in);
+
+/* Run the case described by a parameter file (other file names relative to
+ * its directory) to endtime, writing the netCDF output: the reference's
+ * XBeach_GPU executable.  Returns 0, or non-zero with xbg_run_error(). */
+int xbg_run_case(const char* paramfile, int verbose);
+const char* xbg_run_error(void);
+
 int xbg_sync(xbg_ctx* ctx);
 void* xbg_stream(xbg_ctx* ctx);   /* the context's hipStream_t */
 

@@ -1,0 +1,539 @@
+// Host driver of the surf-beat model over libxbg.so: the reference's main
+// program (Wave_gpu.cu:1639-2990) and main loop (mainloopGPU,
+// Wave_gpu.cu:610-937) on the C-ABI.
+//
+//   XBG_param.txt          readparamstr / findparameter (read_input.cpp:595-1474)
+//   bathymetry .md/.dep/.bot  readbathyHead / readbathy / readXBbathy
+//                          (read_input.cpp:1508-1634)
+//   sea level, wind        readWLfile / readWNDfile (read_input.cpp:259-405)
+//   wave boundary          type 1 ReadCstBnd + GenCstWave, type 4 ReadJSWPBnd +
+//                          makjonswap + GenWGnLBW (read_input.cpp:407-533,
+//                          Wavestep.cu:24-382, 512-753)
+//   sanity / defaults      checkparamsanity (read_input.cpp:1227-1444)
+//   initial state          Wave_gpu.cu:1988-2056, 2629-2691
+//   output                 netCDF (creatncfileUD / defncvar / writencvarstep)
+//
+// Exposed as xbg_run_case() (and the xbg_run executable); the parameter-line
+// parser is exposed as xbg_param_line() for tests.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/xbg.h"
+
+namespace {
+
+const double PI_ = 3.14159265;  // read_input.cpp / Wave_gpu.cu (double literal)
+
+std::vector<std::string> split(const std::string& s, char delim) {
+    std::vector<std::string> out;
+    std::stringstream ss(s);
+    std::string item;
+    while (std::getline(ss, item, delim))
+        if (!item.empty()) out.push_back(item);
+    return out;
+}
+std::string trim(const std::string& str, const std::string& ws = " ") {
+    size_t b = str.find_first_not_of(ws);
+    if (b == std::string::npos) return "";
+    size_t e = str.find_last_not_of(ws);
+    return str.substr(b, e - b + 1);
+}
+// read_input.cpp:1446-1474: "key = value ; comment"; the key must match
+// exactly; the value is the text up to the first ';', trimmed.
+std::string findparameter(const std::string& key, const std::string& line) {
+    std::vector<std::string> sp = split(line, '=');
+    std::string val;
+    if (sp.size() > 1) {
+        if (trim(sp[0]) == key) {
+            std::vector<std::string> r = split(sp[1], ';');
+            if (!r.empty()) val = r[0];
+        }
+    }
+    return trim(val);
+}
+// split a data line by tab, then space, then comma (read_input.cpp:487-505)
+std::vector<std::string> fields(const std::string& line, size_t need) {
+    std::vector<std::string> e = split(line, '\t');
+    if (e.size() < need) e = split(line, ' ');
+    if (e.size() < need) e = split(line, ',');
+    return e;
+}
+void copystr(char* dst, size_t n, const std::string& s) {
+    std::strncpy(dst, s.c_str(), n - 1);
+    dst[n - 1] = 0;
+}
+
+struct SL {
+    double time, wlev0, wlev1;
+};
+struct Wind {
+    double time, U, V;
+};
+
+std::string g_err;
+int fail(const std::string& m) {
+    g_err = m;
+    return XBG_E_ARG;
+}
+
+std::string join_dir(const std::string& dir, const std::string& f) {
+    if (f.empty() || f[0] == '/' || dir.empty()) return f;
+    return dir + "/" + f;
+}
+
+}  // namespace
+
+extern "C" {
+
+// readparamstr (read_input.cpp:595-1225) for one line.  Angles thetamin,
+// thetamax, dtheta arrive in degrees and are stored in radians; grdalpha
+// and lat stay in degrees (converted where the reference converts them).
+int xbg_param_line(const char* cline, xbg_params* p, xbg_inputs* in) {
+    if (!cline || !p || !in) return XBG_E_ARG;
+    std::string line(cline);
+    std::string v;
+    auto num = [&](const char* k, double& dst) {
+        v = findparameter(k, line);
+        if (!v.empty()) dst = std::stod(v);
+    };
+    auto inum = [&](const char* k, int& dst) {
+        v = findparameter(k, line);
+        if (!v.empty()) dst = std::stoi(v);
+    };
+    auto str = [&](const char* k, char* dst, size_t n) {
+        v = findparameter(k, line);
+        if (!v.empty()) copystr(dst, n, v);
+    };
+    str("bathy", in->bathy, sizeof(in->bathy));
+    str("depfile", in->bathy, sizeof(in->bathy));
+    inum("swave", p->swave);
+    inum("flow", p->flow);
+    inum("sedtrans", p->sedtrans);
+    inum("morphology", p->morphology);
+    inum("roller", p->roller);
+    num("eps", p->eps);
+    num("cf", p->cf);
+    num("cfsand", p->cfsand);
+    num("cfreef", p->cfreef);
+    num("nuh", p->nuh);
+    num("nuhfac", p->nuhfac);
+    inum("usesmago", p->usesmago);
+    num("smag", p->smag);
+    num("lat", p->lat);
+    num("Cd", p->Cd);
+    num("wci", p->wci);
+    num("hwci", p->hwci);
+    num("fc", p->hwci);  // the reference stores fc into hwci (read_input.cpp:754-759)
+    inum("breakmodel", p->breakmodel);
+    num("gamma", p->gammaa);
+    num("n", p->n);
+    num("alpha", p->alpha);
+    num("gammax", p->gammax);
+    num("beta", p->beta);
+    num("fw", p->fw);
+    num("fwsand", p->fwsand);
+    num("fwreef", p->fwreef);
+    num("D50", p->D50);
+    num("D90", p->D90);
+    num("rhosed", p->rhosed);
+    num("wws", p->wws);
+    num("drydzmax", p->drydzmax);
+    num("wetdzmax", p->wetdzmax);
+    num("maxslpchg", p->maxslpchg);
+    num("por", p->por);
+    num("morfac", p->morfac);
+    num("sus", p->sus);
+    num("bed", p->bed);
+    num("facsk", p->facsk);
+    num("facas", p->facas);
+    num("dt", in->dt);
+    num("CFL", p->CFL);
+    num("sedstart", p->sedstart);
+    num("outputtimestep", p->outputtimestep);
+    num("endtime", p->endtime);
+    str("outfile", in->outfile, sizeof(in->outfile));
+    str("SedThkfile", in->sedthk, sizeof(in->sedthk));
+    str("wavebndfile", in->wavebndfile, sizeof(in->wavebndfile));
+    str("slbndfile", in->slbndfile, sizeof(in->slbndfile));
+    str("windbndfile", in->windfile, sizeof(in->windfile));
+    v = findparameter("outvars", line);
+    if (!v.empty()) {
+        // read_input.cpp:1068-1094: keep the supported names, in order
+        static const char* sup[] = {"hh", "uu", "vv", "wci", "zs", "zb", "cfm", "dzb", "stdep", "Fx", "Fy",
+                                    "cgx", "cgy", "cx", "cy", "ctheta", "D", "E", "H", "urms", "ueu", "vev",
+                                    "thetamean", "Hmean", "uumean", "vvmean", "hhmean", "zsmax", "zsmean",
+                                    "Cmean", "sigm", "k", "c", "kh", "cg", "sinh2kh", "dhdx", "dhdy", "dudx",
+                                    "dudy", "dvdx", "dvdy", "C", "R", "DR", "ee", "vmageu", "vmagev", "dzsdx",
+                                    "dzsdy", "dzsdt", "fwm", "hu", "hum", "hv", "hvm", "uv", "vu", "ududx",
+                                    "vdvdy", "udvdx", "vdudy", "ust", "rr", "kturb", "rolthick", "ceqsg"};
+        std::string acc = in->outvars;
+        for (auto& nv : split(v, ',')) {
+            std::string vv = trim(nv);
+            for (const char* s : sup)
+                if (vv == s) {
+                    if (!acc.empty()) acc += ",";
+                    acc += vv;
+                    break;
+                }
+        }
+        copystr(in->outvars, sizeof(in->outvars), acc);
+    }
+    inum("wavebndtype", in->wavebndtype);
+    v = findparameter("thetamin", line);
+    if (!v.empty()) p->thetamin = std::stod(v) * PI_ / 180.0;
+    v = findparameter("thetamax", line);
+    if (!v.empty()) p->thetamax = std::stod(v) * PI_ / 180.0;
+    v = findparameter("dtheta", line);
+    if (!v.empty()) p->dtheta = std::stod(v) * PI_ / 180.0;
+    num("dtbc", p->dtbc);
+    num("rtlength", p->rtlength);
+    num("sprdthr", p->sprdthr);
+    inum("random", p->random);
+    inum("nx", p->nx);
+    inum("ny", p->ny);
+    num("dx", p->dx);
+    num("grdalpha", p->grdalpha);
+    num("g", p->g);
+    num("rho", p->rho);
+    num("nmax", p->nmax);
+    num("fcutoff", p->fcutoff);
+    inum("nspr", p->nspr);
+    return XBG_OK;
+}
+
+void xbg_default_inputs(xbg_inputs* in) {
+    std::memset(in, 0, sizeof(*in));
+    copystr(in->outfile, sizeof(in->outfile), "XBGPU_output.nc");  // XBeachGPU.h:122
+    in->wavebndtype = 2;                                            // XBeachGPU.h:119
+}
+
+const char* xbg_run_error(void) { return g_err.c_str(); }
+
+int xbg_run_case(const char* paramfile, int verbose) {
+    if (!paramfile) return fail("NULL parameter file");
+    std::string pf(paramfile), dir;
+    size_t sl = pf.find_last_of('/');
+    if (sl != std::string::npos) dir = pf.substr(0, sl);
+    xbg_params P;
+    xbg_default_params(&P);
+    xbg_inputs IN;
+    xbg_default_inputs(&IN);
+    {
+        std::ifstream fs(pf);
+        if (fs.fail()) return fail(pf + " could not be opened");
+        std::string line;
+        while (std::getline(fs, line))
+            if (!line.empty() && line[0] != '#') xbg_param_line(line.c_str(), &P, &IN);
+    }
+    // ---- bathymetry (Wave_gpu.cu:1740-1780, read_input.cpp:1508-1634)
+    std::string bathy = join_dir(dir, IN.bathy);
+    if (bathy.empty()) return fail("No bathymetry file specified");
+    std::string ext = split(bathy, '.').back();
+    std::vector<float> zb;
+    if (ext == "md") {
+        std::ifstream fs(bathy);
+        if (fs.fail()) return fail(bathy + " could not be opened");
+        std::string line;
+        std::getline(fs, line);
+        auto e = fields(line, 5);
+        if (e.size() < 5) return fail(bathy + ": header needs nx ny dx _ grdalpha");
+        P.nx = std::stoi(e[0]);
+        P.ny = std::stoi(e[1]);
+        P.dx = std::stod(e[2]);
+        P.grdalpha = std::stod(e[4]);
+        zb.assign((size_t)P.nx * P.ny, 0.0f);
+        // rows are written from ny down to 1, each prefixed by its 1-based index
+        for (int f = P.ny; f >= 1; f--) {
+            int j;
+            if (!(fs >> j) || j < 1 || j > P.ny) return fail(bathy + ": bad row index");
+            for (int i = 0; i < P.nx; i++)
+                if (!(fs >> zb[i + (size_t)(j - 1) * P.nx])) return fail(bathy + ": short row");
+        }
+    } else if (ext == "dep" || ext == "bot") {
+        if (P.nx <= 0 || P.ny <= 0 || P.dx < 1e-7) return fail("nx, ny and dx are required with ." + ext);
+        std::ifstream fs(bathy);
+        if (fs.fail()) return fail(bathy + " could not be opened");
+        zb.assign((size_t)P.nx * P.ny, 0.0f);
+        for (size_t k = 0; k < zb.size(); k++)
+            if (!(fs >> zb[k])) return fail(bathy + ": too few values");
+    } else {
+        return fail("bathymetry format ." + ext + " is not supported (use .md, .dep or .bot)");
+    }
+    P.grdalpha = P.grdalpha * PI_ / 180;  // Wave_gpu.cu:1770
+    // ---- sea level and wind (Wave_gpu.cu:1783-1830)
+    std::vector<SL> slb;
+    if (IN.slbndfile[0]) {
+        std::ifstream fs(join_dir(dir, IN.slbndfile));
+        if (fs.fail()) return fail(std::string(IN.slbndfile) + " could not be opened");
+        std::string line;
+        while (std::getline(fs, line)) {
+            if (line.empty()) continue;
+            auto e = fields(line, 2);
+            if (e.size() < 2) return fail("sea-level file format");
+            SL s{std::stod(e[0]), std::stod(e[1]), std::stod(e[1])};
+            if (e.size() >= 3) s.wlev1 = std::stod(e[2]);
+            slb.push_back(s);
+        }
+    } else {
+        slb = {{0.0, 0.0, 0.0}, {P.endtime, 0.0, 0.0}};
+    }
+    std::vector<Wind> wnd;
+    if (IN.windfile[0]) {
+        std::ifstream fs(join_dir(dir, IN.windfile));
+        if (fs.fail()) return fail(std::string(IN.windfile) + " could not be opened");
+        std::string line;
+        while (std::getline(fs, line)) {
+            if (line.empty()) continue;
+            auto e = fields(line, 3);
+            if (e.size() < 3) return fail("wind file format");
+            double spd = std::stod(e[1]), th = (1.5 * PI_ - P.grdalpha) - std::stod(e[2]) * PI_ / 180;
+            wnd.push_back({std::stod(e[0]), spd * std::cos(th), spd * std::sin(th)});
+        }
+    } else {
+        double te = std::max(P.endtime, slb.back().time);
+        wnd = {{0.0, 0.0, 0.0}, {te, 0.0, 0.0}};
+    }
+    // ---- wave boundary list (Wave_gpu.cu:1833-1880)
+    std::vector<xbg_wavebnd> wb;
+    if (IN.wavebndfile[0]) {
+        std::ifstream fs(join_dir(dir, IN.wavebndfile));
+        if (fs.fail()) return fail(std::string(IN.wavebndfile) + " could not be opened");
+        if (IN.wavebndtype != 1 && IN.wavebndtype != 4)
+            return fail("wavebndtype " + std::to_string(IN.wavebndtype) + " is not supported (use 1 or 4)");
+        size_t need = IN.wavebndtype == 4 ? 6 : 5;
+        std::string line;
+        while (std::getline(fs, line)) {
+            if (line.empty()) continue;
+            auto e = fields(line, need);
+            if (e.size() < need) return fail("wave boundary file format");
+            xbg_wavebnd w{};
+            w.time = std::stod(e[0]);
+            w.Hs = std::stod(e[1]);
+            w.Tp = std::stod(e[2]);
+            w.Dp = (1.5 * PI_ - P.grdalpha) - std::stod(e[3]) * PI_ / 180;
+            w.s = std::stod(e[4]);
+            w.gamma = need == 6 ? std::stod(e[5]) : 3.3;
+            wb.push_back(w);
+        }
+        if (wb.empty()) return fail("empty wave boundary file");
+        if (IN.wavebndtype == 4) {
+            xbg_wavebnd last = wb.back();
+            last.time = wb.back().time + P.rtlength;  // read_input.cpp:528-529
+            wb.push_back(last);
+        }
+    } else {
+        IN.wavebndtype = 1;
+        double te = std::max(P.endtime, slb.back().time);
+        xbg_wavebnd w{0.0, 0.0001, 10.0, (1.5 * PI_ - P.grdalpha), 1000.0, 3.3};
+        wb.push_back(w);
+        w.time = te;
+        wb.push_back(w);
+    }
+    // ---- checkparamsanity (read_input.cpp:1227-1444)
+    const double tiny = 0.0000001;
+    if (P.nx <= 0 || P.ny <= 0 || P.dx < tiny) return fail("nx or ny or dx could not be specified");
+    if (std::fabs(P.endtime - 0.0) <= tiny) P.endtime = 1.0 / tiny;
+    double endbnd = std::min(slb.back().time, wnd.back().time);
+    endbnd = std::min(endbnd, wb.back().time);
+    P.endtime = std::min(P.endtime, endbnd);
+    if (P.swave == 0) {
+        P.ntheta = 1;
+        P.dtheta = (P.thetamax - P.thetamin);
+    }
+    if (std::fabs(P.outputtimestep - 0.0) <= tiny) P.outputtimestep = P.endtime;
+    if (std::fabs(P.cfsand - 0.0) < tiny && P.cf < tiny) P.cf = tiny;
+    if (std::fabs(P.fwsand - 0.0) < tiny && P.fw < tiny) P.fw = tiny;
+    std::vector<std::string> outvars = split(IN.outvars, ',');
+    if (outvars.empty() && P.outputtimestep > 0)
+        outvars = {"zb", "zs", "uu", "vv", "H", "thetamean", "D", "urms", "ueu", "vev", "C",
+                   "dzb", "Fx", "Fy", "hh", "Hmean", "uumean", "vvmean", "hhmean", "zsmean", "Cmean"};
+    xbg_finalize_params(&P);
+    // ---- initial state (Wave_gpu.cu:1988-2056)
+    const int nx = P.nx, ny = P.ny, nt = P.ntheta;
+    const size_t N = (size_t)nx * ny;
+    std::vector<float> zs(N), hh(N), zero(N, 0.0f), stdep(N, 5.0f);
+    for (int f = ny; f >= 1; f--)
+        for (int i = 0; i < nx; i++) {
+            size_t k = i + (size_t)(f - 1) * nx;
+            double zsbnd = slb[0].wlev0 + (slb[0].wlev1 - slb[0].wlev0) * (f - 1) / ny;
+            zs[k] = std::max((float)zsbnd, -1 * zb[k]);
+            hh[k] = (float)std::max(zb[k] + zsbnd, P.eps);
+        }
+    const double offdepth = hh[0 + (size_t)std::floor(ny * 0.5) * nx];
+    if (IN.sedthk[0]) {
+        std::ifstream fs(join_dir(dir, IN.sedthk));
+        if (fs.fail()) return fail(std::string(IN.sedthk) + " could not be opened");
+        std::string line;
+        std::getline(fs, line);
+        for (int f = ny; f >= 1; f--) {
+            int j;
+            if (!(fs >> j) || j < 1 || j > ny) return fail("sediment thickness file: bad row index");
+            for (int i = 0; i < nx; i++)
+                if (!(fs >> stdep[i + (size_t)(j - 1) * nx])) return fail("sediment thickness file: short row");
+        }
+    }
+    // ---- wave boundary records (waveinitGPU, Wavestep.cu:24-382)
+    const int nrec_blk = IN.wavebndtype == 1 ? (int)wb.size() : xbg_bnd_nrec(&P);
+    std::vector<double> Stfile((size_t)nrec_blk * nt * ny), qfile((size_t)nrec_blk * 4 * ny);
+    double Trep = 10.0;
+    std::vector<float> theta(nt), cxs(nt), sxn(nt);
+    xbg_theta_tables(&P, theta.data(), cxs.data(), sxn.data());
+    auto gen_block = [&](const xbg_wavebnd& w) -> int {
+        int nf, nd;
+        if (xbg_jonswap_size(&w, &nf, &nd)) return fail("jonswap size");
+        std::vector<double> fr(nf), di(nd), S((size_t)nf * nd);
+        if (xbg_jonswap(&w, fr.data(), di.data(), S.data())) return fail("JONSWAP spectrum: NaN");
+        unsigned seed = 0;
+        if (P.random == 1) seed = (unsigned)std::time(nullptr);
+        return xbg_wave_groups(&P, offdepth, nf, nd, fr.data(), di.data(), S.data(), seed, &Trep, Stfile.data(),
+                               qfile.data());
+    };
+    if (P.swave == 1) {
+        if (IN.wavebndtype == 1) {
+            std::vector<double> Tp(wb.size());
+            if (xbg_constant_waves(&P, (int)wb.size(), wb.data(), theta.data(), Stfile.data(), qfile.data(), Tp.data()))
+                return fail("constant wave boundary");
+            Trep = Tp[0];
+        } else if (gen_block(wb[0])) {
+            return XBG_E_ARG;
+        }
+    }
+    // ---- device state
+    xbg_ctx* ctx = nullptr;
+    int rc = xbg_create(&P, &ctx);
+    if (rc) return fail(std::string("xbg_create: ") + xbg_last_error(nullptr));
+    auto chk = [&](int r, const char* what) {
+        if (r && g_err.empty()) g_err = std::string(what) + ": " + xbg_last_error(ctx);
+        return r;
+    };
+    std::vector<float> ee(N * nt, 0.0f);
+    if (P.swave == 1)
+        for (int t = 0; t < nt; t++)
+            for (int j = 0; j < ny; j++) ee[(size_t)j * nx + (size_t)t * N] = (float)Stfile[j + (size_t)t * ny];
+    rc = chk(xbg_set_field(ctx, "zb", zb.data(), N), "zb") || chk(xbg_set_field(ctx, "zs", zs.data(), N), "zs") ||
+         chk(xbg_set_field(ctx, "hh", hh.data(), N), "hh") || chk(xbg_set_field(ctx, "stdep", stdep.data(), N), "stdep") ||
+         chk(xbg_set_field(ctx, "ee", ee.data(), N * nt), "ee") || chk(xbg_init_state(ctx, Trep), "init");
+    if (!rc && P.swave == 1) rc = chk(xbg_set_bnd_series(ctx, Stfile.data(), qfile.data(), nrec_blk), "bnd series");
+    double dtf = 0, dtw = 0, dt = 0;
+    if (!rc) rc = chk(xbg_cfl_dt(ctx, 1, &dtf, &dtw), "initial dt");
+    dt = dtf * 0.5;  // Wave_gpu.cu:2640
+    if (!rc && dt < 0.00000001) rc = fail("timestep too small");
+    // ---- output file (Wave_gpu.cu:2940-2959)
+    xbg_nc* nc = nullptr;
+    std::vector<float> buf;
+    auto ndim = [&](const std::string& v) {
+        if (v == "ee" || v == "rr") return 4;
+        if (v == "zb") return P.morphology == 1 ? 3 : 2;
+        return 3;
+    };
+    auto fetch = [&](const std::string& v) -> const float* {
+        size_t n = xbg_field_size(ctx, v.c_str());
+        if (n == 0) {
+            buf.assign(ndim(v) == 4 ? N * nt : N, 0.0f);  // not held on the device: zeros
+            return buf.data();
+        }
+        buf.resize(n);
+        if (xbg_get_field(ctx, v.c_str(), buf.data(), n)) return nullptr;
+        return buf.data();
+    };
+    if (!rc && !outvars.empty() && P.outputtimestep > 0) {
+        std::string of = join_dir(dir, IN.outfile);
+        nc = xbg_nc_create(of.c_str(), nx, ny, nt, P.dx, P.dtheta, P.thetamin, 0.0);
+        if (!nc) rc = fail(of + " could not be created");
+        for (auto& v : outvars) {
+            if (rc) break;
+            const float* d = fetch(v);
+            if (!d || xbg_nc_def_var(nc, v.c_str(), ndim(v), d)) rc = fail("output variable " + v);
+        }
+    }
+    double totaltime = 0.0, nextoutputtime = P.outputtimestep;
+    int WAV = 1, SLs = 1, WNs = 1, nstep = 0;
+    if (!rc) rc = chk(xbg_reset_means(ctx), "means");
+    // ---- main loop (Wave_gpu.cu:610-937)
+    while (!rc && totaltime < P.endtime) {
+        nstep++;
+        double wdt = dt;
+        if ((rc = chk(xbg_cfl_dt(ctx, 0, &dtf, &dtw), "cfl"))) break;
+        rc = xbg_next_dt(dtf, dtw, P.swave == 1, wdt, totaltime, nextoutputtime, P.endtime, &dt);
+        if (rc) {
+            g_err = "Model crashed!";
+            break;
+        }
+        totaltime = totaltime + dt;
+        // wavebnd (Wavestep.cu:512-753): record list / block bookkeeping
+        int rec = 0;
+        double tnext = 1.0, tsl = 0.0;
+        if (P.swave == 1) {
+            if (wb[WAV].time - totaltime < 0.0) {
+                WAV++;
+                if (WAV >= (int)wb.size()) WAV = (int)wb.size() - 1;
+                if (IN.wavebndtype == 4) {
+                    if (gen_block(wb[WAV - 1])) {
+                        rc = XBG_E_ARG;
+                        break;
+                    }
+                    if ((rc = chk(xbg_set_bnd_series(ctx, Stfile.data(), qfile.data(), nrec_blk), "bnd series"))) break;
+                }
+            }
+            if (IN.wavebndtype == 1) {
+                rec = WAV - 1;
+                tnext = wb[WAV].time - wb[WAV - 1].time;
+                tsl = totaltime - wb[WAV - 1].time;
+            } else {
+                rec = (int)std::min(std::floor((totaltime - wb[WAV - 1].time) / P.dtbc),
+                                    std::ceil(P.rtlength / P.dtbc) - 1);
+                tnext = P.dtbc;
+                tsl = totaltime - (rec * P.dtbc + wb[WAV - 1].time);
+            }
+        }
+        // flowbnd (Wave_gpu.cu:1020-1103): sea level, then wind
+        if (slb[SLs].time - totaltime < 0.0 && SLs + 1 < (int)slb.size()) SLs++;
+        double zsbndi = xbg_interptime(slb[SLs].wlev0, slb[SLs - 1].wlev0, slb[SLs].time - slb[SLs - 1].time,
+                                       totaltime - slb[SLs - 1].time);
+        double zsbndn = xbg_interptime(slb[SLs].wlev1, slb[SLs - 1].wlev1, slb[SLs].time - slb[SLs - 1].time,
+                                       totaltime - slb[SLs - 1].time);
+        if (wnd[WNs].time - totaltime < 0.0 && WNs + 1 < (int)wnd.size()) WNs++;
+        double windu = xbg_interptime(wnd[WNs].U, wnd[WNs - 1].U, wnd[WNs].time - wnd[WNs - 1].time,
+                                      totaltime - wnd[WNs - 1].time);
+        double windv = xbg_interptime(wnd[WNs].V, wnd[WNs - 1].V, wnd[WNs].time - wnd[WNs - 1].time,
+                                      totaltime - wnd[WNs - 1].time);
+        if (P.swave == 1) {
+            if ((rc = chk(xbg_use_bnd_record(ctx, rec, tsl, tnext, Trep, zsbndi, zsbndn, windu, windv), "bnd"))) break;
+        } else {
+            if ((rc = chk(xbg_set_flow_forcing(ctx, nullptr, nullptr, tsl, tnext, zsbndi, zsbndn, windu, windv),
+                          "flow forcing")))
+                break;
+        }
+        if ((rc = chk(xbg_step(ctx, dt, totaltime), "step"))) break;
+        if ((rc = chk(xbg_accumulate_means(ctx), "means"))) break;
+        if (nextoutputtime - totaltime <= dt * 0.001 && P.outputtimestep > 0) {
+            nextoutputtime = nextoutputtime + P.outputtimestep;
+            if ((rc = chk(xbg_divide_means(ctx, nstep), "means"))) break;
+            if (nc) {
+                if (xbg_nc_write_time(nc, totaltime)) rc = fail("output time");
+                for (auto& v : outvars) {
+                    if (rc || ndim(v) == 2) continue;
+                    const float* d = fetch(v);
+                    if (!d || xbg_nc_write_var_step(nc, v.c_str(), d)) rc = fail("output " + v);
+                }
+            }
+            if (verbose) printf("Writing output, totaltime:%f s, Mean dt=%f\n", totaltime, P.outputtimestep / nstep);
+            if (!rc) rc = chk(xbg_reset_means(ctx), "means");
+            nstep = 0;
+        }
+    }
+    if (nc) xbg_nc_close(nc);
+    xbg_destroy(ctx);
+    if (verbose && rc) fprintf(stderr, "%s\n", g_err.c_str());
+    return rc;
+}
+
+}  // extern "C"
