@@ -1,0 +1,43 @@
+"""Per-field error of the FP32 step (the reference's precision, which the
+HIP kernels reproduce bit for bit) against the FP64 build of the same
+oracle, on the small seeded spun-up case (200 x 50, 12 bins).  Measured:
+after 1 step 1.7e-8 (zs) .. 7.2e-8 (C); after 100 steps <= 2e-6; after 1000
+steps zs 1.6e-4, uu 8.1e-4, vv 1.3e-3, ee 1.5e-5, H 6.5e-6, C 4.8e-4,
+zb 1.7e-6 (chaotic surf-zone flow amplifies rounding)."""
+import numpy as np
+import pytest
+
+from oracle.pyorc import Oracle
+from xbeach_gpu_amd import setups
+
+FIELDS = ("zs", "uu", "vv", "ee", "H", "C", "zb")
+
+
+def rel(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("nsteps,tol", [(1, 2e-7), (100, 1e-5), (1000, 5e-3)])
+def test_fp32_step_against_fp64_build(nsteps, tol):
+    p = setups.small_params()
+    elev = setups.small_bathy(p.nx, p.ny, p.dx)
+    st = setups.spun_up_state(p, elev, seed=1234)
+    frc = setups.GroupForcing(p, 1.5, 10.0, seed=1234)
+    o4, o8 = Oracle(p, 4), Oracle(p, 8)
+    for k, v in st.items():
+        o4.set(k, v)
+        o8.set(k, v)
+    o4.init_state(frc.Trep)
+    o8.init_state(frc.Trep)
+    t, dt = 150.0, 0.15
+    for _ in range(nsteps):
+        t += dt
+        St, qo, qn = frc.St(t), frc.qbnd(t - 0.5), frc.qbnd(t + 0.5)
+        for o in (o4, o8):
+            o.set_forcing(St, qo, qn, frc.Trep, 0.5, 1.0)
+            o.step(dt, t)
+    errs = {k: rel(o4.get(k), o8.get(k)) for k in FIELDS}
+    print(f"FP32 vs FP64 after {nsteps} steps:", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert all(e <= tol for e in errs.values()), errs
