@@ -7,21 +7,14 @@ import math
 import pytest
 
 from xbeach_gpu_amd import default_params, lib
+from xbeach_gpu_amd._ffi import XbgInputs as Inputs
 from xbeach_gpu_amd._ffi import XbgParams
 
 PI_REF = 3.14159265
 
 
-class Inputs(C.Structure):
-    _fields_ = [("bathy", C.c_char * 512), ("outfile", C.c_char * 512), ("sedthk", C.c_char * 512),
-                ("wavebndfile", C.c_char * 512), ("slbndfile", C.c_char * 512), ("windfile", C.c_char * 512),
-                ("wavebndtype", C.c_int), ("dt", C.c_double), ("outvars", C.c_char * 2048)]
-
-
 def parse(*lines):
     L = lib()
-    L.xbg_param_line.argtypes = [C.c_char_p, C.POINTER(XbgParams), C.POINTER(Inputs)]
-    L.xbg_default_inputs.argtypes = [C.POINTER(Inputs)]
     p = XbgParams()
     L.xbg_default_params(C.byref(p))
     inp = Inputs()
@@ -79,7 +72,6 @@ def test_outvars_keeps_supported_names_in_order():
 
 def test_run_case_reports_input_errors(tmp_path):
     L = lib()
-    L.xbg_run_error.restype = C.c_char_p
     pf = tmp_path / "XBG_param.txt"
     pf.write_text("nx = 10\nny = 8\n")
     assert L.xbg_run_case(str(pf).encode(), 0) != 0

@@ -32,7 +32,6 @@ def write_case(d, nx=64, ny=32, dx=5.0, endtime=30.0, out=10.0, extra=""):
 
 def test_run_case_writes_output(tmp_path):
     L = lib()
-    L.xbg_run_error.restype = C.c_char_p
     pf = write_case(tmp_path)
     rc = L.xbg_run_case(str(pf).encode(), 0)
     assert rc == 0, L.xbg_run_error()

@@ -113,7 +113,20 @@ PROTOTYPES = {
     "xbg_nc_write_time": (C.c_int, [C.c_void_p, C.c_double]),
     "xbg_nc_write_var_step": (C.c_int, [C.c_void_p, C.c_char_p, _F]),
     "xbg_nc_close": (C.c_int, [C.c_void_p]),
+    "xbg_default_inputs": (None, [C.c_void_p]),
+    "xbg_param_line": (C.c_int, [C.c_char_p, _P, C.c_void_p]),
+    "xbg_run_case": (C.c_int, [C.c_char_p, C.c_int]),
+    "xbg_run_error": (C.c_char_p, []),
 }
+
+
+class XbgInputs(C.Structure):
+    """Mirror of ``xbg_inputs`` (the file names and options of XBGPUParam,
+    XBeachGPU.h:116-139)."""
+
+    _fields_ = [("bathy", C.c_char * 512), ("outfile", C.c_char * 512), ("sedthk", C.c_char * 512),
+                ("wavebndfile", C.c_char * 512), ("slbndfile", C.c_char * 512), ("windfile", C.c_char * 512),
+                ("wavebndtype", C.c_int), ("dt", C.c_double), ("outvars", C.c_char * 2048)]
 
 
 class XbgWavebnd(C.Structure):
