@@ -139,11 +139,13 @@ _lib = None
 
 
 def lib(path: str | None = None) -> C.CDLL:
-    """Load libxbg.so once.  Raises if the HIP library has not been built."""
+    """Load libxbg.so once (the in-tree build, or the file named by the
+    XBG_LIB environment variable).  Raises if the HIP library has not been
+    built."""
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("XBG_LIB") or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(
             f"libxbg.so not found at {p}: build it with `python -c 'import __graft_entry__ as g; g.build()'`"

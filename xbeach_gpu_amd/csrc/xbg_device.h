@@ -40,6 +40,7 @@ struct KP {
     // operations and elementary functions as the per-cell code
     float sed_drho, sed_dester, sed_Ass, sed_ucrfac, rvr_alph, rvr_bet;
     int usesmago, breakmodel, roller, zb_accum, swave;
+    int wtiles_x, wtiles_y;  // wave-kernel tile grid
     float theta[MAXT], cxsth[MAXT], sxnth[MAXT];
 };
 

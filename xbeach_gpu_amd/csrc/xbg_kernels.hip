@@ -395,6 +395,32 @@ constexpr int PH = WTY + 4;
 constexpr int PN = PW * PH;
 constexpr int WTHREADS = BX * WTY;
 
+// Tile of a workgroup.  Workgroups are dealt to the 8 XCDs of the MI355X in
+// turn (workgroup b runs on XCD b % 8), each XCD with its own L2; the tiles
+// are renumbered so that each XCD works through one contiguous band of tile
+// rows, whose halo rows and columns its L2 then already holds.
+#ifndef XBG_XCD_REMAP
+#define XBG_XCD_REMAP 1
+#endif
+XDEV int xcd_tile(int b, int ntiles) {
+#if XBG_XCD_REMAP
+    const int NX = 8;
+    int per = (ntiles + NX - 1) / NX;
+    int tall = ntiles % NX == 0 ? NX : ntiles % NX;
+    int xcd = b % NX, local = b / NX;
+    return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+#else
+    return b;
+#endif
+}
+
+// Workgroup barrier for LDS traffic only: this wave's LDS operations have
+// completed, then all waves meet.  __syncthreads() would also wait for every
+// outstanding global load, which would pull the wait for the plane being
+// prefetched from HBM in front of the barrier; the planes read here are not
+// written by this kernel, so only LDS needs ordering.
+XDEV void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" : : : "memory"); }
+
 // Staging of one directional plane of the tile (+halo) into LDS.  Halo
 // cells hold the values at the clamped neighbour indices
 // (mminus/pplus/mminus2/pplus2 equal clamping), so the stencil reads are
@@ -405,47 +431,54 @@ constexpr int WTHREADS = BX * WTY;
 constexpr int NSLOT = (PN + WTHREADS - 1) / WTHREADS;
 struct Slots {
     int code[NSLOT];
-    bool bnd;  // tile touches the offshore column (x0 == 0): uniform per tile
 };
 XDEV Slots make_slots(const KP& p, int x0, int y0, int tid) {
     Slots sl;
-    sl.bnd = x0 == 0;
 #pragma unroll
     for (int it = 0; it < NSLOT; it++) {
         int l = tid + it * WTHREADS;
         int r = l / PW, c = l - r * PW;
         int gxx = min(max(x0 + c - 2, 0), p.nx - 1);
         int gyy = min(max(y0 + r - 2, 0), p.ny - 1);
-        sl.code[it] = gxx == 0 ? -(gyy + 1) : gxx + gyy * p.nx;
+        sl.code[it] = l >= PN ? 0 : (gxx == 0 ? -(gyy + 1) : gxx + gyy * p.nx);
     }
     return sl;
 }
 
+// The raw values of one plane (+halo) for the thread's staging slots: the
+// global loads are issued here and consumed by store_plane one bin later,
+// so their latency overlaps the arithmetic of the current bin.
 template <bool IS_EE>
-XDEV void fill_plane(float* pl, const KP& p, const SP& s, const Fld& f, const Slots& sl, int th, int tid) {
+XDEV void load_plane(float* raw, const KP& p, const Fld& f, const Slots& sl, int th) {
     // opaque bin index: keeps the compiler from precomputing (and holding
     // in registers) the plane addresses of every bin of the unrolled loop
     int thv = th;
     asm volatile("" : "+s"(thv));
     const uint32_t pbase = (uint32_t)thv * p.N;
     const float* plane = (IS_EE ? f.ee_in : f.rr_in) + pbase;
+    // one load per slot from a per-lane address (unused slots read the
+    // plane's first element): no divergent branches around the loads
+#pragma unroll
+    for (int it = 0; it < NSLOT; it++) {
+        int code = sl.code[it];
+        const float* src;
+        if (IS_EE)  // offshore column from the boundary spectrum
+            src = code >= 0 ? plane + (uint32_t)code : f.St + ((uint32_t)(-code - 1) + (uint32_t)thv * p.ny);
+        else        // rr: the offshore column carries its own value
+            src = plane + (code >= 0 ? (uint32_t)code : (uint32_t)(-code - 1) * p.nx);
+        raw[it] = *src;
+    }
+}
+template <bool IS_EE>
+XDEV void store_plane(float* pl, const SP& s, const Slots& sl, const float* raw, int tid) {
 #pragma unroll
     for (int it = 0; it < NSLOT; it++) {
         int l = tid + it * WTHREADS;
         if (l < PN) {
-            int code = sl.code[it];
-            float v;
+            float v = raw[it];
             if (IS_EE) {
-                // only the tiles of the first column hold offshore cells
-                if (!sl.bnd || code >= 0) {
-                    v = plane[(uint32_t)code];
-                } else {
-                    v = f.St[(uint32_t)(-code - 1) + (uint32_t)thv * p.ny] * s.taper;
-                }
-                v = fmx(v, 0.000f) / s.sigm;  // sanity + action (wave_CPU.cpp:58-82, 289-309)
-            } else {
-                // rr: the offshore column carries its own value
-                v = plane[code >= 0 ? (uint32_t)code : (uint32_t)(-code - 1) * p.nx];
+                if (sl.code[it] < 0) v = v * s.taper;  // offshorebndWav (wave_CPU.cpp:24-56)
+                v = fmx(v, 0.000f) / s.sigm;           // sanity + action (wave_CPU.cpp:58-82, 289-309)
             }
             pl[l] = v;
         }
@@ -499,25 +532,35 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
         }
         ea[b] = e0[b] - s.dt * (ea[b] + ta);
     };
-    fill_plane<IS_EE>(pl[0], p, s, f, sl, 0, tid);
-    __syncthreads();
+    // plane th is read from LDS while plane th+1 is written to the other
+    // buffer (from registers loaded one bin earlier) and plane th+2 is in
+    // flight from HBM
+    float raw[NSLOT];
+    load_plane<IS_EE>(raw, p, f, sl, 0);
+    store_plane<IS_EE>(pl[0], s, sl, raw, tid);
+    if (nt > 1) load_plane<IS_EE>(raw, p, f, sl, 1);
+    lds_barrier();
     const int c0 = (cy + 2) * PW + (cx + 2);
 #pragma unroll
     for (int th = 0; th < A; th++) {
         if (th >= nt) break;
-        if (th + 1 < nt) fill_plane<IS_EE>(pl[(th + 1) & 1], p, s, f, sl, th + 1, tid);
         const float* q = pl[th & 1];
         float ec = q[c0];
         e0[th] = ec;
         ea[th] = adv_xy(p, f.dirs, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
                         q[c0 + PW], q[c0 + 2 * PW]);
         if (th >= 2) finish(th - 2);
-        // materialise this bin's results before the barrier: otherwise the
-        // arithmetic is sunk past all barriers and the nine stencil values of
-        // every bin stay live in registers
-        asm volatile("" : "+v"(ea[th]));
-        if (th >= 2) asm volatile("" : "+v"(ea[th - 2]));
-        __syncthreads();
+        // materialise this bin's results (otherwise the arithmetic is sunk
+        // past the barriers and the stencil values of every bin stay live),
+        // and keep the staging of the next plane behind them (its wait for
+        // HBM must not move ahead of this bin's arithmetic)
+        asm volatile("" : "+v"(ea[th]) : : "memory");
+        if (th >= 2) asm volatile("" : "+v"(ea[th - 2]) : : "memory");
+        if (th + 1 < nt) {
+            store_plane<IS_EE>(pl[(th + 1) & 1], s, sl, raw, tid);
+            if (th + 2 < nt) load_plane<IS_EE>(raw, p, f, sl, th + 2);
+        }
+        lds_barrier();
     }
 #pragma unroll
     for (int b = 0; b < A; b++) {
@@ -542,7 +585,8 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     __shared__ float wred[WTHREADS / 64];
     __shared__ float stash[NT > 0 ? NT : 1][WTHREADS];
     const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * WTY;
+    const int tile = xcd_tile(blockIdx.x, p.wtiles_x * p.wtiles_y);
+    const int x0 = (tile % p.wtiles_x) * BX, y0 = (tile / p.wtiles_x) * WTY;
     const int nx = p.nx, ny = p.ny;
     const int x = min(x0 + cx, nx - 1), y = min(y0 + cy, ny - 1);
     const bool valid = (x0 + cx < nx) && (y0 + cy < ny);
@@ -609,7 +653,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     if (tid == 0) {
         float m = wred[0];
         for (int w = 1; w < WTHREADS / 64; w++) m = fmn(m, wred[w]);
-        f.red_wave[blockIdx.x + blockIdx.y * gridDim.x] = m;
+        f.red_wave[tile] = m;
     }
 }
 
@@ -1654,6 +1698,8 @@ void make_kp(xbg_ctx* c) {
     k.sed_ucrfac = k.D50 <= 0.0005f ? 0.19f * xm::powf(k.D50, 0.1f) : 8.50f * xm::powf(k.D50, 0.6f);
     k.rvr_alph = -1.0f * xm::log10f(xm::expf(1.0f)) / 0.3539f;
     k.rvr_bet = xm::expf(-0.6065f / 0.3539f);
+    k.wtiles_x = (P.nx + BX - 1) / BX;
+    k.wtiles_y = (P.ny + WTY - 1) / WTY;
     k.usesmago = P.usesmago; k.breakmodel = P.breakmodel; k.roller = P.roller;
     k.zb_accum = P.zb_accum; k.swave = P.swave;
     xbg_theta_tables(&P, k.theta, k.cxsth, k.sxnth);
@@ -1731,7 +1777,7 @@ int launch_wave(xbg_ctx* c, const SP& s) {
     dim3 g = grid2(p);
     LAUNCH(c, "k_dispersion", k_dispersion, g, blk2, p, s, c->f);
     dim3 gr((p.nx + 255) / 256, 2, 1);
-    dim3 gw((p.nx + BX - 1) / BX, (p.ny + WTY - 1) / WTY, 1), bw(BX, WTY, 1);
+    dim3 gw(p.wtiles_x * p.wtiles_y, 1, 1), bw(BX, WTY, 1);
     switch (p.nt) {
         case 12:
             LAUNCH(c, "k_wave", k_wave<12>, gw, bw, p, s, c->f);
