@@ -20,7 +20,7 @@
 //         fdlibm e_log.c
 //   sin/cos  reduction by pi/2 (two-part constant, valid |x| < 2^19), kernel
 //         polynomials of fdlibm k_sin.c / k_cos.c on |r| <= pi/4
-//   expm1 Taylor series (Horner, 24 terms) on |x| <= 1, exp(x)-1 outside
+//   expm1 fdlibm s_expm1.c rational form on |x| <= 1, exp(x)-1 outside
 //   sinh  Taylor series on |x| < 1, (e^x - e^-x)/2 otherwise
 //   tanh  expm1(2|x|) / (expm1(2|x|) + 2), +-1 beyond |x| = 22
 //   pow   exp(y * log|x|) with the integer-exponent sign rule; y = 2 and y = 3
@@ -118,12 +118,42 @@ XM_FN double log10(double x) { return log(x) / 2.30258509299404568402; }
 XM_FN double expm1(double x) {
     if (isnan_(x)) return x;
     if (x > 1.0 || x < -1.0) return exp(x) - 1.0;
-    // Taylor series x * sum_{k=0..23} x^k/(k+1)!, Horner form, coefficients
-    // 1/(k+1)! rounded to double
-    const double c[24] = {1.00000000000000000e+00, 5.00000000000000000e-01, 1.66666666666666657e-01, 4.16666666666666644e-02, 8.33333333333333322e-03, 1.38888888888888894e-03, 1.98412698412698413e-04, 2.48015873015873016e-05, 2.75573192239858925e-06, 2.75573192239858883e-07, 2.50521083854417202e-08, 2.08767569878681002e-09, 1.60590438368216133e-10, 1.14707455977297245e-11, 7.64716373181981641e-13, 4.77947733238738525e-14, 2.81145725434552060e-15, 1.56192069685862253e-16, 8.22063524662432950e-18, 4.11031762331216484e-19, 1.95729410633912626e-20, 8.89679139245057408e-22, 3.86817017063068413e-23, 1.61173757109611839e-24};
-    double s = c[23];
-    for (int k = 22; k >= 0; k--) s = c[k] + s * x;
-    return x * s;
+    // fdlibm s_expm1.c on |x| <= 1: reduction by one ln2 above 0.5 ln2,
+    // rational approximation of the remainder, exact reconstruction
+    const double ln2hi = 6.93147180369123816490e-01, ln2lo = 1.90821492927058770002e-10;
+    const double Q1 = -3.33333333333331316428e-02, Q2 = 1.58730158725481460165e-03,
+                 Q3 = -7.93650757867487942473e-05, Q4 = 4.00821782732936239552e-06,
+                 Q5 = -2.01099218183624371326e-07;
+    double ax = x < 0.0 ? -x : x;
+    int k = 0;
+    double c = 0.0;
+    if (ax > 0.5 * 6.93147180559945286227e-01) {
+        double hi, lo;
+        if (x > 0.0) {
+            hi = x - ln2hi;
+            lo = ln2lo;
+            k = 1;
+        } else {
+            hi = x + ln2hi;
+            lo = -ln2lo;
+            k = -1;
+        }
+        x = hi - lo;
+        c = (hi - x) - lo;
+    } else if (ax < 5.55111512312578270212e-17) {  // 2^-54
+        return x;
+    }
+    double hfx = 0.5 * x;
+    double hxs = x * hfx;
+    double r1 = 1.0 + hxs * (Q1 + hxs * (Q2 + hxs * (Q3 + hxs * (Q4 + hxs * Q5))));
+    double t = 3.0 - r1 * hfx;
+    double e = hxs * ((r1 - t) / (6.0 - x * t));
+    if (k == 0) return x - (x * e - hxs);
+    e = (x * (e - c) - c);
+    e = e - hxs;
+    if (k == -1) return 0.5 * (x - e) - 0.5;
+    if (x < -0.25) return -2.0 * (e - (x + 0.5));
+    return 1.0 + 2.0 * (x - e);
 }
 
 XM_FN double sinh(double x) {
