@@ -122,6 +122,22 @@ def cpu_baseline(p, budget_s=20.0):
             "sample": f"{n} whole steps of the {p.nx}x{p.ny}x{p.ntheta} domain, oracle FP32, 1 thread"}
 
 
+def measured_traffic(p):
+    """HBM bytes per k_wave launch from the FETCH_SIZE / WRITE_SIZE counter
+    runs summarised in profiles/ (rocprofv3 --pmc on this workload), each
+    scaled by the reading of the same counter for a plain 1 GiB copy (the
+    copy's FETCH_SIZE reads half its bytes, WRITE_SIZE all of them).  None
+    when no summary for this grid exists."""
+    path = os.path.join(ROOT, "profiles", "k_wave_traffic_%dx%dx%d.json" % (p.nx, p.ny, p.ntheta))
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    fetch = t["fetch_kib"] * 1024 * t["copy_bytes"] / (t["copy_fetch_kib"] * 1024)
+    write = t["write_kib"] * 1024 * t["copy_bytes"] / (t["copy_write_kib"] * 1024)
+    return fetch + write
+
+
 def copy_calibration(nbytes=1 << 30):
     """A plain copy kernel of known size: each launch reads and writes
     nbytes once."""
@@ -182,7 +198,7 @@ def main():
                                f"wave+flow+sediment+bed, zb_accum={p.zb_accum}",
                    "nx": p.nx, "ny": p.ny, "ntheta": p.ntheta},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_wave",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(p), "kernel": "k_wave",
                      "kernel_avg_us": kw_us, "alg_bytes_per_cell": alg_bytes(p, "k_wave")},
         "step_roofline_frac": alg_bytes(p) * cps / (HBM_PEAK_GBS * 1e9),
         "kernels_us": {k: round(v[2], 2) for k, v in sorted(prof.items())},

@@ -2,15 +2,16 @@
 //
 // One step of the reference (Wave_gpu.cu:702-728) is ~92 launches, each a
 // 16x16-block stencil followed by a device synchronisation, plus ~25
-// cudaMalloc/cudaFree pairs and two full-field D2H copies for the time step
-// (SURVEY appendix B).  Here the same step is 17 stream-ordered launches over
-// buffers allocated once:
+// cudaMalloc/cudaFree pairs and two full-field D2H copies for the time step.
+// Here the same step is 17 stream-ordered launches over buffers allocated
+// once:
 //
 //   k_bnd_offshore  ubnd1D                                  (1 workgroup)
 //   k_dispersion    dispersion -> k, c, cg
 //   k_wave<NT>      offshorebndWav .. breakerdelay fused per cell: the
-//                   nTheta bins of ee and rr live in registers; one read and
-//                   one write of each directional field per step
+//                   direction planes of ee and rr stream through LDS tiles,
+//                   the bins of a cell stay in registers; one read and one
+//                   write of each directional field per step
 //   k_wave_force    wavforce + twodimbnd(noix) for Fx, Fy, ust
 //   k_flow_nuh      smago + wetu/wetv masks
 //   k_flow_u        u momentum (wlevslopes .. eulerustep, latbnd)
@@ -1427,7 +1428,6 @@ __global__ __launch_bounds__(256) void k_sed_update(KP p, Fld f) {
     }
 }
 
-// Copy the bed-update products back when morphology is off (C only).
 // ---------------------------------------------------------------------------
 // Init, reductions, output averages
 // ---------------------------------------------------------------------------
