@@ -297,6 +297,36 @@ int xbg_nc_write_var_step(xbg_nc* nc, const char* name, const float* var);
 int xbg_nc_close(xbg_nc* nc);
 
 /* ------------------------------------------------------------------------
+ * netCDF input and the wave-boundary reuse file (host code, no netCDF
+ * library; CDF-1/2/5 headers, any numeric stored type).  Failures return
+ * XBG_E_ARG with the reason in xbg_nc_error().
+ * ---------------------------------------------------------------------- */
+const char* xbg_nc_error(void);
+
+/* readgridncsize (writenetcdf.cpp:760-927): nx, ny from the (last two when
+ * it has three) dimensions of variable "zb", dx = |x[0] - x[nx-1]|/(nx-1)
+ * of the x coordinate variable (1-D or 2-D), rounded to float. */
+int xbg_nc_grid_size(const char* path, int* nx, int* ny, double* dx);
+/* readnczb (writenetcdf.cpp:743-757): the first nx*ny values of "zb". */
+int xbg_nc_read_zb(const char* path, int nx, int ny, float* zb);
+
+/* read_reuse_bndnc_head (writenetcdf.cpp:1089-1131): ntheta, thetamin,
+ * thetamax (first and last stored bin centres), dtheta =
+ * (thetamax - thetamin)/ntheta, rtlength = tt[last], dtbc = tt[last] -
+ * tt[last-1]; nrec (may be NULL) = records (blocks) in the file. */
+int xbg_bndnc_head(const char* path, xbg_params* p, int* nrec);
+/* read_reuse_bndnc (writenetcdf.cpp:1168-1245): record rec into the
+ * generator's Stfile / qfile layout (see xbg_wave_groups); the file must
+ * hold ny rows, ntheta bins and ntt times. */
+int xbg_bndnc_read(const char* path, int rec, int ny, int ntheta, int ntt, double* Stfile, double* qfile);
+/* createbndnc (append 0: new file XBG_bnd_reuse.nc layout, writenetcdf.cpp:
+ * 929-1023) / writebndnc (append 1: one more record, writenetcdf.cpp:
+ * 1025-1087) of one generated block. */
+int xbg_bndnc_write(const char* path, int append, int ntt, int ny, int ntheta, double dy, double dtheta,
+                    double thetamin, double dtbc, double time, double Hs, double Trep, double Tp, double Dp,
+                    const double* Stfile, const double* qfile);
+
+/* ------------------------------------------------------------------------
  * Host driver (input files, boundary bookkeeping, main loop, output):
  * the reference's main program and mainloopGPU (Wave_gpu.cu:610-937,
  * 1639-2990) over this C-ABI.
@@ -305,7 +335,7 @@ int xbg_nc_close(xbg_nc* nc);
 /* The non-numeric part of XBGPUParam (XBeachGPU.h:116-139). */
 typedef struct xbg_inputs {
     char bathy[512], outfile[512], sedthk[512], wavebndfile[512], slbndfile[512], windfile[512];
-    int wavebndtype;       /* 1 constant (ReadCstBnd), 4 JONSWAP (ReadJSWPBnd) */
+    int wavebndtype;       /* 1 constant (ReadCstBnd), 3 reuse file (read_reuse_bndnc), 4 JONSWAP */
     double dt;             /* fixed time step key (read, not used by the GPU loop) */
     char outvars[2048];    /* comma-separated output variable names */
 } xbg_inputs;

@@ -47,3 +47,76 @@ def test_run_case_writes_output(tmp_path):
         assert H[-1, :, 1:10].mean() > 0.2           # waves entered from the offshore boundary
         zs = f.variables["zs"][:]
         assert abs(zs[0, :, 0].mean() - 0.1) < 1e-6  # initial water level from the sea-level file
+
+
+def read_out(path, names):
+    with scipy.io.netcdf_file(str(path), "r", mmap=False) as f:
+        return {k: f.variables[k][:].copy() for k in names}
+
+
+def test_nc_bathymetry_matches_md(tmp_path):
+    """The same bed given as .md text and as a netCDF grid (zb(y, x), x
+    coordinate giving dx) gives identical output."""
+    L = lib()
+    nx, ny, dx = 48, 24, 4.0
+    a, b = tmp_path / "md", tmp_path / "nc"
+    a.mkdir()
+    b.mkdir()
+    x = np.arange(nx) * dx
+    zb = np.round(np.tile(-np.clip(-6.0 + 0.03 * x, -6.0, 2.0), (ny, 1)) * 256) / 256   # exact in text and float
+    zb[ny // 3:ny // 2, nx // 2:] += 0.5
+    for d in (a, b):
+        write_case(d, nx=nx, ny=ny, dx=dx, endtime=20.0, out=10.0)
+    with open(a / "beach.md", "w") as f:
+        f.write(f"{nx}\t{ny}\t{dx}\t0\t0\n")
+        for j in range(ny, 0, -1):
+            f.write(f"{j} " + " ".join(f"{v:.8f}" for v in zb[j - 1]) + "\n")
+    with scipy.io.netcdf_file(str(b / "beach.nc"), "w", version=2) as f:
+        f.createDimension("y", ny)
+        f.createDimension("x", nx)
+        f.createVariable("x", "d", ("x",))[:] = x
+        f.createVariable("y", "d", ("y",))[:] = np.arange(ny) * dx
+        f.createVariable("zb", "f", ("y", "x"))[:] = zb.astype(np.float32)
+    pb = b / "XBG_param.txt"
+    pb.write_text(pb.read_text().replace("bathy = beach.md;", "bathy = beach.nc;"))
+    for d in (a, b):
+        rc = L.xbg_run_case(str(d / "XBG_param.txt").encode(), 0)
+        assert rc == 0, L.xbg_run_error()
+    names = ("zs", "uu", "vv", "H", "zb", "ee")
+    ra, rb = read_out(a / "out.nc", names), read_out(b / "out.nc", names)
+    for k in names:
+        assert np.array_equal(ra[k], rb[k]), k
+
+
+def test_generated_boundary_is_saved_and_reused(tmp_path):
+    """A JONSWAP run (wavebndtype 4) keeps its generated block in
+    XBG_bnd_reuse.nc; a second run with wavebndtype 3 reads it back: bins
+    and timing from the file (thetamin/thetamax = first/last stored bin
+    centre, dtheta = their span / ntheta, as the reference reads them),
+    Trep = 15 s, and waves enter the domain."""
+    from xbeach_gpu_amd import ncio
+    L = lib()
+    gen = tmp_path / "gen"
+    gen.mkdir()
+    rc = L.xbg_run_case(str(write_case(gen, endtime=20.0)).encode(), 0)
+    assert rc == 0, L.xbg_run_error()
+    reuse = gen / "XBG_bnd_reuse.nc"
+    h = ncio.reuse_head(reuse)
+    assert (h.ntheta, h.nrec, h.rtlength, h.dtbc) == (12, 1, 120.0, 1.0)
+    c0 = np.deg2rad(-55.0)
+    assert abs(h.thetamin - c0) < 1e-7 and abs(h.thetamax + c0) < 1e-7
+    blk = ncio.reuse_read(reuse, 0, 32, 12, 121)
+    E = blk.Stfile.sum(axis=1)
+    assert np.isfinite(E).all() and E[20:].mean() > 0
+    use = tmp_path / "use"
+    use.mkdir()
+    pf = write_case(use, endtime=20.0)
+    (use / "XBG_bnd_reuse.nc").write_bytes(reuse.read_bytes())
+    pf.write_text(pf.read_text().replace("wavebndfile = jons.txt;\nwavebndtype = 4;",
+                                         "wavebndfile = XBG_bnd_reuse.nc;\nwavebndtype = 3;"))
+    rc = L.xbg_run_case(str(pf).encode(), 0)
+    assert rc == 0, L.xbg_run_error()
+    with scipy.io.netcdf_file(str(use / "out.nc"), "r", mmap=False) as f:
+        assert np.allclose(f.variables["theta"][:], h.thetamin + (np.arange(12) + 0.5) * h.dtheta, atol=1e-6)
+        H = f.variables["H"][:]
+        assert np.isfinite(H).all() and H[-1, :, 1:10].mean() > 0.1

@@ -113,6 +113,12 @@ PROTOTYPES = {
     "xbg_nc_write_time": (C.c_int, [C.c_void_p, C.c_double]),
     "xbg_nc_write_var_step": (C.c_int, [C.c_void_p, C.c_char_p, _F]),
     "xbg_nc_close": (C.c_int, [C.c_void_p]),
+    "xbg_nc_error": (C.c_char_p, []),
+    "xbg_nc_grid_size": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "xbg_nc_read_zb": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _F]),
+    "xbg_bndnc_head": (C.c_int, [C.c_char_p, _P, C.POINTER(C.c_int)]),
+    "xbg_bndnc_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]),
+    "xbg_bndnc_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 9 + [_D, _D]),
     "xbg_default_inputs": (None, [C.c_void_p]),
     "xbg_param_line": (C.c_int, [C.c_char_p, _P, C.c_void_p]),
     "xbg_run_case": (C.c_int, [C.c_char_p, C.c_int]),

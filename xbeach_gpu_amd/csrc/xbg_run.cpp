@@ -4,10 +4,13 @@
 //
 //   XBG_param.txt          readparamstr / findparameter (read_input.cpp:595-1474)
 //   bathymetry .md/.dep/.bot  readbathyHead / readbathy / readXBbathy
-//                          (read_input.cpp:1508-1634)
+//                          (read_input.cpp:1508-1634); .nc readgridncsize /
+//                          readnczb (writenetcdf.cpp:743-927)
 //   sea level, wind        readWLfile / readWNDfile (read_input.cpp:259-405)
-//   wave boundary          type 1 ReadCstBnd + GenCstWave, type 4 ReadJSWPBnd +
-//                          makjonswap + GenWGnLBW (read_input.cpp:407-533,
+//   wave boundary          type 1 ReadCstBnd + GenCstWave, type 3 reuse file
+//                          (read_reuse_bndnc*), type 4 ReadJSWPBnd +
+//                          makjonswap + GenWGnLBW, generated blocks saved to
+//                          XBG_bnd_reuse.nc (read_input.cpp:407-533,
 //                          Wavestep.cu:24-382, 512-753)
 //   sanity / defaults      checkparamsanity (read_input.cpp:1227-1444)
 //   initial state          Wave_gpu.cu:1988-2056, 2629-2691
@@ -264,8 +267,19 @@ int xbg_run_case(const char* paramfile, int verbose) {
         zb.assign((size_t)P.nx * P.ny, 0.0f);
         for (size_t k = 0; k < zb.size(); k++)
             if (!(fs >> zb[k])) return fail(bathy + ": too few values");
+    } else if (ext == "nc") {
+        // readgridncsize / readnczb (writenetcdf.cpp:743-927); grdalpha
+        // comes from the parameter file (default 0)
+        int gnx = 0, gny = 0;
+        double gdx = 0.0;
+        if (xbg_nc_grid_size(bathy.c_str(), &gnx, &gny, &gdx)) return fail(xbg_nc_error());
+        P.nx = gnx;
+        P.ny = gny;
+        P.dx = gdx;
+        zb.assign((size_t)P.nx * P.ny, 0.0f);
+        if (xbg_nc_read_zb(bathy.c_str(), P.nx, P.ny, zb.data())) return fail(xbg_nc_error());
     } else {
-        return fail("bathymetry format ." + ext + " is not supported (use .md, .dep or .bot)");
+        return fail("bathymetry format ." + ext + " is not supported (use .md, .nc, .dep or .bot)");
     }
     P.grdalpha = P.grdalpha * PI_ / 180;  // Wave_gpu.cu:1770
     // ---- sea level and wind (Wave_gpu.cu:1783-1830)
@@ -303,11 +317,22 @@ int xbg_run_case(const char* paramfile, int verbose) {
     }
     // ---- wave boundary list (Wave_gpu.cu:1833-1880)
     std::vector<xbg_wavebnd> wb;
-    if (IN.wavebndfile[0]) {
-        std::ifstream fs(join_dir(dir, IN.wavebndfile));
+    const std::string wbfile = join_dir(dir, IN.wavebndfile);
+    if (IN.wavebndfile[0] && IN.wavebndtype == 3) {
+        // read_reuse_bndnc_head / read_reuse_bndnc_vec (writenetcdf.cpp:
+        // 1089-1166): bins and block timing from the file, one block per record
+        int nrec = 0;
+        if (xbg_bndnc_head(wbfile.c_str(), &P, &nrec)) return fail(xbg_nc_error());
+        for (int i = 0; i < nrec + 1; i++) {
+            xbg_wavebnd w{};
+            w.time = P.rtlength * i;
+            wb.push_back(w);
+        }
+    } else if (IN.wavebndfile[0]) {
+        std::ifstream fs(wbfile);
         if (fs.fail()) return fail(std::string(IN.wavebndfile) + " could not be opened");
         if (IN.wavebndtype != 1 && IN.wavebndtype != 4)
-            return fail("wavebndtype " + std::to_string(IN.wavebndtype) + " is not supported (use 1 or 4)");
+            return fail("wavebndtype " + std::to_string(IN.wavebndtype) + " is not supported (use 1, 3 or 4)");
         size_t need = IN.wavebndtype == 4 ? 6 : 5;
         std::string line;
         while (std::getline(fs, line)) {
@@ -386,6 +411,22 @@ int xbg_run_case(const char* paramfile, int verbose) {
     double Trep = 10.0;
     std::vector<float> theta(nt), cxs(nt), sxn(nt);
     xbg_theta_tables(&P, theta.data(), cxs.data(), sxn.data());
+    // the generated blocks are kept in XBG_bnd_reuse.nc for a later run with
+    // wavebndtype 3 (createbndnc / writebndnc, Wavestep.cu:150-199, 560-610)
+    const std::string reuse = join_dir(dir, "XBG_bnd_reuse.nc");
+    auto save_block = [&](int append, const xbg_wavebnd& w) -> int {
+        if (xbg_bndnc_write(reuse.c_str(), append, nrec_blk, ny, nt, P.dx, P.dtheta, P.thetamin, P.dtbc, w.time, w.Hs,
+                            Trep, w.Tp, w.Dp, Stfile.data(), qfile.data()))
+            return fail(xbg_nc_error());
+        return XBG_OK;
+    };
+    // read_reuse_bndnc (writenetcdf.cpp:1168-1245); Trep = 15 as it sets it
+    auto read_block = [&](int rec) -> int {
+        if (xbg_bndnc_read(wbfile.c_str(), rec, ny, nt, nrec_blk, Stfile.data(), qfile.data()))
+            return fail(xbg_nc_error());
+        Trep = 15.0;
+        return XBG_OK;
+    };
     auto gen_block = [&](const xbg_wavebnd& w) -> int {
         int nf, nd;
         if (xbg_jonswap_size(&w, &nf, &nd)) return fail("jonswap size");
@@ -402,7 +443,9 @@ int xbg_run_case(const char* paramfile, int verbose) {
             if (xbg_constant_waves(&P, (int)wb.size(), wb.data(), theta.data(), Stfile.data(), qfile.data(), Tp.data()))
                 return fail("constant wave boundary");
             Trep = Tp[0];
-        } else if (gen_block(wb[0])) {
+        } else if (IN.wavebndtype == 3) {
+            if (read_block(0)) return XBG_E_ARG;
+        } else if (gen_block(wb[0]) || save_block(0, wb[0])) {
             return XBG_E_ARG;
         }
     }
@@ -475,8 +518,10 @@ int xbg_run_case(const char* paramfile, int verbose) {
             if (wb[WAV].time - totaltime < 0.0) {
                 WAV++;
                 if (WAV >= (int)wb.size()) WAV = (int)wb.size() - 1;
-                if (IN.wavebndtype == 4) {
-                    if (gen_block(wb[WAV - 1])) {
+                if (IN.wavebndtype == 3 || IN.wavebndtype == 4) {
+                    int r = IN.wavebndtype == 3 ? read_block(WAV - 1)
+                                                : (gen_block(wb[WAV - 1]) || save_block(1, wb[WAV - 1]));
+                    if (r) {
                         rc = XBG_E_ARG;
                         break;
                     }
