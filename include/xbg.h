@@ -310,6 +310,15 @@ int xbg_nc_grid_size(const char* path, int* nx, int* ny, double* dx);
 /* readnczb (writenetcdf.cpp:743-757): the first nx*ny values of "zb". */
 int xbg_nc_read_zb(const char* path, int nx, int ny, float* zb);
 
+/* readSWANSPEC (writenetcdf.cpp:1288-1460): 2-D spectrum of "file.nc?var"
+ * (var "z" by default), dims (y, x) with coordinate variables; x holds the
+ * frequencies when all x lie in [0, 10], else the nautical directions
+ * (degrees).  Returns freq[nf], dir[nd] in the grid convention
+ * (1.5*pi - grdalpha) - dir*pi/180 and S[f + d*nf] per radian, ready for
+ * xbg_wave_groups.  _size gives nf and nd first. */
+int xbg_nc_spectrum_size(const char* spec, int* nf, int* nd);
+int xbg_nc_spectrum(const char* spec, double grdalpha, double* freq, double* dir, double* S);
+
 /* read_reuse_bndnc_head (writenetcdf.cpp:1089-1131): ntheta, thetamin,
  * thetamax (first and last stored bin centres), dtheta =
  * (thetamax - thetamin)/ntheta, rtlength = tt[last], dtbc = tt[last] -
@@ -335,7 +344,7 @@ int xbg_bndnc_write(const char* path, int append, int ntt, int ny, int ntheta, d
 /* The non-numeric part of XBGPUParam (XBeachGPU.h:116-139). */
 typedef struct xbg_inputs {
     char bathy[512], outfile[512], sedthk[512], wavebndfile[512], slbndfile[512], windfile[512];
-    int wavebndtype;       /* 1 constant (ReadCstBnd), 3 reuse file (read_reuse_bndnc), 4 JONSWAP */
+    int wavebndtype;       /* 1 constant, 2 XBeach files, 3 reuse file, 4 JONSWAP, 5 spectra */
     double dt;             /* fixed time step key (read, not used by the GPU loop) */
     char outvars[2048];    /* comma-separated output variable names */
 } xbg_inputs;
@@ -346,6 +355,13 @@ void xbg_default_inputs(xbg_inputs* in);
 /* readparamstr (read_input.cpp:595-1225) for one "key = value;" line of
  * XBG_param.txt.  thetamin/thetamax/dtheta are converted to radians. */
 int xbg_param_line(const char* line, xbg_params* p, xbg_inputs* in);
+
+/* readXbbndstep (read_input.cpp:161-220): one block of XBeach boundary
+ * files (binary doubles), nrec_read records of q (qfile layout) and E
+ * (Stfile layout), each value rounded to float.  Errors via
+ * xbg_run_error(). */
+int xbg_xbbnd_read(const char* qpath, const char* epath, int ny, int ntheta, int nrec_read, double* Stfile,
+                   double* qfile);
 
 /* Run the case described by a parameter file (other file names relative to
  * its directory) to endtime, writing the netCDF output: the reference's

@@ -120,3 +120,54 @@ def test_generated_boundary_is_saved_and_reused(tmp_path):
         assert np.allclose(f.variables["theta"][:], h.thetamin + (np.arange(12) + 0.5) * h.dtheta, atol=1e-6)
         H = f.variables["H"][:]
         assert np.isfinite(H).all() and H[-1, :, 1:10].mean() > 0.1
+
+
+def test_spectrum_file_boundary_matches_jonswap(tmp_path):
+    """wavebndtype 5 (a list of 2-D spectrum files, readSWANSPEC) on a file
+    holding the JONSWAP spectrum of the type-4 case reproduces its run."""
+    from xbeach_gpu_amd import bnd_jonswap as bj
+    L = lib()
+    js = bj.jonswap(1.0, 8.0, 270.0, 10.0, 3.3)
+    pi_ref = 3.14159265
+    a, b = tmp_path / "t4", tmp_path / "t5"
+    a.mkdir()
+    b.mkdir()
+    write_case(a, endtime=20.0)
+    pf = write_case(b, endtime=20.0)
+    with scipy.io.netcdf_file(str(b / "spec.nc"), "w", version=2) as f:
+        f.createDimension("dir", len(js.dir))
+        f.createDimension("freq", len(js.freq))
+        f.createVariable("freq", "d", ("freq",))[:] = js.freq
+        f.createVariable("dir", "d", ("dir",))[:] = (1.5 * pi_ref - js.dir) * 180.0 / pi_ref
+        f.createVariable("z", "d", ("dir", "freq"))[:] = js.S * pi_ref / 180.0
+    (b / "specs.txt").write_text("0 spec.nc\n")
+    pf.write_text(pf.read_text().replace("wavebndfile = jons.txt;\nwavebndtype = 4;",
+                                         "wavebndfile = specs.txt;\nwavebndtype = 5;"))
+    for d in (a, b):
+        rc = L.xbg_run_case(str(d / "XBG_param.txt").encode(), 0)
+        assert rc == 0, L.xbg_run_error()
+    ra, rb = read_out(a / "out.nc", ("H", "zs", "uu")), read_out(b / "out.nc", ("H", "zs", "uu"))
+    assert ra["H"][-1, :, 1:10].mean() > 0.2
+    for k in ra:
+        assert np.allclose(ra[k], rb[k], rtol=1e-4, atol=1e-4 * np.abs(ra[k]).max()), k
+
+
+def test_xbeach_boundary_files_run(tmp_path):
+    """wavebndtype 2 (XBeach files): header thetamin thetamax dtheta dtbc
+    rtlength, then time Trep qfile Efile per block of binary doubles."""
+    from xbeach_gpu_amd import bnd_jonswap as bj
+    from xbeach_gpu_amd import default_params
+    L = lib()
+    pf = write_case(tmp_path, endtime=20.0)
+    p = default_params(nx=64, ny=32, dx=5.0, thetamin=-60, thetamax=60, dtheta=10)
+    p.rtlength, p.dtbc = 120.0, 1.0
+    blk = bj.wave_groups(p, 6.0, bj.jonswap(1.0, 8.0, 270.0, 10.0, 3.3), seed=0)
+    blk.qfile[:-1].astype("<f8").tofile(tmp_path / "q0.bin")
+    blk.Stfile[:-1].astype("<f8").tofile(tmp_path / "E0.bin")
+    (tmp_path / "xb.txt").write_text(f"-60\t60\t10\t1\t120\n0\t{blk.Trep}\tq0.bin\tE0.bin\n")
+    pf.write_text(pf.read_text().replace("wavebndfile = jons.txt;\nwavebndtype = 4;",
+                                         "wavebndfile = xb.txt;\nwavebndtype = 2;"))
+    rc = L.xbg_run_case(str(pf).encode(), 0)
+    assert rc == 0, L.xbg_run_error()
+    H = read_out(tmp_path / "out.nc", ("H",))["H"]
+    assert np.isfinite(H).all() and H[-1, :, 1:10].mean() > 0.2

@@ -237,3 +237,87 @@ def test_reuse_file_from_another_writer(tmp_path):
     assert (h.ntheta, h.nrec, h.rtlength, h.dtbc) == (nth, 1, ntt - 1.0, 1.0)
     got = ncio.reuse_read(path, 0, ny, nth, ntt)
     assert np.array_equal(got.Stfile, b.Stfile) and np.array_equal(got.qfile[:, :2], b.qfile[:, :2])
+
+
+PI_REF = 3.14159265
+
+
+def write_spec(path, x, y, z, var="z", xname="x", yname="y"):
+    with scipy.io.netcdf_file(str(path), "w", version=2) as f:
+        f.createDimension(yname, len(y))
+        f.createDimension(xname, len(x))
+        f.createVariable(xname, "d", (xname,))[:] = x
+        f.createVariable(yname, "d", (yname,))[:] = y
+        f.createVariable(var, "d", (yname, xname))[:] = z
+
+
+def test_spectrum_x_is_frequency(tmp_path):
+    """x within [0, 10]: x = frequencies, y = nautical directions (deg);
+    S[d, f] = z[d, f] * 180 / pi, directions to the grid convention."""
+    rng = np.random.default_rng(2)
+    fr = np.linspace(0.04, 0.4, 19)
+    dn = np.arange(200.0, 345.0, 5.0)
+    z = rng.uniform(0, 1, size=(len(dn), len(fr)))
+    write_spec(tmp_path / "s.nc", fr, dn, z)
+    sp = ncio.read_spectrum(tmp_path / "s.nc", grdalpha=0.1)
+    assert np.array_equal(sp.freq, fr)
+    assert np.array_equal(sp.dir, (1.5 * PI_REF - 0.1) - dn * PI_REF / 180.0)
+    assert np.array_equal(sp.S, z * 180.0 / PI_REF)
+
+
+def test_spectrum_x_is_direction_non_square(tmp_path):
+    """x outside [0, 10]: x = directions, y = frequencies; the spectrum is
+    the transpose of z for any shape (nx = 30 directions, ny = 13
+    frequencies), under a variable name given after '?'."""
+    rng = np.random.default_rng(4)
+    dn = np.arange(0.0, 360.0, 12.0)
+    fr = np.linspace(0.05, 0.35, 13)
+    z = rng.uniform(0, 1, size=(len(fr), len(dn)))
+    write_spec(tmp_path / "t.nc", dn, fr, z, var="VaDens", xname="dir", yname="freq")
+    sp = ncio.read_spectrum(f"{tmp_path / 't.nc'}?VaDens")
+    assert sp.S.shape == (len(dn), len(fr))
+    assert np.array_equal(sp.freq, fr)
+    assert np.array_equal(sp.dir, 1.5 * PI_REF - dn * PI_REF / 180.0)
+    assert np.array_equal(sp.S, z.T * 180.0 / PI_REF)
+    with pytest.raises(ncio.NcError, match="no variable \"z\""):
+        ncio.read_spectrum(tmp_path / "t.nc")
+
+
+def test_spectrum_file_drives_the_generator_like_jonswap(tmp_path):
+    """A file holding the JONSWAP spectrum of makjonswap (per degree,
+    nautical directions) gives the generator the same spectrum back, so a
+    type-5 boundary equals the type-4 one of the same sea state."""
+    from xbeach_gpu_amd import bnd_jonswap as bj
+    from xbeach_gpu_amd import default_params
+    js = bj.jonswap(1.2, 9.0, 265.0, 15.0, 3.3)
+    dn = (1.5 * PI_REF - js.dir) * 180.0 / PI_REF
+    write_spec(tmp_path / "j.nc", js.freq, dn, js.S * PI_REF / 180.0)
+    sp = ncio.read_spectrum(tmp_path / "j.nc")
+    assert np.allclose(sp.dir, js.dir, rtol=0, atol=1e-12)
+    assert np.allclose(sp.S, js.S, rtol=1e-14, atol=0)
+    p = default_params(nx=50, ny=8, dx=5.0, thetamin=-60, thetamax=60, dtheta=10)
+    p.rtlength = 200.0
+    a, b = bj.wave_groups(p, 8.0, js, seed=0), bj.wave_groups(p, 8.0, sp, seed=0)
+    assert a.Trep == pytest.approx(b.Trep, rel=1e-12)
+    assert np.allclose(a.Stfile, b.Stfile, rtol=1e-9, atol=1e-9 * a.Stfile.max())
+    assert np.allclose(a.qfile, b.qfile, rtol=1e-9, atol=1e-9 * np.abs(a.qfile).max())
+
+
+def test_xbeach_boundary_files(tmp_path):
+    """readXbbndstep: binary doubles, q in the qfile layout and E in the
+    Stfile layout, nrec_read records, each value rounded to float."""
+    from xbeach_gpu_amd import bnd_jonswap as bj
+    rng = np.random.default_rng(6)
+    ny, nth, nrec = 5, 3, 7
+    q = rng.normal(size=(nrec - 1, 4, ny))
+    E = rng.uniform(0, 100, size=(nrec - 1, nth, ny))
+    q.astype("<f8").tofile(tmp_path / "q.bin")
+    E.astype("<f8").tofile(tmp_path / "E.bin")
+    blk = bj.read_xbeach_block(tmp_path / "q.bin", tmp_path / "E.bin", ny, nth, nrec, nrec - 1, 12.0)
+    assert np.array_equal(blk.qfile[:-1], q.astype(np.float32).astype(np.float64))
+    assert np.array_equal(blk.Stfile[:-1], E.astype(np.float32).astype(np.float64))
+    assert (blk.qfile[-1] == 0).all() and (blk.Stfile[-1] == 0).all() and blk.Trep == 12.0
+    with pytest.raises(RuntimeError, match="expected"):
+        bj.read_xbeach_block(tmp_path / "q.bin", tmp_path / "E.bin", ny, nth, nrec, nrec, 12.0)
+    with pytest.raises(RuntimeError, match="Unable to open"):
+        bj.read_xbeach_block(tmp_path / "none.bin", tmp_path / "E.bin", ny, nth, nrec, 2, 12.0)

@@ -116,6 +116,9 @@ PROTOTYPES = {
     "xbg_nc_error": (C.c_char_p, []),
     "xbg_nc_grid_size": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "xbg_nc_read_zb": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _F]),
+    "xbg_nc_spectrum_size": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "xbg_nc_spectrum": (C.c_int, [C.c_char_p, C.c_double, _D, _D, _D]),
+    "xbg_xbbnd_read": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, _D, _D]),
     "xbg_bndnc_head": (C.c_int, [C.c_char_p, _P, C.POINTER(C.c_int)]),
     "xbg_bndnc_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]),
     "xbg_bndnc_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 9 + [_D, _D]),

@@ -137,3 +137,15 @@ def synth(N: int, idx, G) -> np.ndarray:
     _chk(lib().xbg_bnd_synth(int(N), len(idx), idx.ctypes.data_as(C.POINTER(C.c_int)), _dp(gr), _dp(gi),
                              _dp(out)), "xbg_bnd_synth")
     return out
+
+
+def read_xbeach_block(qpath: str, epath: str, ny: int, ntheta: int, nrec: int, nrec_read: int,
+                      Trep: float) -> BoundaryBlock:
+    """readXbbndstep (read_input.cpp:161-220): an XBeach boundary block from
+    its binary q and E files; records from nrec_read on stay zero."""
+    St = np.zeros((nrec, ntheta, ny))
+    q = np.zeros((nrec, 4, ny))
+    rc = lib().xbg_xbbnd_read(str(qpath).encode(), str(epath).encode(), ny, ntheta, nrec_read, _dp(St), _dp(q))
+    if rc != 0:
+        raise RuntimeError(lib().xbg_run_error().decode())
+    return BoundaryBlock(Trep, St, q)

@@ -10,6 +10,8 @@
 //                                       (writenetcdf.cpp:1089-1245)
 //   xbg_bndnc_write                     createbndnc / writebndnc
 //                                       (writenetcdf.cpp:929-1087)
+//   xbg_nc_spectrum(_size)              readSWANSPEC (writenetcdf.cpp:
+//                                       1288-1460): 2-D wave spectra
 //
 // The reader takes the classic (CDF-1), 64-bit-offset (CDF-2) and 64-bit-
 // data (CDF-5) formats: header parsed from the file, values converted from
@@ -464,6 +466,92 @@ int xbg_nc_read_zb(const char* path, int nx, int ny, float* zb) {
         if (int rc = F.get(*z, 0, 1, v.data())) return rc;
     }
     for (uint64_t k = 0; k < n; k++) zb[k] = (float)v[k];
+    return XBG_OK;
+}
+
+// readSWANSPEC (writenetcdf.cpp:1288-1460): a 2-D spectrum in variable var
+// of "file.nc?var" ("z" when no '?'), dimensions (y, x) with coordinate
+// variables of the same names.  When every x lies in [0, 10] x holds the
+// frequencies and y the nautical directions (degrees), otherwise the other
+// way round.  Directions go to the grid convention (1.5 pi - grdalpha) -
+// dir * pi / 180 and the density from per degree to per radian (* 180 / pi),
+// with pi = 3.14159265 as the reference defines it.  S[f + d*nf].
+//
+// For x = directions the reference fills S[i + j*nx] from zz[j + i*ny],
+// the transpose only when nx == ny; this reads the transpose for any shape.
+namespace {
+const double PI_REF = 3.14159265;
+
+struct Spec2D {
+    std::vector<double> x, y, z;  // z (ny, nx)
+};
+
+int read_spec(const char* spec, Spec2D& out) {
+    if (!spec) return ncfail("NULL spectrum file");
+    std::string s(spec), file = s, var = "z";
+    size_t q = s.find('?');
+    if (q != std::string::npos) {
+        file = s.substr(0, q);
+        var = s.substr(q + 1);
+        size_t q2 = var.find('?');
+        if (q2 != std::string::npos) var = var.substr(0, q2);
+    }
+    NcFile F;
+    if (int rc = F.open(file, "rb")) return rc;
+    const Var* v = need_var(F.h, var, file);
+    if (!v) return XBG_E_ARG;
+    if (v->dims.size() != 2 || v->rec) return ncfail(file + ": " + var + " must be a 2-D (y, x) variable");
+    const uint64_t ny = F.h.dims[v->dims[0]].len, nx = F.h.dims[v->dims[1]].len;
+    if (nx < 2 || ny < 2) return ncfail(file + ": spectrum needs at least 2 x 2 values");
+    const Var* yc = need_var(F.h, F.h.dims[v->dims[0]].name, file);
+    const Var* xc = need_var(F.h, F.h.dims[v->dims[1]].name, file);
+    if (!yc || !xc) return XBG_E_ARG;
+    if (yc->rec || xc->rec || F.h.slab(*yc) < ny || F.h.slab(*xc) < nx)
+        return ncfail(file + ": coordinate variables of " + var);
+    std::vector<double> yv(F.h.slab(*yc)), xv(F.h.slab(*xc));
+    out.z.resize(nx * ny);
+    if (int rc = F.get(*yc, 0, 1, yv.data())) return rc;
+    if (int rc = F.get(*xc, 0, 1, xv.data())) return rc;
+    if (int rc = F.get(*v, 0, 1, out.z.data())) return rc;
+    out.x.assign(xv.begin(), xv.begin() + nx);
+    out.y.assign(yv.begin(), yv.begin() + ny);
+    return XBG_OK;
+}
+
+bool x_is_dir(const Spec2D& sp) {
+    for (double v : sp.x)
+        if (v < 0.0 || v > 10.0) return true;
+    return false;
+}
+}  // namespace
+
+int xbg_nc_spectrum_size(const char* spec, int* nf, int* nd) {
+    if (!nf || !nd) return ncfail("NULL argument");
+    Spec2D sp;
+    if (int rc = read_spec(spec, sp)) return rc;
+    const bool xd = x_is_dir(sp);
+    *nf = (int)(xd ? sp.y.size() : sp.x.size());
+    *nd = (int)(xd ? sp.x.size() : sp.y.size());
+    return XBG_OK;
+}
+
+int xbg_nc_spectrum(const char* spec, double grdalpha, double* freq, double* dir, double* S) {
+    if (!freq || !dir || !S) return ncfail("NULL argument");
+    Spec2D sp;
+    if (int rc = read_spec(spec, sp)) return rc;
+    const size_t nx = sp.x.size(), ny = sp.y.size();
+    const bool xd = x_is_dir(sp);
+    const size_t nf = xd ? ny : nx, nd = xd ? nx : ny;
+    const std::vector<double>& fc = xd ? sp.y : sp.x;
+    const std::vector<double>& dc = xd ? sp.x : sp.y;
+    for (size_t i = 0; i < nf; i++) freq[i] = fc[i];
+    for (size_t d = 0; d < nd; d++) dir[d] = (1.5 * PI_REF - grdalpha) - dc[d] * PI_REF / 180.0;
+    for (size_t d = 0; d < nd; d++)
+        for (size_t f = 0; f < nf; f++) {
+            // z is (y, x): x = freq -> z[f + d*nx]; x = dir -> z[d + f*nx]
+            double z = xd ? sp.z[d + f * nx] : sp.z[f + d * nx];
+            S[f + d * nf] = z * 180.0 / PI_REF;
+        }
     return XBG_OK;
 }
 

@@ -7,10 +7,12 @@
 //                          (read_input.cpp:1508-1634); .nc readgridncsize /
 //                          readnczb (writenetcdf.cpp:743-927)
 //   sea level, wind        readWLfile / readWNDfile (read_input.cpp:259-405)
-//   wave boundary          type 1 ReadCstBnd + GenCstWave, type 3 reuse file
+//   wave boundary          type 1 ReadCstBnd + GenCstWave, type 2 XBeach
+//                          files (readXbbnd*), type 3 reuse file
 //                          (read_reuse_bndnc*), type 4 ReadJSWPBnd +
-//                          makjonswap + GenWGnLBW, generated blocks saved to
-//                          XBG_bnd_reuse.nc (read_input.cpp:407-533,
+//                          makjonswap + GenWGnLBW, type 5 ReadSPECBnd +
+//                          readSWANSPEC + GenWGnLBW; generated blocks saved
+//                          to XBG_bnd_reuse.nc (read_input.cpp:25-593,
 //                          Wavestep.cu:24-382, 512-753)
 //   sanity / defaults      checkparamsanity (read_input.cpp:1227-1444)
 //   initial state          Wave_gpu.cu:1988-2056, 2629-2691
@@ -212,6 +214,32 @@ int xbg_param_line(const char* cline, xbg_params* p, xbg_inputs* in) {
     return XBG_OK;
 }
 
+// readXbbndstep (read_input.cpp:161-220): one XBeach boundary block from
+// its binary files of doubles, nrec_read records each (the reference reads
+// ceil(rtlength/dtbc)), q in the qfile layout j + k*ny + m*ny*4 and E in
+// the Stfile layout j + itheta*ny + m*ny*ntheta; values pass through float
+// (DECNUM) as the reference stores them.  Records beyond nrec_read are left
+// as they are.
+int xbg_xbbnd_read(const char* qpath, const char* epath, int ny, int ntheta, int nrec_read, double* Stfile,
+                   double* qfile) {
+    if (!qpath || !epath || !Stfile || !qfile || ny < 1 || ntheta < 1 || nrec_read < 1)
+        return fail("xbg_xbbnd_read: bad argument");
+    auto rd = [](const char* path, size_t n, double* out) -> int {
+        FILE* f = fopen(path, "rb");
+        if (!f) return fail(std::string("Unable to open file ") + path);
+        std::vector<double> v(n);
+        size_t got = fread(v.data(), sizeof(double), n, f);
+        fclose(f);
+        if (got != n)
+            return fail(std::string(path) + ": holds " + std::to_string(got) + " values, " + std::to_string(n) +
+                        " expected");
+        for (size_t k = 0; k < n; k++) out[k] = (double)(float)v[k];
+        return XBG_OK;
+    };
+    if (int rc = rd(qpath, (size_t)4 * ny * nrec_read, qfile)) return rc;
+    return rd(epath, (size_t)ntheta * ny * nrec_read, Stfile);
+}
+
 void xbg_default_inputs(xbg_inputs* in) {
     std::memset(in, 0, sizeof(*in));
     copystr(in->outfile, sizeof(in->outfile), "XBGPU_output.nc");  // XBeachGPU.h:122
@@ -317,8 +345,52 @@ int xbg_run_case(const char* paramfile, int verbose) {
     }
     // ---- wave boundary list (Wave_gpu.cu:1833-1880)
     std::vector<xbg_wavebnd> wb;
+    std::vector<std::string> wbq, wbe;  // per record: XBeach q/E files (type 2), spectrum file (type 5)
     const std::string wbfile = join_dir(dir, IN.wavebndfile);
-    if (IN.wavebndfile[0] && IN.wavebndtype == 3) {
+    if (IN.wavebndfile[0] && (IN.wavebndtype == 2 || IN.wavebndtype == 5)) {
+        // readXbbndhead / readXbbndfile (read_input.cpp:25-148), ReadSPECBnd
+        // (read_input.cpp:534-593): fields split on tabs, else spaces, else
+        // commas; a last entry rtlength after the final one
+        std::ifstream fs(wbfile);
+        if (fs.fail()) return fail(std::string(IN.wavebndfile) + " could not be opened");
+        auto split3 = [](const std::string& line, size_t need) {
+            std::vector<std::string> e = split(line, '\t');
+            if (e.size() < need) e = split(line, ' ');
+            if (e.size() < need) e = split(line, ',');
+            return e;
+        };
+        std::string line;
+        if (IN.wavebndtype == 2) {
+            std::getline(fs, line);
+            auto e = split3(line, 5);
+            if (e.size() < 5) return fail("Wave bnd file, header format error: 5 fields expected");
+            P.thetamin = std::stod(e[0]) * PI_ / 180.0;
+            P.thetamax = std::stod(e[1]) * PI_ / 180.0;
+            P.dtheta = std::stod(e[2]) * PI_ / 180.0;
+            P.dtbc = std::stod(e[3]);
+            P.rtlength = std::stod(e[4]);
+        }
+        const size_t need = IN.wavebndtype == 2 ? 4 : 2;
+        while (std::getline(fs, line)) {
+            if (line.empty()) continue;
+            auto e = split3(line, need);
+            if (e.size() < need) return fail("Wave bnd file format error");
+            xbg_wavebnd w{};
+            w.time = std::stod(e[0]);
+            if (IN.wavebndtype == 2) {
+                w.Tp = std::stod(e[1]);  // the block's Trep
+                wbq.push_back(join_dir(dir, trim(e[2], " \r")));
+                wbe.push_back(join_dir(dir, trim(e[3], " \r")));
+            } else {
+                wbq.push_back(join_dir(dir, trim(e[1], " \r")));
+            }
+            wb.push_back(w);
+        }
+        if (wb.empty()) return fail("empty wave boundary file");
+        xbg_wavebnd last = wb.back();
+        last.time = wb.back().time + P.rtlength;
+        wb.push_back(last);
+    } else if (IN.wavebndfile[0] && IN.wavebndtype == 3) {
         // read_reuse_bndnc_head / read_reuse_bndnc_vec (writenetcdf.cpp:
         // 1089-1166): bins and block timing from the file, one block per record
         int nrec = 0;
@@ -332,7 +404,7 @@ int xbg_run_case(const char* paramfile, int verbose) {
         std::ifstream fs(wbfile);
         if (fs.fail()) return fail(std::string(IN.wavebndfile) + " could not be opened");
         if (IN.wavebndtype != 1 && IN.wavebndtype != 4)
-            return fail("wavebndtype " + std::to_string(IN.wavebndtype) + " is not supported (use 1, 3 or 4)");
+            return fail("wavebndtype " + std::to_string(IN.wavebndtype) + " is not supported (use 1 to 5)");
         size_t need = IN.wavebndtype == 4 ? 6 : 5;
         std::string line;
         while (std::getline(fs, line)) {
@@ -427,11 +499,34 @@ int xbg_run_case(const char* paramfile, int verbose) {
         Trep = 15.0;
         return XBG_OK;
     };
-    auto gen_block = [&](const xbg_wavebnd& w) -> int {
+    // readXbbndstep (read_input.cpp:161-220): ceil(rtlength/dtbc) records
+    auto xb_block = [&](int k) -> int {
+        int nread = (int)std::ceil(P.rtlength / P.dtbc);
+        if (nread > nrec_blk) nread = nrec_blk;
+        if (xbg_xbbnd_read(wbq[k].c_str(), wbe[k].c_str(), ny, nt, nread, Stfile.data(), qfile.data()))
+            return XBG_E_ARG;
+        Trep = wb[k].Tp;
+        return XBG_OK;
+    };
+    // makjonswap (type 4) or readSWANSPEC (type 5), then GenWGnLBW
+    auto gen_block = [&](int k) -> int {
+        const xbg_wavebnd& w = wb[k];
         int nf, nd;
-        if (xbg_jonswap_size(&w, &nf, &nd)) return fail("jonswap size");
-        std::vector<double> fr(nf), di(nd), S((size_t)nf * nd);
-        if (xbg_jonswap(&w, fr.data(), di.data(), S.data())) return fail("JONSWAP spectrum: NaN");
+        std::vector<double> fr, di, S;
+        if (IN.wavebndtype == 5) {
+            if (xbg_nc_spectrum_size(wbq[k].c_str(), &nf, &nd)) return fail(xbg_nc_error());
+            fr.resize(nf);
+            di.resize(nd);
+            S.resize((size_t)nf * nd);
+            if (xbg_nc_spectrum(wbq[k].c_str(), P.grdalpha, fr.data(), di.data(), S.data()))
+                return fail(xbg_nc_error());
+        } else {
+            if (xbg_jonswap_size(&w, &nf, &nd)) return fail("jonswap size");
+            fr.resize(nf);
+            di.resize(nd);
+            S.resize((size_t)nf * nd);
+            if (xbg_jonswap(&w, fr.data(), di.data(), S.data())) return fail("JONSWAP spectrum: NaN");
+        }
         unsigned seed = 0;
         if (P.random == 1) seed = (unsigned)std::time(nullptr);
         return xbg_wave_groups(&P, offdepth, nf, nd, fr.data(), di.data(), S.data(), seed, &Trep, Stfile.data(),
@@ -443,9 +538,11 @@ int xbg_run_case(const char* paramfile, int verbose) {
             if (xbg_constant_waves(&P, (int)wb.size(), wb.data(), theta.data(), Stfile.data(), qfile.data(), Tp.data()))
                 return fail("constant wave boundary");
             Trep = Tp[0];
+        } else if (IN.wavebndtype == 2) {
+            if (xb_block(0)) return XBG_E_ARG;
         } else if (IN.wavebndtype == 3) {
             if (read_block(0)) return XBG_E_ARG;
-        } else if (gen_block(wb[0]) || save_block(0, wb[0])) {
+        } else if (gen_block(0) || save_block(0, wb[0])) {
             return XBG_E_ARG;
         }
     }
@@ -518,9 +615,10 @@ int xbg_run_case(const char* paramfile, int verbose) {
             if (wb[WAV].time - totaltime < 0.0) {
                 WAV++;
                 if (WAV >= (int)wb.size()) WAV = (int)wb.size() - 1;
-                if (IN.wavebndtype == 3 || IN.wavebndtype == 4) {
-                    int r = IN.wavebndtype == 3 ? read_block(WAV - 1)
-                                                : (gen_block(wb[WAV - 1]) || save_block(1, wb[WAV - 1]));
+                if (IN.wavebndtype >= 2) {
+                    int r = IN.wavebndtype == 2   ? xb_block(WAV - 1)
+                            : IN.wavebndtype == 3 ? read_block(WAV - 1)
+                                                  : (gen_block(WAV - 1) || save_block(1, wb[WAV - 1]));
                     if (r) {
                         rc = XBG_E_ARG;
                         break;

@@ -1,6 +1,7 @@
 """netCDF input of the surf-beat model over the C-ABI of include/xbg.h:
-bathymetry grids (readgridncsize / readnczb, writenetcdf.cpp:743-927) and
-the wave-boundary reuse file XBG_bnd_reuse.nc (createbndnc / writebndnc /
+bathymetry grids (readgridncsize / readnczb, writenetcdf.cpp:743-927),
+2-D wave spectra (readSWANSPEC, writenetcdf.cpp:1288-1460) and the
+wave-boundary reuse file XBG_bnd_reuse.nc (createbndnc / writebndnc /
 read_reuse_bndnc*, writenetcdf.cpp:929-1245).
 
 The reader and writer are C++ (xbeach_gpu_amd/csrc/xbg_ncio.cpp) without
@@ -14,7 +15,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._ffi import XbgParams, lib
-from .bnd_jonswap import BoundaryBlock
+from .bnd_jonswap import BoundaryBlock, Spectrum
 
 
 class NcError(RuntimeError):
@@ -38,6 +39,19 @@ def read_bathy(path: str):
     zb = np.empty((ny.value, nx.value), np.float32)
     _chk(L.xbg_nc_read_zb(str(path).encode(), nx.value, ny.value, zb.ctypes.data_as(C.POINTER(C.c_float))))
     return nx.value, ny.value, dx.value, zb
+
+
+def read_spectrum(spec: str, grdalpha: float = 0.0) -> Spectrum:
+    """readSWANSPEC: a 2-D spectrum from "file.nc?var" (var "z" by default)
+    in the generator's convention (grid directions, per radian)."""
+    L = lib()
+    nf, nd = C.c_int(), C.c_int()
+    _chk(L.xbg_nc_spectrum_size(str(spec).encode(), C.byref(nf), C.byref(nd)))
+    f = np.empty(nf.value)
+    d = np.empty(nd.value)
+    S = np.empty((nd.value, nf.value))
+    _chk(L.xbg_nc_spectrum(str(spec).encode(), grdalpha, _dp(f), _dp(d), _dp(S)))
+    return Spectrum(f, d, S)
 
 
 @dataclass
