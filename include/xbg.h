@@ -319,18 +319,18 @@ int xbg_nc_read_zb(const char* path, int nx, int ny, float* zb);
 int xbg_nc_spectrum_size(const char* spec, int* nf, int* nd);
 int xbg_nc_spectrum(const char* spec, double grdalpha, double* freq, double* dir, double* S);
 
-/* read_reuse_bndnc_head (writenetcdf.cpp:1089-1131): ntheta, thetamin,
+/* read_reuse_bndnc_head (writenetcdf.cpp:1090-1152): ntheta, thetamin,
  * thetamax (first and last stored bin centres), dtheta =
  * (thetamax - thetamin)/ntheta, rtlength = tt[last], dtbc = tt[last] -
  * tt[last-1]; nrec (may be NULL) = records (blocks) in the file. */
 int xbg_bndnc_head(const char* path, xbg_params* p, int* nrec);
-/* read_reuse_bndnc (writenetcdf.cpp:1168-1245): record rec into the
+/* read_reuse_bndnc (writenetcdf.cpp:1190-1286): record rec into the
  * generator's Stfile / qfile layout (see xbg_wave_groups); the file must
  * hold ny rows, ntheta bins and ntt times. */
 int xbg_bndnc_read(const char* path, int rec, int ny, int ntheta, int ntt, double* Stfile, double* qfile);
 /* createbndnc (append 0: new file XBG_bnd_reuse.nc layout, writenetcdf.cpp:
  * 929-1023) / writebndnc (append 1: one more record, writenetcdf.cpp:
- * 1025-1087) of one generated block. */
+ * 1025-1088) of one generated block. */
 int xbg_bndnc_write(const char* path, int append, int ntt, int ny, int ntheta, double dy, double dtheta,
                     double thetamin, double dtbc, double time, double Hs, double Trep, double Tp, double Dp,
                     const double* Stfile, const double* qfile);

@@ -1,7 +1,7 @@
 """netCDF input (xbeach_gpu_amd/csrc/xbg_ncio.cpp): bathymetry grids as
 readgridncsize / readnczb read them (writenetcdf.cpp:743-927) and the
 wave-boundary reuse file of createbndnc / writebndnc / read_reuse_bndnc
-(writenetcdf.cpp:929-1245).
+(writenetcdf.cpp:929-1286).
 
 Files are written by scipy's independent netCDF-3 writer (classic and
 64-bit-offset) or, for the 64-bit-data format scipy lacks, byte by byte from

@@ -7,9 +7,9 @@
 //                                       "zb" (y, x) or (t, y, x)
 //   xbg_bndnc_head / xbg_bndnc_read     read_reuse_bndnc_head /
 //                                       read_reuse_bndnc
-//                                       (writenetcdf.cpp:1089-1245)
+//                                       (writenetcdf.cpp:1090-1286)
 //   xbg_bndnc_write                     createbndnc / writebndnc
-//                                       (writenetcdf.cpp:929-1087)
+//                                       (writenetcdf.cpp:929-1088)
 //   xbg_nc_spectrum(_size)              readSWANSPEC (writenetcdf.cpp:
 //                                       1288-1460): 2-D wave spectra
 //
@@ -555,7 +555,7 @@ int xbg_nc_spectrum(const char* spec, double grdalpha, double* freq, double* dir
     return XBG_OK;
 }
 
-// read_reuse_bndnc_head (writenetcdf.cpp:1089-1131): the direction bins and
+// read_reuse_bndnc_head (writenetcdf.cpp:1090-1152): the direction bins and
 // block timing of a reuse file.  thetamin / thetamax are the first and last
 // stored bin centres and dtheta = (thetamax - thetamin) / ntheta, as the
 // reference sets them; rtlength = tt[ntt-1], dtbc = tt[ntt-1] - tt[ntt-2].
@@ -584,7 +584,7 @@ int xbg_bndnc_head(const char* path, xbg_params* p, int* nrec) {
     return XBG_OK;
 }
 
-// read_reuse_bndnc (writenetcdf.cpp:1168-1245): record rec of a reuse file
+// read_reuse_bndnc (writenetcdf.cpp:1190-1286): record rec of a reuse file
 // into the generator's layout, Stfile[j + itheta*ny + m*ny*ntheta] and
 // qfile[j + k*ny + m*ny*4] (k = 0 qx, 1 qy, 2 and 3 zero).  The file must
 // hold ny rows, ntheta bins and ntt times.
@@ -624,7 +624,7 @@ int xbg_bndnc_read(const char* path, int rec, int ny, int ntheta, int ntt, doubl
 }
 
 // createbndnc (append = 0, a new file) / writebndnc (append = 1, a record
-// added to an existing reuse file), writenetcdf.cpp:929-1087.  Stfile and
+// added to an existing reuse file), writenetcdf.cpp:929-1088.  Stfile and
 // qfile are one block in the generator's layout (ntt records); the stored
 // coordinates are tt[m] = m*dtbc, yy[j] = j*dy and theta[i] = i*dtheta +
 // thetamin + 0.5*dtheta.

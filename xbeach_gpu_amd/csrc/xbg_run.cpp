@@ -392,7 +392,7 @@ int xbg_run_case(const char* paramfile, int verbose) {
         wb.push_back(last);
     } else if (IN.wavebndfile[0] && IN.wavebndtype == 3) {
         // read_reuse_bndnc_head / read_reuse_bndnc_vec (writenetcdf.cpp:
-        // 1089-1166): bins and block timing from the file, one block per record
+        // 1090-1188): bins and block timing from the file, one block per record
         int nrec = 0;
         if (xbg_bndnc_head(wbfile.c_str(), &P, &nrec)) return fail(xbg_nc_error());
         for (int i = 0; i < nrec + 1; i++) {
@@ -492,7 +492,7 @@ int xbg_run_case(const char* paramfile, int verbose) {
             return fail(xbg_nc_error());
         return XBG_OK;
     };
-    // read_reuse_bndnc (writenetcdf.cpp:1168-1245); Trep = 15 as it sets it
+    // read_reuse_bndnc (writenetcdf.cpp:1190-1286); Trep = 15 as it sets it
     auto read_block = [&](int rec) -> int {
         if (xbg_bndnc_read(wbfile.c_str(), rec, ny, nt, nrec_blk, Stfile.data(), qfile.data()))
             return fail(xbg_nc_error());

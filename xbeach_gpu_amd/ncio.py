@@ -2,7 +2,7 @@
 bathymetry grids (readgridncsize / readnczb, writenetcdf.cpp:743-927),
 2-D wave spectra (readSWANSPEC, writenetcdf.cpp:1288-1460) and the
 wave-boundary reuse file XBG_bnd_reuse.nc (createbndnc / writebndnc /
-read_reuse_bndnc*, writenetcdf.cpp:929-1245).
+read_reuse_bndnc*, writenetcdf.cpp:929-1286).
 
 The reader and writer are C++ (xbeach_gpu_amd/csrc/xbg_ncio.cpp) without
 the netCDF library; this module only shapes arrays.
