@@ -3,7 +3,7 @@
 // One step of the reference (Wave_gpu.cu:702-728) is ~92 launches, each a
 // 16x16-block stencil followed by a device synchronisation, plus ~25
 // cudaMalloc/cudaFree pairs and two full-field D2H copies for the time step.
-// Here the same step is 17 stream-ordered launches over buffers allocated
+// Here the same step is 14 stream-ordered launches over buffers allocated
 // once:
 //
 //   k_bnd_offshore  ubnd1D                                  (1 workgroup)
@@ -16,11 +16,10 @@
 //   k_flow_nuh      smago + wetu/wetv masks
 //   k_flow_u        u momentum (wlevslopes .. eulerustep, latbnd)
 //   k_flow_v        v momentum (vdvdy_adv .. eulervstep, vvlatbnd)
-//   k_flow_wrk      wrkuu2Ocorr + wrkvv2Ocorr
-//   k_flow_corr     uu2Ocorr + vv2Ocorr + depthhu/depthhv
-//   k_flow_cont     CalcQFlow + continuity
-//   k_flow_wrkzs    wrkzs2Ocorr
-//   k_flow_zs       zs2Ocorr + latbnd(zs) + hsbnd
+//   k_flow_uvcorr   wrkuu2Ocorr + wrkvv2Ocorr + uu2Ocorr + vv2Ocorr +
+//                   depthhu/depthhv (LDS tile with halo)
+//   k_flow_zsall    CalcQFlow + continuity + wrkzs2Ocorr + zs2Ocorr +
+//                   latbnd(zs) + hsbnd (LDS tile with halo)
 //   k_flow_uvvu     calcuvvu + uvlatbnd
 //   k_sed_flux      longturb + Rvr + Susp
 //   k_sed_conc      Sbvr + Erosus + Conc + CClatbnd + bedupdate + zblatbnd
@@ -72,7 +71,8 @@ struct Fld {
     // flow intermediates
     float* nuh;
     uint8_t* wet;  // bit0 wetu, bit1 wetv (udepthmomcont / vdepthmomcont)
-    float *uu_b, *vv_b, *w1u, *w2u, *w1v, *w2v, *zsn, *w1z, *w2z;
+    float *uu_b, *vv_b;
+    float* zs_out;  // new water level of the zs pass (the other zs buffer)
     // sediment intermediates
     float *Sus, *Svs, *C_out, *hh_b, *zb_b, *stdep_b, *dzbx, *dzby, *ceqsg;
     double* zb64_b;
@@ -105,7 +105,7 @@ XDEV float tab(const float* t, int th) {
 // The alongshore mean of uu[0,:] is taken over the pre-update column and
 // summed in row order by one lane, as a sequential sweep would.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bnd_offshore(KP p, SP s, Fld f, int waves) {
+__global__ __launch_bounds__(1024) void k_bnd_offshore(KP p, SP s, Fld f, int waves) {
     __shared__ float col[1024];
     __shared__ float avg;
     const int nx = p.nx, ny = p.ny;
@@ -930,114 +930,143 @@ __global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
     f.vv_b[ix2(p, x, y)] = eulervstep_at(p, s, f, x, ys);
 }
 
-// wrkuu2Ocorr (Flow_kernel.cu:1960-2044) and wrkvv2Ocorr (2046-2128)
-__global__ __launch_bounds__(256) void k_flow_wrk(KP p, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
+// wrkuu2Ocorr (1960-2044), wrkvv2Ocorr (2046-2128), uu2Ocorr (2210-2245),
+// vv2Ocorr (2247-2282) and depthhu / depthhv (653-750, uumin = 0) on the
+// corrected velocities (Flow_kernel.cu), in one pass over a 64 x 4 tile:
+//   1. zs, zb, qx, qy, u, uold, v, vold on the tile + 3-cell halo into LDS
+//      (slots hold the values at the clamped coordinates, which equal the
+//      mminus/pplus/mminus2/pplus2 neighbours);
+//   2. the four correction fluxes on the tile + 1-cell halo, into LDS;
+//   3. the corrected u, v of the cell and its flux depths.
+constexpr int DW = BX + 6, DH = BY + 6;  // data tile:  x0-3 .. x0+BX+2, y0-3 .. y0+BY+2
+constexpr int KW = BX + 2, KH = BY + 2;  // flux tile:  x0-1 .. x0+BX,   y0-1 .. y0+BY
+enum { T_ZS, T_ZB, T_QX, T_QY, T_U, T_UO, T_V, T_VO, T_N };
+__global__ __launch_bounds__(256) void k_flow_uvcorr(KP p, SP s, Fld f) {
+    __shared__ float dt_[T_N][DH][DW];
+    __shared__ float kt[4][KH][KW];
     const int nx = p.nx, ny = p.ny;
-    int xm = m1(x), xm2 = m2(x), xp = p1(x, nx), xp2 = p2(x, nx);
-    int ym = m1(y), ym2 = m2(y), yp = p1(y, ny), yp2 = p2(y, ny);
-    size_t i = ix2(p, x, y);
-    const float *zs = f.zs, *zb = f.zb, *qx = f.qx, *qy = f.qy;
-    auto I = [&](int a, int b) { return ix2(p, a, b); };
+    const int tid = threadIdx.x + threadIdx.y * BX;
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    const float* src[T_N] = {f.zs, f.zb, f.qx, f.qy, f.uu_b, f.uuold, f.vv_b, f.vvold};
+    for (int l = tid; l < DW * DH; l += BX * BY) {
+        int r = l / DW, c = l - r * DW;
+        int gxx = min(max(x0 - 3 + c, 0), nx - 1), gyy = min(max(y0 - 3 + r, 0), ny - 1);
+        size_t g = ix2(p, gxx, gyy);
+#pragma unroll
+        for (int k = 0; k < T_N; k++) dt_[k][r][c] = src[k][g];
+    }
+    __syncthreads();
+    // field k at unclamped (ux, uy)
+    auto D = [&](int k, int ux, int uy) { return dt_[k][uy - (y0 - 3)][ux - (x0 - 3)]; };
     const float two_eps = p.eps * 2.0f;
-    // u
-    {
-        const float *uu = f.uu_b, *uo = f.uuold;
-        float w1 = 0.0f, w2 = 0.0f;
-        float minzs = fmn(fmn(fmn(fmn(zs[I(xm2, y)], zs[I(xm, y)]), zs[i]), zs[I(xp, y)]), zs[I(xp2, y)]);
-        float minzb = fmn(fmn(fmn(fmn(zb[I(xm2, y)], zb[I(xm, y)]), zb[i]), zb[I(xp, y)]), zb[I(xp2, y)]);
-        if (x > 0 && minzs + minzb > two_eps) {
-            float qs = qx[i] + qx[I(xm, y)];
-            if (qs > 0.0f) w1 = 0.5f * minmod(uu[i] - uo[I(xm, y)], uu[I(xm, y)] - uo[I(xm2, y)]);
-            if (qs < 0.0f) w1 = -0.5f * minmod(uo[i] - uu[I(xm, y)], uo[I(xp, y)] - uu[i]);
+    for (int l = tid; l < KW * KH; l += BX * BY) {
+        int r = l / KW, c = l - r * KW;
+        int x = x0 - 1 + c, y = y0 - 1 + r;
+        float w1u = 0.0f, w2u = 0.0f, w1v = 0.0f, w2v = 0.0f;
+        if (x >= 0 && y >= 0 && x < nx && y < ny) {
+            // u (wrkuu2Ocorr)
+            float minzs = fmn(fmn(fmn(fmn(D(T_ZS, x - 2, y), D(T_ZS, x - 1, y)), D(T_ZS, x, y)), D(T_ZS, x + 1, y)),
+                              D(T_ZS, x + 2, y));
+            float minzb = fmn(fmn(fmn(fmn(D(T_ZB, x - 2, y), D(T_ZB, x - 1, y)), D(T_ZB, x, y)), D(T_ZB, x + 1, y)),
+                              D(T_ZB, x + 2, y));
+            if (x > 0 && minzs + minzb > two_eps) {
+                float qs = D(T_QX, x, y) + D(T_QX, x - 1, y);
+                if (qs > 0.0f)
+                    w1u = 0.5f * minmod(D(T_U, x, y) - D(T_UO, x - 1, y), D(T_U, x - 1, y) - D(T_UO, x - 2, y));
+                if (qs < 0.0f)
+                    w1u = -0.5f * minmod(D(T_UO, x, y) - D(T_U, x - 1, y), D(T_UO, x + 1, y) - D(T_U, x, y));
+            }
+            minzs = fmn(fmn(fmn(D(T_ZS, x, y), D(T_ZS, x, y - 1)), D(T_ZS, x, y + 1)), D(T_ZS, x, y + 2));
+            minzs = fmn(minzs, fmn(fmn(fmn(D(T_ZS, x + 1, y), D(T_ZS, x + 1, y - 1)), D(T_ZS, x + 1, y + 1)),
+                                   D(T_ZS, x + 1, y + 2)));
+            minzb = fmn(fmn(fmn(D(T_ZB, x, y), D(T_ZB, x, y - 1)), D(T_ZB, x, y + 1)), D(T_ZB, x, y + 2));
+            minzb = fmn(minzb, fmn(fmn(fmn(D(T_ZB, x + 1, y), D(T_ZB, x + 1, y - 1)), D(T_ZB, x + 1, y + 1)),
+                                   D(T_ZB, x + 1, y + 2)));
+            if (x > 0 && y > 0 && minzs + minzb > two_eps) {
+                float qs = D(T_QY, x + 1, y) + D(T_QY, x, y);
+                if (qs > 0.0f)
+                    w2u = 0.5f * minmod(D(T_U, x, y + 1) - D(T_UO, x, y), D(T_U, x, y) - D(T_UO, x, y - 1));
+                if (qs < 0.0f)
+                    w2u = -0.5f * minmod(D(T_UO, x, y + 1) - D(T_U, x, y), D(T_UO, x, y + 2) - D(T_U, x, y + 1));
+            }
+            // v (wrkvv2Ocorr)
+            bool in = x > 0 && y > 0 && y < ny - 2;
+            minzs = fmn(fmn(fmn(fmn(D(T_ZS, x, y - 2), D(T_ZS, x, y - 1)), D(T_ZS, x, y)), D(T_ZS, x, y + 1)),
+                        D(T_ZS, x, y + 2));
+            minzb = fmn(fmn(fmn(fmn(D(T_ZB, x, y - 2), D(T_ZB, x, y - 1)), D(T_ZB, x, y)), D(T_ZB, x, y + 1)),
+                        D(T_ZB, x, y + 2));
+            if (in && minzs + minzb > two_eps) {
+                float qs = D(T_QY, x, y) + D(T_QY, x, y - 1);
+                if (qs > 0.0f)
+                    w1v = 0.5f * minmod(D(T_V, x, y) - D(T_VO, x, y - 1), D(T_V, x, y - 1) - D(T_VO, x, y - 2));
+                if (qs < 0.0f)
+                    w1v = -0.5f * minmod(D(T_VO, x, y) - D(T_V, x, y - 1), D(T_VO, x, y + 1) - D(T_V, x, y));
+            }
+            minzs = fmn(fmn(fmn(D(T_ZS, x - 1, y), D(T_ZS, x, y)), D(T_ZS, x + 1, y)), D(T_ZS, x + 2, y));
+            minzs = fmn(minzs, fmn(fmn(fmn(D(T_ZS, x - 1, y + 1), D(T_ZS, x, y + 1)), D(T_ZS, x + 1, y + 1)),
+                                   D(T_ZS, x + 2, y + 1)));
+            minzb = fmn(fmn(fmn(D(T_ZB, x - 1, y), D(T_ZB, x, y)), D(T_ZB, x + 1, y)), D(T_ZB, x + 2, y));
+            minzb = fmn(minzb, fmn(fmn(fmn(D(T_ZB, x - 1, y + 1), D(T_ZB, x, y + 1)), D(T_ZB, x + 1, y + 1)),
+                                   D(T_ZB, x + 2, y + 1)));
+            if (in && minzs + minzb > two_eps) {
+                float qs = D(T_QX, x, y + 1) + D(T_QX, x, y);
+                if (qs > 0.0f)
+                    w2v = 0.5f * minmod(D(T_V, x + 1, y) - D(T_VO, x, y), D(T_V, x, y) - D(T_VO, x - 1, y));
+                if (qs < 0.0f)
+                    w2v = -0.5f * minmod(D(T_VO, x + 1, y) - D(T_V, x, y), D(T_VO, x + 2, y) - D(T_V, x + 1, y));
+            }
         }
-        minzs = fmn(fmn(fmn(zs[i], zs[I(x, ym)]), zs[I(x, yp)]), zs[I(x, yp2)]);
-        minzs = fmn(minzs, fmn(fmn(fmn(zs[I(xp, y)], zs[I(xp, ym)]), zs[I(xp, yp)]), zs[I(xp, yp2)]));
-        minzb = fmn(fmn(fmn(zb[i], zb[I(x, ym)]), zb[I(x, yp)]), zb[I(x, yp2)]);
-        minzb = fmn(minzb, fmn(fmn(fmn(zb[I(xp, y)], zb[I(xp, ym)]), zb[I(xp, yp)]), zb[I(xp, yp2)]));
-        if (x > 0 && y > 0 && minzs + minzb > two_eps) {
-            float qs = qy[I(xp, y)] + qy[i];
-            if (qs > 0.0f) w2 = 0.5f * minmod(uu[I(x, yp)] - uo[i], uu[i] - uo[I(x, ym)]);
-            if (qs < 0.0f) w2 = -0.5f * minmod(uo[I(x, yp)] - uu[i], uo[I(x, yp2)] - uu[I(x, yp)]);
-        }
-        f.w1u[i] = w1;
-        f.w2u[i] = w2;
+        kt[0][r][c] = w1u;
+        kt[1][r][c] = w2u;
+        kt[2][r][c] = w1v;
+        kt[3][r][c] = w2v;
     }
-    // v
-    {
-        const float *vv = f.vv_b, *vo = f.vvold;
-        float w1 = 0.0f, w2 = 0.0f;
-        float minzs = fmn(fmn(fmn(fmn(zs[I(x, ym2)], zs[I(x, ym)]), zs[i]), zs[I(x, yp)]), zs[I(x, yp2)]);
-        float minzb = fmn(fmn(fmn(fmn(zb[I(x, ym2)], zb[I(x, ym)]), zb[i]), zb[I(x, yp)]), zb[I(x, yp2)]);
-        bool in = x > 0 && y > 0 && y < ny - 2;
-        if (in && minzs + minzb > two_eps) {
-            float qs = qy[i] + qy[I(x, ym)];
-            if (qs > 0.0f) w1 = 0.5f * minmod(vv[i] - vo[I(x, ym)], vv[I(x, ym)] - vo[I(x, ym2)]);
-            if (qs < 0.0f) w1 = -0.5f * minmod(vo[i] - vv[I(x, ym)], vo[I(x, yp)] - vv[i]);
-        }
-        minzs = fmn(fmn(fmn(zs[I(xm, y)], zs[i]), zs[I(xp, y)]), zs[I(xp2, y)]);
-        minzs = fmn(minzs, fmn(fmn(fmn(zs[I(xm, yp)], zs[I(x, yp)]), zs[I(xp, yp)]), zs[I(xp2, yp)]));
-        minzb = fmn(fmn(fmn(zb[I(xm, y)], zb[i]), zb[I(xp, y)]), zb[I(xp2, y)]);
-        minzb = fmn(minzb, fmn(fmn(fmn(zb[I(xm, yp)], zb[I(x, yp)]), zb[I(xp, yp)]), zb[I(xp2, yp)]));
-        if (in && minzs + minzb > two_eps) {
-            float qs = qx[I(x, yp)] + qx[i];
-            if (qs > 0.0f) w2 = 0.5f * minmod(vv[I(xp, y)] - vo[i], vv[i] - vo[I(xm, y)]);
-            if (qs < 0.0f) w2 = -0.5f * minmod(vo[I(xp, y)] - vv[i], vo[I(xp2, y)] - vv[I(xp, y)]);
-        }
-        f.w1v[i] = w1;
-        f.w2v[i] = w2;
-    }
-}
-
-// uu2Ocorr (Flow_kernel.cu:2210-2245), vv2Ocorr (2247-2282), then
-// depthhu / depthhv (653-750, uumin = 0) on the corrected velocities.
-__global__ __launch_bounds__(256) void k_flow_corr(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
-    const int nx = p.nx, ny = p.ny;
-    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= nx || y >= ny) return;
+    auto K = [&](int k, int ux, int uy) { return kt[k][uy - (y0 - 1)][ux - (x0 - 1)]; };
     size_t i = ix2(p, x, y);
-    auto I = [&](int a, int b) { return ix2(p, a, b); };
-    const float *qx = f.qx, *qy = f.qy;
-    const float dx = p.dx, dt = s.dt;
-    float u = f.uu_b[i], v = f.vv_b[i];
+    const float dx = p.dx, dts = s.dt;
+    float u = D(T_U, x, y), v = D(T_V, x, y);
     if (x > 0 && y > 0) {
-        float qe = 0.5f * (qx[I(xp, y)] + qx[i]);
-        float qw = 0.5f * (qx[i] + qx[I(xm, y)]);
-        float qn = 0.5f * (qy[I(x, ym)] + qy[I(xp, ym)]);
-        float qs = 0.5f * (qy[i] + qy[I(xm, y)]);
-        u = u - dt / hum_at(p, f, x, y) *
-                    ((qe * f.w1u[I(xp, y)] - qw * f.w1u[i]) / dx + (qs * f.w2u[i] - qn * f.w2u[I(x, ym)]) / dx);
-        qe = 0.5f * (qx[I(x, yp)] + qx[i]);
-        qw = 0.5f * (qx[I(xm, y)] + qx[I(xm, yp)]);
-        qn = 0.5f * (qy[i] + qy[I(x, ym)]);
-        qs = 0.5f * (qy[I(x, yp)] + qy[i]);
-        v = v - dt / hvm_at(p, f, x, y) *
-                    ((qe * f.w2v[i] - qw * f.w2v[I(xm, y)]) / dx + (qs * f.w1v[I(x, yp)] - qn * f.w1v[i]) / dx);
+        float qe = 0.5f * (D(T_QX, x + 1, y) + D(T_QX, x, y));
+        float qw = 0.5f * (D(T_QX, x, y) + D(T_QX, x - 1, y));
+        float qn = 0.5f * (D(T_QY, x, y - 1) + D(T_QY, x + 1, y - 1));
+        float qs = 0.5f * (D(T_QY, x, y) + D(T_QY, x - 1, y));
+        // the neighbours of the reference's uu2Ocorr are clamped: at the
+        // last column xplus is the cell itself
+        int xp = p1(x, nx), ym = m1(y), xm = m1(x), yp = p1(y, ny);
+        u = u - dts / hum_at(p, f, x, y) *
+                    ((qe * K(0, xp, y) - qw * K(0, x, y)) / dx + (qs * K(1, x, y) - qn * K(1, x, ym)) / dx);
+        qe = 0.5f * (D(T_QX, x, y + 1) + D(T_QX, x, y));
+        qw = 0.5f * (D(T_QX, x - 1, y) + D(T_QX, x - 1, y + 1));
+        qn = 0.5f * (D(T_QY, x, y) + D(T_QY, x, y - 1));
+        qs = 0.5f * (D(T_QY, x, y + 1) + D(T_QY, x, y));
+        v = v - dts / hvm_at(p, f, x, y) *
+                    ((qe * K(3, x, y) - qw * K(3, xm, y)) / dx + (qs * K(2, x, yp) - qn * K(2, x, y)) / dx);
     }
     f.uu[i] = u;
     f.vv[i] = v;
-    const float *zs = f.zs, *zb = f.zb;
     const float ummn = 0.0f;
-    size_t r = I(xp, y), t = I(x, yp);
-    float hb = fmx(-1.0f * zb[i], -1.0f * zb[r]);
+    float zsi = D(T_ZS, x, y), zbi = D(T_ZB, x, y);
+    float zsr = D(T_ZS, x + 1, y), zbr = D(T_ZB, x + 1, y);
+    float zst = D(T_ZS, x, y + 1), zbt = D(T_ZB, x, y + 1);
+    float hb = fmx(-1.0f * zbi, -1.0f * zbr);
     float h;
-    if (u > ummn) h = zs[i] - hb;
-    else if (u < -1.0f * ummn) h = zs[r] - hb;
-    else h = fmx(fmx(zs[i], zs[r]) - hb, p.eps);
+    if (u > ummn) h = zsi - hb;
+    else if (u < -1.0f * ummn) h = zsr - hb;
+    else h = fmx(fmx(zsi, zsr) - hb, p.eps);
     f.hu[i] = fmx(h, 0.0f);
-    hb = fmx(-1.0f * zb[i], -1.0f * zb[t]);
-    if (v > ummn) h = zs[i] - hb;
-    else if (v < -1 * ummn) h = zs[t] - hb;
-    else h = fmx(fmx(zs[i], zs[t]) - hb, p.eps);
+    hb = fmx(-1.0f * zbi, -1.0f * zbt);
+    if (v > ummn) h = zsi - hb;
+    else if (v < -1 * ummn) h = zst - hb;
+    else h = fmx(fmx(zsi, zst) - hb, p.eps);
     f.hv[i] = fmx(h, 0.0f);
 }
 
-// CalcQFlow (Flow_kernel.cu:1917-1958) + continuity (1626-1681)
-__global__ __launch_bounds__(256) void k_flow_cont(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
+// CalcQFlow (Flow_kernel.cu:1917-1958) + continuity (1626-1681) at (x, y):
+// the water level after continuity; *dzdt_out gets dzsdt.
+XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, float* dzdt_out) {
     size_t i = ix2(p, x, y);
     float z = f.zs[i];
     if (x > 0) {
@@ -1045,64 +1074,89 @@ __global__ __launch_bounds__(256) void k_flow_cont(KP p, SP s, Fld f) {
         float qxi = f.uu[i] * f.hu[i], qxl = f.uu[l] * f.hu[l];
         float qyi = f.vv[i] * f.hv[i], qyb = f.vv[b] * f.hv[b];
         float dzdt = (qxl - qxi + qyb - qyi) / p.dx;
-        f.dzsdt[i] = dzdt;
+        *dzdt_out = dzdt;
         z = z + dzdt * s.dt;
     }
-    f.zsn[i] = z;
-}
-
-// wrkzs2Ocorr (Flow_kernel.cu:2130-2208)
-__global__ __launch_bounds__(256) void k_flow_wrkzs(KP p, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
-    const int nx = p.nx, ny = p.ny;
-    int xm = m1(x), xp = p1(x, nx), xp2 = p2(x, nx);
-    int ym = m1(y), yp = p1(y, ny), yp2 = p2(y, ny);
-    size_t i = ix2(p, x, y);
-    auto I = [&](int a, int b) { return ix2(p, a, b); };
-    const float *zs = f.zsn, *zo = f.zsold, *zb = f.zb;
-    const float two_eps = p.eps * 2.0f;
-    float u = f.uu[i], v = f.vv[i];
-    float w1 = 0.0f, w2 = 0.0f;
-    float minzs = fmn(fmn(fmn(zs[I(xm, y)], zs[i]), zs[I(xp, y)]), zs[I(xp2, y)]);
-    float minzb = fmn(fmn(fmn(zb[I(xm, y)], zb[i]), zb[I(xp, y)]), zb[I(xp2, y)]);
-    if (x > 0 && minzs + minzb > two_eps) {
-        if (u > 0.0f) w1 = u * 0.5f * minmod(zs[I(xp, y)] - zo[i], zs[i] - zo[I(xm, y)]);
-        if (u < 0.0f) w1 = -1.0f * u * 0.5f * minmod(zo[I(xp2, y)] - zs[I(xp, y)], zo[I(xp, y)] - zs[i]);
-    }
-    minzs = fmn(fmn(fmn(zs[I(x, ym)], zs[i]), zs[I(x, yp)]), zs[I(x, yp2)]);
-    minzb = fmn(fmn(fmn(zb[I(x, ym)], zb[i]), zb[I(x, yp)]), zb[I(x, yp2)]);
-    if (x > 0 && y > 0 && minzs + minzb > two_eps) {
-        if (v > 0.0f) w2 = v * 0.5f * minmod(zs[I(x, yp)] - zo[i], zs[i] - zo[I(x, ym)]);
-        if (v < 0.0f) w2 = v * 0.5f * minmod(zo[I(x, yp2)] - zs[I(x, yp)], zo[I(x, yp)] - zs[i]);
-    }
-    f.w1z[i] = w1;
-    f.w2z[i] = w2;
-}
-
-// zs2Ocorr (Flow_kernel.cu:2284-2319) value at (x, y)
-XDEV float zs2_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
-    size_t i = ix2(p, x, y);
-    float z = f.zsn[i];
-    if (x > 0 && y > 0)
-        z = z - s.dt * ((f.w1z[i] - f.w1z[ix2(p, x - 1, y)]) / p.dx + (f.w2z[i] - f.w2z[ix2(p, x, y - 1)]) / p.dx);
     return z;
 }
 
-// zs2Ocorr + latbnd(zs) (rows 0 / ny-1 copy rows 1 / ny-2) + hsbnd
-// (Flow_kernel.cu:1683-1710); qx/qy get the correction fluxes.
-__global__ __launch_bounds__(256) void k_flow_zs(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
+// Continuity, wrkzs2Ocorr (Flow_kernel.cu:2130-2208), zs2Ocorr (2284-2319),
+// latbnd(zs) and hsbnd (1683-1710) in one pass over a 64 x 4 tile:
+//   1. water level after continuity on the tile + (-2..+1, -3..+1) halo
+//      (the halo the correction fluxes of step 2 reach), into LDS;
+//   2. correction fluxes w1/w2 on the tile + (-1, -2..-1) halo, into LDS;
+//   3. the corrected level of the cell (rows 0 / ny-1 take rows 1 / ny-2),
+//      the fluxes qx/qy with their corrections, and hh.
+// Halo slots hold the values at the clamped coordinates, which equal the
+// mminus/pplus/mminus2/pplus2 neighbours.  The new level goes to the other
+// zs buffer: other tiles still read the old one in step 1.
+constexpr int ZW = BX + 4, ZH = BY + 5;  // level tile: x0-2 .. x0+BX+1, y0-3 .. y0+BY+1
+constexpr int FW = BX + 1, FH = BY + 2;  // flux tile:  x0-1 .. x0+BX-1, y0-2 .. y0+BY-1
+__global__ __launch_bounds__(256) void k_flow_zsall(KP p, SP s, Fld f) {
+    __shared__ float zt[ZH][ZW];
+    __shared__ float w1t[FH][FW], w2t[FH][FW];
+    const int nx = p.nx, ny = p.ny;
+    const int tid = threadIdx.x + threadIdx.y * BX;
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    for (int l = tid; l < ZW * ZH; l += BX * BY) {
+        int r = l / ZW, c = l - r * ZW;
+        int ux = x0 - 2 + c, uy = y0 - 3 + r;
+        int gxx = min(max(ux, 0), nx - 1), gyy = min(max(uy, 0), ny - 1);
+        float dzdt = 0.0f;
+        float z = continuity_at(p, s, f, gxx, gyy, &dzdt);
+        zt[r][c] = z;
+        if (ux == gxx && uy == gyy && ux >= x0 && ux < x0 + BX && uy >= y0 && uy < y0 + BY && gxx > 0)
+            f.dzsdt[ix2(p, gxx, gyy)] = dzdt;
+    }
+    __syncthreads();
+    const float two_eps = p.eps * 2.0f;
+    for (int l = tid; l < FW * FH; l += BX * BY) {
+        int r = l / FW, c = l - r * FW;
+        int wx = x0 - 1 + c, wy = y0 - 2 + r;
+        float w1 = 0.0f, w2 = 0.0f;
+        if (wx >= 0 && wy >= 0 && wx < nx && wy < ny) {
+            // level at unclamped offsets (dx, dy) from the flux cell
+            auto Z = [&](int dx_, int dy_) { return zt[wy + dy_ - (y0 - 3)][wx + dx_ - (x0 - 2)]; };
+            int xm = m1(wx), xp = p1(wx, nx), xp2 = p2(wx, nx);
+            int ym = m1(wy), yp = p1(wy, ny), yp2 = p2(wy, ny);
+            auto I = [&](int a, int b) { return ix2(p, a, b); };
+            const float *zo = f.zsold, *zb = f.zb;
+            size_t i = I(wx, wy);
+            float u = f.uu[i], v = f.vv[i];
+            float minzs = fmn(fmn(fmn(Z(-1, 0), Z(0, 0)), Z(1, 0)), Z(2, 0));
+            float minzb = fmn(fmn(fmn(zb[I(xm, wy)], zb[i]), zb[I(xp, wy)]), zb[I(xp2, wy)]);
+            if (wx > 0 && minzs + minzb > two_eps) {
+                if (u > 0.0f) w1 = u * 0.5f * minmod(Z(1, 0) - zo[i], Z(0, 0) - zo[I(xm, wy)]);
+                if (u < 0.0f) w1 = -1.0f * u * 0.5f * minmod(zo[I(xp2, wy)] - Z(1, 0), zo[I(xp, wy)] - Z(0, 0));
+            }
+            minzs = fmn(fmn(fmn(Z(0, -1), Z(0, 0)), Z(0, 1)), Z(0, 2));
+            minzb = fmn(fmn(fmn(zb[I(wx, ym)], zb[i]), zb[I(wx, yp)]), zb[I(wx, yp2)]);
+            if (wx > 0 && wy > 0 && minzs + minzb > two_eps) {
+                if (v > 0.0f) w2 = v * 0.5f * minmod(Z(0, 1) - zo[i], Z(0, 0) - zo[I(wx, ym)]);
+                if (v < 0.0f) w2 = v * 0.5f * minmod(zo[I(wx, yp2)] - Z(0, 1), zo[I(wx, yp)] - Z(0, 0));
+            }
+        }
+        w1t[r][c] = w1;
+        w2t[r][c] = w2;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= nx || y >= ny) return;
     size_t i = ix2(p, x, y);
-    int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
-    float z = zs2_at(p, s, f, x, ys);
+    int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
+    const int cz = x - (x0 - 2), cf = x - (x0 - 1);
+    float z = zt[ys - (y0 - 3)][cz];
+    if (x > 0 && ys > 0) {
+        int rf = ys - (y0 - 2);
+        z = z - s.dt * ((w1t[rf][cf] - w1t[rf][cf - 1]) / p.dx + (w2t[rf][cf] - w2t[rf - 1][cf]) / p.dx);
+    }
     float qxv = f.uu[i] * f.hu[i], qyv = f.vv[i] * f.hv[i];
     if (x > 0 && y > 0) {
-        qxv = qxv + f.w1z[i];
-        qyv = qyv + f.w2z[i];
+        int rf = y - (y0 - 2);
+        qxv = qxv + w1t[rf][cf];
+        qyv = qyv + w2t[rf][cf];
     }
-    f.zs[i] = z;
+    f.zs_out[i] = z;
     f.qx[i] = qxv;
     f.qy[i] = qyv;
     f.hh[i] = fmx(f.zb[i] + z, p.eps);
@@ -1573,7 +1627,8 @@ struct xbg_ctx {
     float* ee[2] = {nullptr, nullptr};
     float* rr[2] = {nullptr, nullptr};
     float* Cb[2] = {nullptr, nullptr};
-    int cur = 0, ccur = 0;
+    int cur = 0, ccur = 0, zcur = 0;
+    float* zsb[2] = {nullptr, nullptr};
     float* St = nullptr;
     float* dirs = nullptr;
     float* qbo = nullptr;
@@ -1727,6 +1782,8 @@ void sync_ptrs(xbg_ctx* c) {
     c->f.rr_in = c->rr[c->cur];
     c->f.rr_out = c->rr[1 - c->cur];
     c->f.C = c->Cb[c->ccur];
+    c->f.zs = c->zsb[c->zcur];
+    c->f.zs_out = c->zsb[1 - c->zcur];
     c->f.C_out = c->Cb[1 - c->ccur];
     c->f.St = c->St;
     c->f.dirs = c->dirs;
@@ -1741,7 +1798,7 @@ void register_fields(xbg_ctx* c) {
     R("ee", &c->ee[0], ND);  // resolved through cur
     R("rr", &c->rr[0], ND);
     R("C", &c->Cb[0], N);
-    R("zs", &f.zs, N); R("uu", &f.uu, N); R("vv", &f.vv, N); R("hh", &f.hh, N); R("zb", &f.zb, N);
+    R("zs", &c->zsb[0], N); R("uu", &f.uu, N); R("vv", &f.vv, N); R("hh", &f.hh, N); R("zb", &f.zb, N);
     R("stdep", &f.stdep, N); R("hu", &f.hu, N); R("hv", &f.hv, N); R("ueu", &f.ueu, N);
     R("vev", &f.vev, N); R("vmageu", &f.vmageu, N); R("vmagev", &f.vmagev, N); R("vu", &f.vu, N);
     R("uv", &f.uv, N); R("qx", &f.qx, N); R("qy", &f.qy, N); R("uuold", &f.uuold, N);
@@ -1764,6 +1821,7 @@ float* resolve(xbg_ctx* c, const std::string& name, size_t* cnt) {
     if (name == "ee") return c->ee[c->cur];
     if (name == "rr") return c->rr[c->cur];
     if (name == "C") return c->Cb[c->ccur];
+    if (name == "zs") return c->zsb[c->zcur];
     return *it->second.slot;
 }
 
@@ -1804,11 +1862,10 @@ int launch_flow(xbg_ctx* c, const SP& s) {
     LAUNCH(c, "k_flow_nuh", k_flow_nuh, g, blk2, p, c->f);
     LAUNCH(c, "k_flow_u", k_flow_u, g, blk2, p, s, c->f);
     LAUNCH(c, "k_flow_v", k_flow_v, g, blk2, p, s, c->f);
-    LAUNCH(c, "k_flow_wrk", k_flow_wrk, g, blk2, p, c->f);
-    LAUNCH(c, "k_flow_corr", k_flow_corr, g, blk2, p, s, c->f);
-    LAUNCH(c, "k_flow_cont", k_flow_cont, g, blk2, p, s, c->f);
-    LAUNCH(c, "k_flow_wrkzs", k_flow_wrkzs, g, blk2, p, c->f);
-    LAUNCH(c, "k_flow_zs", k_flow_zs, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_flow_uvcorr", k_flow_uvcorr, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_flow_zsall", k_flow_zsall, g, blk2, p, s, c->f);
+    c->zcur = 1 - c->zcur;
+    sync_ptrs(c);
     LAUNCH(c, "k_flow_uvvu", k_flow_uvvu, g, blk2, p, c->f);
     return 0;
 }
@@ -1866,13 +1923,14 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
 #define A(ptr, n) if ((rc = dalloc(c, ptr, n)) != 0) { g_create_error = c->err; xbg_destroy(c); return rc; }
     A(&c->ee[0], ND); A(&c->ee[1], ND); A(&c->rr[0], ND); A(&c->rr[1], ND);
     A(&c->Cb[0], N); A(&c->Cb[1], N);
+    A(&c->zsb[0], N); A(&c->zsb[1], N);
     A(&c->St, ny * P.ntheta); A(&c->qbo, 4 * ny); A(&c->qbn, 4 * ny);
     A(&c->dirs, 8 * MAXT);
-    float** two[] = {&f.zs, &f.uu, &f.vv, &f.hh, &f.zb, &f.stdep, &f.hu, &f.hv, &f.ueu, &f.vev, &f.vmageu,
+    float** two[] = {&f.uu, &f.vv, &f.hh, &f.zb, &f.stdep, &f.hu, &f.hv, &f.ueu, &f.vev, &f.vmageu,
                      &f.vmagev, &f.vu, &f.uv, &f.qx, &f.qy, &f.uuold, &f.vvold, &f.zsold, &f.dzsdt,
                      &f.rolthick, &f.dzb, &f.k, &f.c, &f.cg, &f.H, &f.E, &f.Dw, &f.DR, &f.R, &f.urms,
                      &f.ust_raw, &f.ust, &f.thetamean, &f.Sxx, &f.Sxy, &f.Syy, &f.Fx, &f.Fy, &f.nuh,
-                     &f.uu_b, &f.vv_b, &f.w1u, &f.w2u, &f.w1v, &f.w2v, &f.zsn, &f.w1z, &f.w2z, &f.Sus,
+                     &f.uu_b, &f.vv_b, &f.Sus,
                      &f.Svs, &f.hh_b, &f.zb_b, &f.stdep_b, &f.dzbx, &f.dzby, &f.ceqsg, &f.zbc, &f.zbc_b,
                      &c->Hmean, &c->uumean, &c->vvmean, &c->hhmean, &c->zsmean, &c->zsmax, &c->Cmean};
     for (auto** q : two) { A(q, N); }
@@ -2066,7 +2124,7 @@ int xbg_use_bnd_record(xbg_ctx* c, int rec, double timesincelast, double timenex
 int xbg_flow_bnd(xbg_ctx* c, double dt, double totaltime) {
     if (!c) return XBG_E_ARG;
     SP s = make_sp(c, dt, totaltime);
-    LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(256), c->kp, s, c->f, c->P.swave == 1 ? 1 : 0);
+    LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), c->kp, s, c->f, c->P.swave == 1 ? 1 : 0);
     return XBG_OK;
 }
 
@@ -2090,7 +2148,7 @@ int xbg_step(xbg_ctx* c, double dt, double totaltime) {
     SP s = make_sp(c, dt, totaltime);
     int rc;
     if (c->P.flow == 1) {
-        LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(256), c->kp, s, c->f, c->P.swave == 1 ? 1 : 0);
+        LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), c->kp, s, c->f, c->P.swave == 1 ? 1 : 0);
     }
     if (c->P.swave == 1 && (rc = launch_wave(c, s)) != 0) return rc;
     if (c->P.flow == 1 && (rc = launch_flow(c, s)) != 0) return rc;
