@@ -119,7 +119,19 @@ def cpu_baseline(p, budget_s=20.0):
     wall = time.perf_counter() - w0
     o.close()
     return {"value": p.nx * p.ny * n / wall, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} whole steps of the {p.nx}x{p.ny}x{p.ntheta} domain, oracle FP32, 1 thread"}
+            "sample": f"{n} whole steps of the {p.nx}x{p.ny}x{p.ntheta} domain, oracle FP32, 1 thread",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(p):

@@ -26,7 +26,8 @@
  * One context is used by one host thread.  Each context owns one HIP stream
  * and all of its device memory, allocated once in xbg_create; no call after
  * xbg_create allocates device memory, except xbg_set_bnd_series (which sizes
- * the resident boundary record buffers once per series).
+ * the resident boundary record buffers once per series) and xbg_set_points
+ * (the point time-series buffer, once per point list).
  */
 #ifndef XBG_H
 #define XBG_H
@@ -211,6 +212,20 @@ int xbg_reset_means(xbg_ctx* ctx);
 int xbg_divide_means(xbg_ctx* ctx, double nstep);
 
 /*
+ * Point time series of zs and H (TSOfile / TSnode, Wave_gpu.cu:764-781,
+ * 875-890; the reference copies both whole fields to the host every step
+ * for this).  xbg_set_points takes the cells (ix + iy*nx) and a capacity in
+ * records and sizes a device buffer once; xbg_record_points enqueues one
+ * record (a gather of npoints cells, no host sync); xbg_read_points
+ * synchronises, copies the records out (zs[r*npoints + k], H[...]) and
+ * empties the buffer.  A full buffer makes xbg_record_points return
+ * XBG_E_STATE.
+ */
+int xbg_set_points(xbg_ctx* ctx, const int* cells, int npoints, int capacity);
+int xbg_record_points(xbg_ctx* ctx);
+int xbg_read_points(xbg_ctx* ctx, float* zs, float* H, int cap_records, int* nrec);
+
+/*
  * Water volume sum(hh)*dx*dx and bed volume sum(zb)*dx*dx in FP64
  * (compensated device reduction; no counterpart in the reference).
  */
@@ -342,11 +357,15 @@ int xbg_bndnc_write(const char* path, int append, int ntt, int ny, int ntheta, d
  * ---------------------------------------------------------------------- */
 
 /* The non-numeric part of XBGPUParam (XBeachGPU.h:116-139). */
+#define XBG_MAX_TS 64   /* point time-series outputs (TSOfile / TSnode pairs) */
 typedef struct xbg_inputs {
     char bathy[512], outfile[512], sedthk[512], wavebndfile[512], slbndfile[512], windfile[512];
     int wavebndtype;       /* 1 constant, 2 XBeach files, 3 reuse file, 4 JONSWAP, 5 spectra */
     double dt;             /* fixed time step key (read, not used by the GPU loop) */
     char outvars[2048];    /* comma-separated output variable names */
+    int ntsofile, ntsnode; /* TSOfile / TSnode keys in the order given (read_input.cpp:1018-1040) */
+    char tsofile[XBG_MAX_TS][256];
+    int tsnode_i[XBG_MAX_TS], tsnode_j[XBG_MAX_TS];
 } xbg_inputs;
 
 /* Defaults of the non-numeric part (outfile XBGPU_output.nc, wavebndtype 2). */

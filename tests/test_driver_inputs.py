@@ -80,3 +80,31 @@ def test_run_case_reports_input_errors(tmp_path):
     assert L.xbg_run_case(str(pf).encode(), 0) != 0
     assert b"not supported" in L.xbg_run_error()
     assert L.xbg_run_case(str(tmp_path / "missing.txt").encode(), 0) != 0
+
+
+def test_time_series_keys():
+    """TSOfile / TSnode (read_input.cpp:1018-1040): one entry per line, in
+    order; the pairing and clamping happen in the driver's sanity pass."""
+    _, inp = parse("TSOfile = reef.txt", "TSnode = 124,239;", "TSOfile = shore.txt;", "TSnode = 3, 7")
+    assert (inp.ntsofile, inp.ntsnode) == (2, 2)
+    assert inp.tsofile[0].value == b"reef.txt" and inp.tsofile[1].value == b"shore.txt"
+    assert list(inp.tsnode_i[:2]) == [124, 3] and list(inp.tsnode_j[:2]) == [239, 7]
+    L = lib()
+    p = XbgParams()
+    L.xbg_default_params(C.byref(p))
+    inp = Inputs()
+    L.xbg_default_inputs(C.byref(inp))
+    assert L.xbg_param_line(b"TSnode = 5", C.byref(p), C.byref(inp)) != 0
+    assert b"TSnode" in L.xbg_run_error()
+
+
+def test_executable_reports_a_missing_parameter_file(tmp_path):
+    """The xbg_run program (the reference's XBeach_GPU executable) reads
+    XBG_param.txt from its working directory and exits 1 with the reason
+    when it is missing (no GPU is touched before the file is read)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "..", "xbeach_gpu_amd", "xbg_run")
+    assert os.access(exe, os.X_OK)
+    r = subprocess.run([os.path.abspath(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "XBG_param.txt could not be opened" in r.stderr

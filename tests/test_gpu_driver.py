@@ -171,3 +171,83 @@ def test_xbeach_boundary_files_run(tmp_path):
     assert rc == 0, L.xbg_run_error()
     H = read_out(tmp_path / "out.nc", ("H",))["H"]
     assert np.isfinite(H).all() and H[-1, :, 1:10].mean() > 0.2
+
+
+def test_point_time_series_and_log(tmp_path):
+    """TSOfile / TSnode: one "time zs H" line per step up to the last output
+    time (Wave_gpu.cu:764-781, 875-890), the values at output times equal
+    to the netCDF fields at the node; nodes outside the grid are clamped.
+    XBG_log.txt echoes the parameters in XBG_param.txt syntax."""
+    L = lib()
+    pf = write_case(tmp_path, endtime=20.0, out=10.0,
+                    extra="TSOfile = reef.txt;\nTSnode = 20,16;\nTSOfile = corner.txt;\nTSnode = 999,-4;\n")
+    rc = L.xbg_run_case(str(pf).encode(), 0)
+    assert rc == 0, L.xbg_run_error()
+    out = read_out(tmp_path / "out.nc", ("zs", "H"))
+    with scipy.io.netcdf_file(str(tmp_path / "out.nc"), "r", mmap=False) as f:
+        times = f.variables["time"][:].copy()
+    for name, (i, j) in (("reef.txt", (20, 16)), ("corner.txt", (63, 0))):
+        rows = np.loadtxt(tmp_path / name, ndmin=2)
+        assert rows.shape[1] == 3 and len(rows) > 20
+        assert np.all(np.diff(rows[:, 0]) > 0) and abs(rows[-1, 0] - 20.0) < 1e-4
+        for k, t in enumerate(times[1:], start=1):
+            r = rows[np.argmin(np.abs(rows[:, 0] - t))]
+            assert abs(r[0] - t) < 1e-4
+            assert abs(r[1] - out["zs"][k, j, i]) <= 5e-5 + 1e-6 and abs(r[2] - out["H"][k, j, i]) <= 5e-5 + 1e-6
+    log = (tmp_path / "XBG_log.txt").read_text()
+    assert "Starting Computation" in log and "End Computation" in log
+    assert "TSOfile = reef.txt;" in log and "TSnode = 63,0;" in log
+    # the echoed parameters read back into the same values
+    from xbeach_gpu_amd._ffi import XbgInputs, XbgParams
+    p = XbgParams()
+    L.xbg_default_params(C.byref(p))
+    inp = XbgInputs()
+    L.xbg_default_inputs(C.byref(inp))
+    for line in log.splitlines():
+        if " = " in line and not line.startswith(("theta", "dtheta", "grdalpha")):
+            L.xbg_param_line(line.encode(), C.byref(p), C.byref(inp))
+    assert (p.nx, p.ny, p.dx, p.sedtrans, p.morphology) == (64, 32, 5.0, 1, 1)
+    assert p.endtime == 20.0 and p.outputtimestep == 10.0 and inp.wavebndtype == 4
+
+
+def test_generated_blocks_built_ahead_are_deterministic(tmp_path):
+    """Three JONSWAP blocks: block 0 at the start, blocks 1 and 2 built on a
+    host thread while the GPU runs the block before; every block lands in
+    XBG_bnd_reuse.nc in order, and two runs agree bit for bit."""
+    from xbeach_gpu_amd import ncio
+    L = lib()
+    runs = []
+    for name in ("a", "b"):
+        d = tmp_path / name
+        d.mkdir()
+        pf = write_case(d, endtime=30.0, out=10.0)
+        (d / "jons.txt").write_text("0 1.0 8.0 270 10 3.3\n10 1.4 9.0 270 10 3.3\n20 0.7 7.0 270 10 3.3\n")
+        pf.write_text(pf.read_text().replace("rtlength = 120;", "rtlength = 10;"))
+        rc = L.xbg_run_case(str(pf).encode(), 0)
+        assert rc == 0, L.xbg_run_error()
+        runs.append(d)
+    a, b = runs
+    assert (a / "XBG_bnd_reuse.nc").read_bytes() == (b / "XBG_bnd_reuse.nc").read_bytes()
+    ra, rb = read_out(a / "out.nc", ("zs", "H", "ee")), read_out(b / "out.nc", ("zs", "H", "ee"))
+    for k in ra:
+        assert np.array_equal(ra[k], rb[k]), k
+    h = ncio.reuse_head(a / "XBG_bnd_reuse.nc")
+    assert h.nrec == 3
+    with scipy.io.netcdf_file(str(a / "XBG_bnd_reuse.nc"), "r", mmap=False) as f:
+        assert np.array_equal(f.variables["time"][:], [0.0, 10.0, 20.0])
+        assert np.array_equal(f.variables["Hs"][:], [1.0, 1.4, 0.7])
+    E = [ncio.reuse_read(a / "XBG_bnd_reuse.nc", r, 32, 12, 11).Stfile.sum(axis=1).mean() for r in range(3)]
+    assert E[1] > E[0] > E[2] > 0
+
+
+def test_executable_runs_a_case_directory(tmp_path):
+    """xbg_run started in a case directory, as XBeach_GPU is."""
+    import os
+    import subprocess
+    write_case(tmp_path, endtime=10.0, out=10.0)
+    exe = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", "xbeach_gpu_amd", "xbg_run"))
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Writing output" in r.stdout
+    H = read_out(tmp_path / "out.nc", ("H",))["H"]
+    assert H.shape == (2, 32, 64) and np.isfinite(H).all()

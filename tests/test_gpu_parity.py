@@ -247,3 +247,90 @@ def test_bitwise_identical_to_oracle(kw):
     for k in MAIN + MORE:
         assert np.array_equal(d.get(k), o.get(k)), k
     d.close()
+
+
+def test_point_time_series_match_the_fields():
+    """xbg_record_points gathers zs and H at the listed cells after each
+    step (TSOfile / TSnode); the records equal the full fields read back
+    at those cells.  A full buffer is refused until it is read."""
+    from xbeach_gpu_amd import XbgError
+    p, d, o, frc = pair()
+    cells = [0, 17 + 5 * p.nx, p.nx * p.ny - 1, 120 + 30 * p.nx, 17 + 5 * p.nx]
+    d.set_points(cells, capacity=4)
+    want_zs, want_H = [], []
+    t = 150.0
+    for _ in range(4):
+        t = advance(d, o, frc, 1, t=t)
+        d.record_points()
+        want_zs.append(d.get("zs").ravel()[cells])
+        want_H.append(d.get("H").ravel()[cells])
+    with pytest.raises(XbgError):
+        d.record_points()
+    zs, H = d.read_points()
+    assert zs.shape == (4, len(cells))
+    assert np.array_equal(zs, np.array(want_zs)) and np.array_equal(H, np.array(want_H))
+    d.record_points()                          # room again after the read
+    zs, H = d.read_points()
+    assert zs.shape == (1, len(cells)) and np.array_equal(zs[0], d.get("zs").ravel()[cells])
+    zs, H = d.read_points()
+    assert zs.shape == (0, len(cells))
+    with pytest.raises(XbgError):
+        d.set_points([p.nx * p.ny])            # outside the grid
+    d.close()
+
+
+def test_morph_config_size():
+    """The morphology configuration's grid (1024 x 512, dx = 3 m, 24 bins,
+    morfac 10, FP64 bed accumulation): three steps against the oracle."""
+    p = setups.large_params(nx=1024, ny=512, dx=3.0, morfac=10.0)
+    elev = setups.barred_bathy(p.nx, p.ny, p.dx)
+    st = setups.spun_up_state(p, elev, seed=9, Hrms=2.0)
+    frc = setups.GroupForcing(p, 2.8, 11.0, dir_deg=-10.0, s=15.0, seed=3)
+    d = Domain(p)
+    o = Oracle(p, 4)
+    for k, v in st.items():
+        d.set(k, v)
+        o.set(k, v)
+    d.init_state(frc.Trep)
+    o.init_state(frc.Trep)
+    advance(d, o, frc, 3, t=500.0, dt=0.08)
+    check(d, o, 1e-6)
+    assert np.array_equal(d.get("zb"), o.get("zb"))
+    d.close()
+    o.close()
+
+
+def test_ensemble_of_uneven_domains():
+    """Domains of the ensemble's kind (nx in 128..1024, ny in 64..512, dx in
+    2..5 m, 12 or 24 bins, own bathymetry and sea state) advanced side by
+    side in their own contexts with their own CFL time steps: each matches
+    its oracle after three steps (sizes mostly not multiples of the tile)."""
+    rng = np.random.default_rng(20261202)
+    domains = []
+    for n in range(4):
+        nx, ny = int(rng.integers(128, 1025)), int(rng.integers(64, 513))
+        dx = float(rng.uniform(2, 5))
+        ntheta = int(rng.choice([12, 24]))
+        p = setups.small_params(nx=nx, ny=ny, dx=dx, dtheta=120.0 / ntheta, D50=float(rng.uniform(150, 400)) * 1e-6)
+        elev = setups.barred_bathy(nx, ny, dx, slope=1 / float(rng.uniform(30, 80)), bar_h=float(rng.uniform(0, 1.5)))
+        Hm0, Tp = float(rng.uniform(0.5, 4)), float(rng.uniform(6, 16))
+        st = setups.spun_up_state(p, elev, seed=n, Hrms=Hm0 / math.sqrt(2))
+        frc = setups.GroupForcing(p, Hm0, Tp, dir_deg=float(rng.uniform(-30, 30)), s=max(float(rng.uniform(5, 30)), 10.0),
+                                  seed=n)
+        d, o = Domain(p), Oracle(p, 4)
+        for k, v in st.items():
+            d.set(k, v)
+            o.set(k, v)
+        d.init_state(frc.Trep)
+        o.init_state(frc.Trep)
+        domains.append((p, d, o, frc))
+    t = [100.0] * len(domains)
+    for _ in range(3):
+        for n, (p, d, o, frc) in enumerate(domains):
+            dtf, dtw = d.cfl(False)
+            dt = 0.5 * (min(dtf, dtw) if dtw > 0 else dtf)   # each domain its own step
+            t[n] = advance(d, o, frc, 1, t=t[n], dt=dt)
+    for p, d, o, frc in domains:
+        check(d, o, 1e-6)
+        d.close()
+        o.close()

@@ -134,7 +134,7 @@ def test_reader_errors(tmp_path):
     write_grid(good, 2, np.float32, nx=40, ny=30)
     cut = tmp_path / "cut.nc"
     cut.write_bytes(good.read_bytes()[:-400])  # y (120 B) and part of zb
-    with pytest.raises(ncio.NcError, match="short read"):
+    with pytest.raises(ncio.NcError, match="larger than the file|short read"):
         ncio.read_bathy(cut)
     head_only = tmp_path / "head.nc"
     head_only.write_bytes(good.read_bytes()[:60])

@@ -122,11 +122,17 @@ PROTOTYPES = {
     "xbg_bndnc_head": (C.c_int, [C.c_char_p, _P, C.POINTER(C.c_int)]),
     "xbg_bndnc_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _D, _D]),
     "xbg_bndnc_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_double] * 9 + [_D, _D]),
+    "xbg_set_points": (C.c_int, [_CTX, C.POINTER(C.c_int), C.c_int, C.c_int]),
+    "xbg_record_points": (C.c_int, [_CTX]),
+    "xbg_read_points": (C.c_int, [_CTX, _F, _F, C.c_int, C.POINTER(C.c_int)]),
     "xbg_default_inputs": (None, [C.c_void_p]),
     "xbg_param_line": (C.c_int, [C.c_char_p, _P, C.c_void_p]),
     "xbg_run_case": (C.c_int, [C.c_char_p, C.c_int]),
     "xbg_run_error": (C.c_char_p, []),
 }
+
+
+MAX_TS = 64  # XBG_MAX_TS
 
 
 class XbgInputs(C.Structure):
@@ -135,7 +141,9 @@ class XbgInputs(C.Structure):
 
     _fields_ = [("bathy", C.c_char * 512), ("outfile", C.c_char * 512), ("sedthk", C.c_char * 512),
                 ("wavebndfile", C.c_char * 512), ("slbndfile", C.c_char * 512), ("windfile", C.c_char * 512),
-                ("wavebndtype", C.c_int), ("dt", C.c_double), ("outvars", C.c_char * 2048)]
+                ("wavebndtype", C.c_int), ("dt", C.c_double), ("outvars", C.c_char * 2048),
+                ("ntsofile", C.c_int), ("ntsnode", C.c_int), ("tsofile", (C.c_char * 256) * MAX_TS),
+                ("tsnode_i", C.c_int * MAX_TS), ("tsnode_j", C.c_int * MAX_TS)]
 
 
 class XbgWavebnd(C.Structure):

@@ -1636,6 +1636,10 @@ struct xbg_ctx {
     double* Stfile = nullptr;
     double* qfile = nullptr;
     int nrec = 0;
+    // point time series: cell indices and a record buffer [cap][2][npt]
+    int* pt_idx = nullptr;
+    float* pt_buf = nullptr;
+    int npt = 0, pt_cap = 0, pt_count = 0;
     float *Hmean = nullptr, *uumean = nullptr, *vvmean = nullptr, *hhmean = nullptr, *zsmean = nullptr,
           *zsmax = nullptr, *Cmean = nullptr;
     float Trep = 10.0f, tsl = 0.0f, tnext = 1.0f, zsbndi = 0.0f, zsbndn = 0.0f, windu = 0.0f, windv = 0.0f;
@@ -1987,6 +1991,10 @@ int xbg_destroy(xbg_ctx* c) {
         hipEventDestroy(e.a);
         hipEventDestroy(e.b);
     }
+    if (c->Stfile) hipFree(c->Stfile);
+    if (c->qfile) hipFree(c->qfile);
+    if (c->pt_idx) hipFree(c->pt_idx);
+    if (c->pt_buf) hipFree(c->pt_buf);
     if (c->host_red) hipHostFree(c->host_red);
     if (c->host_sum) hipHostFree(c->host_sum);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -2118,6 +2126,75 @@ int xbg_use_bnd_record(xbg_ctx* c, int rec, double timesincelast, double timenex
     c->zsbndn = (float)zsbndn;
     c->windu = (float)windu;
     c->windv = (float)windv;
+    return XBG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// One record of the point time series: zs and H at the listed cells
+// (Wave_gpu.cu:764-781 copies both whole fields to the host for this).
+__global__ __launch_bounds__(64) void k_points(const float* zs, const float* H, const int* idx, int n,
+                                               float* out) {
+    int k = blockIdx.x * 64 + threadIdx.x;
+    if (k < n) {
+        int c = idx[k];
+        out[k] = zs[c];
+        out[n + k] = H[c];
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int xbg_set_points(xbg_ctx* c, const int* cells, int npoints, int capacity) {
+    if (!c || npoints < 0 || (npoints > 0 && (!cells || capacity < 1)))
+        return fail(c, XBG_E_ARG, "xbg_set_points: bad argument");
+    for (int k = 0; k < npoints; k++)
+        if (cells[k] < 0 || (size_t)cells[k] >= c->N)
+            return fail(c, XBG_E_ARG, "xbg_set_points: cell index " + std::to_string(cells[k]) + " outside the grid");
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (c->pt_idx) hipFree(c->pt_idx);
+    if (c->pt_buf) hipFree(c->pt_buf);
+    c->pt_idx = nullptr;
+    c->pt_buf = nullptr;
+    c->npt = c->pt_cap = c->pt_count = 0;
+    if (npoints == 0) return XBG_OK;
+    HIPC(c, hipMalloc((void**)&c->pt_idx, (size_t)npoints * sizeof(int)));
+    HIPC(c, hipMalloc((void**)&c->pt_buf, (size_t)capacity * 2 * npoints * sizeof(float)));
+    HIPC(c, hipMemcpy(c->pt_idx, cells, (size_t)npoints * sizeof(int), hipMemcpyHostToDevice));
+    c->npt = npoints;
+    c->pt_cap = capacity;
+    return XBG_OK;
+}
+
+int xbg_record_points(xbg_ctx* c) {
+    if (!c) return XBG_E_ARG;
+    if (c->npt == 0) return XBG_OK;
+    if (c->pt_count >= c->pt_cap) return fail(c, XBG_E_STATE, "xbg_record_points: buffer full, read it first");
+    float* out = c->pt_buf + (size_t)c->pt_count * 2 * c->npt;
+    LAUNCH(c, "k_points", k_points, dim3((c->npt + 63) / 64), dim3(64), c->f.zs, c->f.H, c->pt_idx, c->npt, out);
+    c->pt_count++;
+    return XBG_OK;
+}
+
+int xbg_read_points(xbg_ctx* c, float* zs, float* H, int cap_records, int* nrec) {
+    if (!c || !nrec) return XBG_E_ARG;
+    const int n = c->pt_count;
+    if (n > 0 && (!zs || !H || cap_records < n))
+        return fail(c, XBG_E_ARG, "xbg_read_points: host buffers hold fewer than " + std::to_string(n) + " records");
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (n > 0) {
+        std::vector<float> tmp((size_t)n * 2 * c->npt);
+        HIPC(c, hipMemcpy(tmp.data(), c->pt_buf, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int r = 0; r < n; r++)
+            for (int k = 0; k < c->npt; k++) {
+                zs[(size_t)r * c->npt + k] = tmp[((size_t)r * 2) * c->npt + k];
+                H[(size_t)r * c->npt + k] = tmp[((size_t)r * 2 + 1) * c->npt + k];
+            }
+    }
+    *nrec = n;
+    c->pt_count = 0;
     return XBG_OK;
 }
 

@@ -28,6 +28,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -349,12 +350,14 @@ void be_doubles(const double* v, size_t n, std::vector<uint8_t>& out) {
 struct NcFile {
     FILE* f = nullptr;
     Nc3 h;
+    uint64_t fsize = 0;
     ~NcFile() {
         if (f) fclose(f);
     }
     int open(const std::string& path, const char* mode) {
         f = fopen(path.c_str(), mode);
         if (!f) return ncfail(path + " could not be opened");
+        if (fseeko(f, 0, SEEK_END) == 0) fsize = (uint64_t)ftello(f);
         std::vector<uint8_t> b;
         size_t want = 4096;
         for (;;) {
@@ -372,6 +375,7 @@ struct NcFile {
     }
     // values [r0, r0 + nr) of a record variable, or all of a fixed one
     int get(const Var& v, uint64_t r0, uint64_t nr, double* out) {
+        if (!fits(v, v.rec ? nr : 1)) return ncfail(v.name + ": larger than the file");
         const uint64_t n = h.slab(v), ts = tsize(v.type);
         if (!v.rec) {
             r0 = 0;
@@ -388,6 +392,20 @@ struct NcFile {
             for (uint64_t i = 0; i < n; i++) out[r * n + i] = be_value(&b[i * ts], v.type);
         }
         return XBG_OK;
+    }
+    // true when nr records (or the whole fixed variable) of v lie inside the
+    // file: checked before anything is sized from header values
+    bool fits(const Var& v, uint64_t nr) const {
+        uint64_t n = 1;
+        for (size_t k = v.rec ? 1 : 0; k < v.dims.size(); k++) {
+            uint64_t len = h.dims[v.dims[k]].len;
+            if (len != 0 && n > (uint64_t(1) << 60) / len) return false;
+            n *= len;
+        }
+        const uint64_t bytes = n * tsize(v.type);
+        if (nr == 0) return true;
+        uint64_t last = v.begin + (v.rec ? (nr - 1) * h.recsize : 0);
+        return bytes <= fsize && last <= fsize && last + bytes <= fsize;
     }
     int put_rec(const Var& v, uint64_t r, const double* vals) {
         std::vector<uint8_t> b;
@@ -419,54 +437,65 @@ const char* xbg_nc_error(void) { return g_ncerr.c_str(); }
 // variable (1-D x or 2-D (y, x)) as |x[0] - x[nx-1]| / (nx - 1), rounded
 // to float as the reference's float temporary does.
 int xbg_nc_grid_size(const char* path, int* nx, int* ny, double* dx) {
-    if (!path || !nx || !ny || !dx) return ncfail("NULL argument");
-    NcFile F;
-    if (int rc = F.open(path, "rb")) return rc;
-    const Var* z = need_var(F.h, "zb", path);
-    if (!z) return XBG_E_ARG;
-    if (z->dims.size() < 2) return ncfail(std::string(path) + ": zb needs (y, x) dimensions");
-    const size_t o = z->dims.size() > 2 ? 1 : 0;
-    const Dim& dy = F.h.dims[z->dims[o]];
-    const Dim& dxd = F.h.dims[z->dims[o + 1]];
-    uint64_t lny = dy.len ? dy.len : F.h.numrecs, lnx = dxd.len ? dxd.len : F.h.numrecs;
-    if (lnx < 2 || lny < 1 || lnx > 0x7fffffff || lny > 0x7fffffff)
-        return ncfail(std::string(path) + ": zb grid size");
-    const Var* xc = need_var(F.h, dxd.name, path);
-    if (!xc) return XBG_E_ARG;
-    std::vector<double> xv(F.h.slab(*xc) * (xc->rec ? F.h.numrecs : 1));
-    if (int rc = F.get(*xc, 0, xc->rec ? F.h.numrecs : 1, xv.data())) return rc;
-    // 1-D: x[i]; 2-D (y, x): the first row
-    if (xv.size() < lnx) return ncfail(std::string(path) + ": x coordinate shorter than the grid");
-    float dxx = (float)(std::fabs(xv[0] - xv[lnx - 1]) / (double)(lnx - 1));
-    *nx = (int)lnx;
-    *ny = (int)lny;
-    *dx = dxx;
-    return XBG_OK;
+    try {
+        if (!path || !nx || !ny || !dx) return ncfail("NULL argument");
+        NcFile F;
+        if (int rc = F.open(path, "rb")) return rc;
+        const Var* z = need_var(F.h, "zb", path);
+        if (!z) return XBG_E_ARG;
+        if (z->dims.size() < 2) return ncfail(std::string(path) + ": zb needs (y, x) dimensions");
+        const size_t o = z->dims.size() > 2 ? 1 : 0;
+        const Dim& dy = F.h.dims[z->dims[o]];
+        const Dim& dxd = F.h.dims[z->dims[o + 1]];
+        uint64_t lny = dy.len ? dy.len : F.h.numrecs, lnx = dxd.len ? dxd.len : F.h.numrecs;
+        if (lnx < 2 || lny < 1 || lnx > 0x7fffffff || lny > 0x7fffffff)
+            return ncfail(std::string(path) + ": zb grid size");
+        const Var* xc = need_var(F.h, dxd.name, path);
+        if (!xc) return XBG_E_ARG;
+        if (!F.fits(*xc, xc->rec ? F.h.numrecs : 1)) return ncfail(std::string(path) + ": x coordinate larger than the file");
+        std::vector<double> xv(F.h.slab(*xc) * (xc->rec ? F.h.numrecs : 1));
+        if (int rc = F.get(*xc, 0, xc->rec ? F.h.numrecs : 1, xv.data())) return rc;
+        // 1-D: x[i]; 2-D (y, x): the first row
+        if (xv.size() < lnx) return ncfail(std::string(path) + ": x coordinate shorter than the grid");
+        float dxx = (float)(std::fabs(xv[0] - xv[lnx - 1]) / (double)(lnx - 1));
+        *nx = (int)lnx;
+        *ny = (int)lny;
+        *dx = dxx;
+        return XBG_OK;
+    } catch (const std::exception& e) {
+        return ncfail(std::string("netCDF: ") + e.what());
+    }
 }
 
 // readnczb (writenetcdf.cpp:743-757): the first nx*ny values of "zb" as
 // float (the first time level of a 3-D variable), row-major (y, x).
 int xbg_nc_read_zb(const char* path, int nx, int ny, float* zb) {
-    if (!path || !zb || nx < 1 || ny < 1) return ncfail("bad argument");
-    NcFile F;
-    if (int rc = F.open(path, "rb")) return rc;
-    const Var* z = need_var(F.h, "zb", path);
-    if (!z) return XBG_E_ARG;
-    const uint64_t n = (uint64_t)nx * ny, per = F.h.slab(*z);
-    std::vector<double> v;
-    if (z->rec) {
-        if (F.h.numrecs == 0 || per == 0) return ncfail(std::string(path) + ": zb holds no records");
-        uint64_t nr = (n + per - 1) / per;
-        if (nr > F.h.numrecs) return ncfail(std::string(path) + ": zb holds fewer than nx*ny values");
-        v.resize(nr * per);
-        if (int rc = F.get(*z, 0, nr, v.data())) return rc;
-    } else {
-        if (per < n) return ncfail(std::string(path) + ": zb holds fewer than nx*ny values");
-        v.resize(per);
-        if (int rc = F.get(*z, 0, 1, v.data())) return rc;
+    try {
+        if (!path || !zb || nx < 1 || ny < 1) return ncfail("bad argument");
+        NcFile F;
+        if (int rc = F.open(path, "rb")) return rc;
+        const Var* z = need_var(F.h, "zb", path);
+        if (!z) return XBG_E_ARG;
+        const uint64_t n = (uint64_t)nx * ny, per = F.h.slab(*z);
+        std::vector<double> v;
+        if (z->rec) {
+            if (F.h.numrecs == 0 || per == 0) return ncfail(std::string(path) + ": zb holds no records");
+            uint64_t nr = (n + per - 1) / per;
+            if (nr > F.h.numrecs) return ncfail(std::string(path) + ": zb holds fewer than nx*ny values");
+            if (!F.fits(*z, nr)) return ncfail(std::string(path) + ": zb larger than the file");
+            v.resize(nr * per);
+            if (int rc = F.get(*z, 0, nr, v.data())) return rc;
+        } else {
+            if (per < n) return ncfail(std::string(path) + ": zb holds fewer than nx*ny values");
+            if (!F.fits(*z, 1)) return ncfail(std::string(path) + ": zb larger than the file");
+            v.resize(per);
+            if (int rc = F.get(*z, 0, 1, v.data())) return rc;
+        }
+        for (uint64_t k = 0; k < n; k++) zb[k] = (float)v[k];
+        return XBG_OK;
+    } catch (const std::exception& e) {
+        return ncfail(std::string("netCDF: ") + e.what());
     }
-    for (uint64_t k = 0; k < n; k++) zb[k] = (float)v[k];
-    return XBG_OK;
 }
 
 // readSWANSPEC (writenetcdf.cpp:1288-1460): a 2-D spectrum in variable var
@@ -508,6 +537,8 @@ int read_spec(const char* spec, Spec2D& out) {
     if (!yc || !xc) return XBG_E_ARG;
     if (yc->rec || xc->rec || F.h.slab(*yc) < ny || F.h.slab(*xc) < nx)
         return ncfail(file + ": coordinate variables of " + var);
+    if (!F.fits(*yc, 1) || !F.fits(*xc, 1) || !F.fits(*v, 1))
+        return ncfail(file + ": " + var + " or its coordinates larger than the file");
     std::vector<double> yv(F.h.slab(*yc)), xv(F.h.slab(*xc));
     out.z.resize(nx * ny);
     if (int rc = F.get(*yc, 0, 1, yv.data())) return rc;
@@ -526,33 +557,41 @@ bool x_is_dir(const Spec2D& sp) {
 }  // namespace
 
 int xbg_nc_spectrum_size(const char* spec, int* nf, int* nd) {
-    if (!nf || !nd) return ncfail("NULL argument");
-    Spec2D sp;
-    if (int rc = read_spec(spec, sp)) return rc;
-    const bool xd = x_is_dir(sp);
-    *nf = (int)(xd ? sp.y.size() : sp.x.size());
-    *nd = (int)(xd ? sp.x.size() : sp.y.size());
-    return XBG_OK;
+    try {
+        if (!nf || !nd) return ncfail("NULL argument");
+        Spec2D sp;
+        if (int rc = read_spec(spec, sp)) return rc;
+        const bool xd = x_is_dir(sp);
+        *nf = (int)(xd ? sp.y.size() : sp.x.size());
+        *nd = (int)(xd ? sp.x.size() : sp.y.size());
+        return XBG_OK;
+    } catch (const std::exception& e) {
+        return ncfail(std::string("netCDF: ") + e.what());
+    }
 }
 
 int xbg_nc_spectrum(const char* spec, double grdalpha, double* freq, double* dir, double* S) {
-    if (!freq || !dir || !S) return ncfail("NULL argument");
-    Spec2D sp;
-    if (int rc = read_spec(spec, sp)) return rc;
-    const size_t nx = sp.x.size(), ny = sp.y.size();
-    const bool xd = x_is_dir(sp);
-    const size_t nf = xd ? ny : nx, nd = xd ? nx : ny;
-    const std::vector<double>& fc = xd ? sp.y : sp.x;
-    const std::vector<double>& dc = xd ? sp.x : sp.y;
-    for (size_t i = 0; i < nf; i++) freq[i] = fc[i];
-    for (size_t d = 0; d < nd; d++) dir[d] = (1.5 * PI_REF - grdalpha) - dc[d] * PI_REF / 180.0;
-    for (size_t d = 0; d < nd; d++)
-        for (size_t f = 0; f < nf; f++) {
-            // z is (y, x): x = freq -> z[f + d*nx]; x = dir -> z[d + f*nx]
-            double z = xd ? sp.z[d + f * nx] : sp.z[f + d * nx];
-            S[f + d * nf] = z * 180.0 / PI_REF;
-        }
-    return XBG_OK;
+    try {
+        if (!freq || !dir || !S) return ncfail("NULL argument");
+        Spec2D sp;
+        if (int rc = read_spec(spec, sp)) return rc;
+        const size_t nx = sp.x.size(), ny = sp.y.size();
+        const bool xd = x_is_dir(sp);
+        const size_t nf = xd ? ny : nx, nd = xd ? nx : ny;
+        const std::vector<double>& fc = xd ? sp.y : sp.x;
+        const std::vector<double>& dc = xd ? sp.x : sp.y;
+        for (size_t i = 0; i < nf; i++) freq[i] = fc[i];
+        for (size_t d = 0; d < nd; d++) dir[d] = (1.5 * PI_REF - grdalpha) - dc[d] * PI_REF / 180.0;
+        for (size_t d = 0; d < nd; d++)
+            for (size_t f = 0; f < nf; f++) {
+                // z is (y, x): x = freq -> z[f + d*nx]; x = dir -> z[d + f*nx]
+                double z = xd ? sp.z[d + f * nx] : sp.z[f + d * nx];
+                S[f + d * nf] = z * 180.0 / PI_REF;
+            }
+        return XBG_OK;
+    } catch (const std::exception& e) {
+        return ncfail(std::string("netCDF: ") + e.what());
+    }
 }
 
 // read_reuse_bndnc_head (writenetcdf.cpp:1090-1152): the direction bins and
@@ -560,28 +599,33 @@ int xbg_nc_spectrum(const char* spec, double grdalpha, double* freq, double* dir
 // stored bin centres and dtheta = (thetamax - thetamin) / ntheta, as the
 // reference sets them; rtlength = tt[ntt-1], dtbc = tt[ntt-1] - tt[ntt-2].
 int xbg_bndnc_head(const char* path, xbg_params* p, int* nrec) {
-    if (!path || !p) return ncfail("NULL argument");
-    NcFile F;
-    if (int rc = F.open(path, "rb")) return rc;
-    int dth = F.h.find_dim("theta"), dtt = F.h.find_dim("tt");
-    if (dth < 0 || dtt < 0) return ncfail(std::string(path) + ": needs dimensions theta and tt");
-    const uint64_t nth = F.h.dims[dth].len, ntt = F.h.dims[dtt].len;
-    if (nth < 1 || ntt < 2) return ncfail(std::string(path) + ": theta or tt too short");
-    const Var* th = need_var(F.h, "theta", path);
-    const Var* tt = need_var(F.h, "tt", path);
-    if (!th || !tt || th->rec || tt->rec) return XBG_E_ARG;
-    std::vector<double> thv(F.h.slab(*th)), ttv(F.h.slab(*tt));
-    if (thv.size() != nth || ttv.size() != ntt) return ncfail(std::string(path) + ": theta/tt variables");
-    if (int rc = F.get(*th, 0, 1, thv.data())) return rc;
-    if (int rc = F.get(*tt, 0, 1, ttv.data())) return rc;
-    p->ntheta = (int)nth;
-    p->thetamin = thv[0];
-    p->thetamax = thv[nth - 1];
-    p->dtheta = (p->thetamax - p->thetamin) / p->ntheta;
-    p->rtlength = ttv[ntt - 1];
-    p->dtbc = ttv[ntt - 1] - ttv[ntt - 2];
-    if (nrec) *nrec = (int)F.h.numrecs;
-    return XBG_OK;
+    try {
+        if (!path || !p) return ncfail("NULL argument");
+        NcFile F;
+        if (int rc = F.open(path, "rb")) return rc;
+        int dth = F.h.find_dim("theta"), dtt = F.h.find_dim("tt");
+        if (dth < 0 || dtt < 0) return ncfail(std::string(path) + ": needs dimensions theta and tt");
+        const uint64_t nth = F.h.dims[dth].len, ntt = F.h.dims[dtt].len;
+        if (nth < 1 || ntt < 2) return ncfail(std::string(path) + ": theta or tt too short");
+        const Var* th = need_var(F.h, "theta", path);
+        const Var* tt = need_var(F.h, "tt", path);
+        if (!th || !tt || th->rec || tt->rec) return XBG_E_ARG;
+        if (!F.fits(*th, 1) || !F.fits(*tt, 1)) return ncfail(std::string(path) + ": theta/tt larger than the file");
+        std::vector<double> thv(F.h.slab(*th)), ttv(F.h.slab(*tt));
+        if (thv.size() != nth || ttv.size() != ntt) return ncfail(std::string(path) + ": theta/tt variables");
+        if (int rc = F.get(*th, 0, 1, thv.data())) return rc;
+        if (int rc = F.get(*tt, 0, 1, ttv.data())) return rc;
+        p->ntheta = (int)nth;
+        p->thetamin = thv[0];
+        p->thetamax = thv[nth - 1];
+        p->dtheta = (p->thetamax - p->thetamin) / p->ntheta;
+        p->rtlength = ttv[ntt - 1];
+        p->dtbc = ttv[ntt - 1] - ttv[ntt - 2];
+        if (nrec) *nrec = (int)F.h.numrecs;
+        return XBG_OK;
+    } catch (const std::exception& e) {
+        return ncfail(std::string("netCDF: ") + e.what());
+    }
 }
 
 // read_reuse_bndnc (writenetcdf.cpp:1190-1286): record rec of a reuse file
@@ -589,38 +633,42 @@ int xbg_bndnc_head(const char* path, xbg_params* p, int* nrec) {
 // qfile[j + k*ny + m*ny*4] (k = 0 qx, 1 qy, 2 and 3 zero).  The file must
 // hold ny rows, ntheta bins and ntt times.
 int xbg_bndnc_read(const char* path, int rec, int ny, int ntheta, int ntt, double* Stfile, double* qfile) {
-    if (!path || !Stfile || !qfile || ny < 1 || ntheta < 1 || ntt < 1 || rec < 0) return ncfail("bad argument");
-    NcFile F;
-    if (int rc = F.open(path, "rb")) return rc;
-    int dth = F.h.find_dim("theta"), dtt = F.h.find_dim("tt"), dyy = F.h.find_dim("yy");
-    if (dth < 0 || dtt < 0 || dyy < 0) return ncfail(std::string(path) + ": needs dimensions theta, yy and tt");
-    if (F.h.dims[dth].len != (uint64_t)ntheta || F.h.dims[dtt].len != (uint64_t)ntt ||
-        F.h.dims[dyy].len != (uint64_t)ny)
-        return ncfail(std::string(path) + ": theta/yy/tt sizes " + std::to_string(F.h.dims[dth].len) + "/" +
-                      std::to_string(F.h.dims[dyy].len) + "/" + std::to_string(F.h.dims[dtt].len) +
-                      " differ from the model's " + std::to_string(ntheta) + "/" + std::to_string(ny) + "/" +
-                      std::to_string(ntt));
-    const Var* ee = need_var(F.h, "ee_bnd", path);
-    const Var* qx = need_var(F.h, "qx_bnd", path);
-    const Var* qy = need_var(F.h, "qy_bnd", path);
-    if (!ee || !qx || !qy) return XBG_E_ARG;
-    const size_t T = ntt, Y = ny, D = ntheta;
-    if (!ee->rec || !qx->rec || !qy->rec || F.h.slab(*ee) != D * Y * T || F.h.slab(*qx) != Y * T ||
-        F.h.slab(*qy) != Y * T)
-        return ncfail(std::string(path) + ": ee_bnd / qx_bnd / qy_bnd shapes");
-    std::vector<double> e(D * Y * T), x(Y * T), y(Y * T);
-    if (int rc = F.get(*ee, rec, 1, e.data())) return rc;
-    if (int rc = F.get(*qx, rec, 1, x.data())) return rc;
-    if (int rc = F.get(*qy, rec, 1, y.data())) return rc;
-    for (size_t m = 0; m < T; m++)
-        for (size_t j = 0; j < Y; j++) {
-            qfile[j + 0 * Y + m * Y * 4] = x[m + j * T];
-            qfile[j + 1 * Y + m * Y * 4] = y[m + j * T];
-            qfile[j + 2 * Y + m * Y * 4] = 0.0;
-            qfile[j + 3 * Y + m * Y * 4] = 0.0;
-            for (size_t t = 0; t < D; t++) Stfile[j + t * Y + m * Y * D] = e[m + j * T + t * Y * T];
-        }
-    return XBG_OK;
+    try {
+        if (!path || !Stfile || !qfile || ny < 1 || ntheta < 1 || ntt < 1 || rec < 0) return ncfail("bad argument");
+        NcFile F;
+        if (int rc = F.open(path, "rb")) return rc;
+        int dth = F.h.find_dim("theta"), dtt = F.h.find_dim("tt"), dyy = F.h.find_dim("yy");
+        if (dth < 0 || dtt < 0 || dyy < 0) return ncfail(std::string(path) + ": needs dimensions theta, yy and tt");
+        if (F.h.dims[dth].len != (uint64_t)ntheta || F.h.dims[dtt].len != (uint64_t)ntt ||
+            F.h.dims[dyy].len != (uint64_t)ny)
+            return ncfail(std::string(path) + ": theta/yy/tt sizes " + std::to_string(F.h.dims[dth].len) + "/" +
+                          std::to_string(F.h.dims[dyy].len) + "/" + std::to_string(F.h.dims[dtt].len) +
+                          " differ from the model's " + std::to_string(ntheta) + "/" + std::to_string(ny) + "/" +
+                          std::to_string(ntt));
+        const Var* ee = need_var(F.h, "ee_bnd", path);
+        const Var* qx = need_var(F.h, "qx_bnd", path);
+        const Var* qy = need_var(F.h, "qy_bnd", path);
+        if (!ee || !qx || !qy) return XBG_E_ARG;
+        const size_t T = ntt, Y = ny, D = ntheta;
+        if (!ee->rec || !qx->rec || !qy->rec || F.h.slab(*ee) != D * Y * T || F.h.slab(*qx) != Y * T ||
+            F.h.slab(*qy) != Y * T)
+            return ncfail(std::string(path) + ": ee_bnd / qx_bnd / qy_bnd shapes");
+        std::vector<double> e(D * Y * T), x(Y * T), y(Y * T);
+        if (int rc = F.get(*ee, rec, 1, e.data())) return rc;
+        if (int rc = F.get(*qx, rec, 1, x.data())) return rc;
+        if (int rc = F.get(*qy, rec, 1, y.data())) return rc;
+        for (size_t m = 0; m < T; m++)
+            for (size_t j = 0; j < Y; j++) {
+                qfile[j + 0 * Y + m * Y * 4] = x[m + j * T];
+                qfile[j + 1 * Y + m * Y * 4] = y[m + j * T];
+                qfile[j + 2 * Y + m * Y * 4] = 0.0;
+                qfile[j + 3 * Y + m * Y * 4] = 0.0;
+                for (size_t t = 0; t < D; t++) Stfile[j + t * Y + m * Y * D] = e[m + j * T + t * Y * T];
+            }
+        return XBG_OK;
+    } catch (const std::exception& e) {
+        return ncfail(std::string("netCDF: ") + e.what());
+    }
 }
 
 // createbndnc (append = 0, a new file) / writebndnc (append = 1, a record
@@ -631,79 +679,83 @@ int xbg_bndnc_read(const char* path, int rec, int ny, int ntheta, int ntt, doubl
 int xbg_bndnc_write(const char* path, int append, int ntt, int ny, int ntheta, double dy, double dtheta,
                     double thetamin, double dtbc, double time, double Hs, double Trep, double Tp, double Dp,
                     const double* Stfile, const double* qfile) {
-    if (!path || !Stfile || !qfile || ntt < 1 || ny < 1 || ntheta < 1) return ncfail("bad argument");
-    const size_t T = ntt, Y = ny, D = ntheta;
-    std::vector<double> e(D * Y * T), x(Y * T), y(Y * T);
-    for (size_t m = 0; m < T; m++)
-        for (size_t j = 0; j < Y; j++) {
-            x[m + j * T] = qfile[j + 0 * Y + m * Y * 4];
-            y[m + j * T] = qfile[j + 1 * Y + m * Y * 4];
-            for (size_t t = 0; t < D; t++) e[m + j * T + t * Y * T] = Stfile[j + t * Y + m * Y * D];
+    try {
+        if (!path || !Stfile || !qfile || ntt < 1 || ny < 1 || ntheta < 1) return ncfail("bad argument");
+        const size_t T = ntt, Y = ny, D = ntheta;
+        std::vector<double> e(D * Y * T), x(Y * T), y(Y * T);
+        for (size_t m = 0; m < T; m++)
+            for (size_t j = 0; j < Y; j++) {
+                x[m + j * T] = qfile[j + 0 * Y + m * Y * 4];
+                y[m + j * T] = qfile[j + 1 * Y + m * Y * 4];
+                for (size_t t = 0; t < D; t++) e[m + j * T + t * Y * T] = Stfile[j + t * Y + m * Y * D];
+            }
+        NcFile F;
+        uint64_t r = 0;
+        if (append) {
+            if (int rc = F.open(path, "r+b")) return rc;
+            int dth = F.h.find_dim("theta"), dtt = F.h.find_dim("tt"), dyy = F.h.find_dim("yy");
+            if (dth < 0 || dtt < 0 || dyy < 0 || F.h.dims[dth].len != D || F.h.dims[dtt].len != T ||
+                F.h.dims[dyy].len != Y)
+                return ncfail(std::string(path) + ": not a reuse file of this block size");
+            r = F.h.numrecs;
+        } else {
+            F.f = fopen(path, "w+b");
+            if (!F.f) return ncfail(std::string(path) + " could not be created");
+            Nc3& h = F.h;
+            h.version = 2;
+            h.dims = {{"tt", T}, {"yy", Y}, {"theta", D}, {"time", 0}};
+            auto var = [](const char* n, std::vector<int> d) {
+                Var v;
+                v.name = n;
+                v.dims = std::move(d);
+                v.type = T_DOUBLE;
+                return v;
+            };
+            h.vars = {var("time", {3}),   var("tt", {0}),   var("yy", {1}),   var("theta", {2}),
+                      var("Trep", {3}),   var("Hs", {3}),   var("Tp", {3}),   var("Dp", {3}),
+                      var("ee_bnd", {3, 2, 1, 0}), var("qx_bnd", {3, 1, 0}), var("qy_bnd", {3, 1, 0})};
+            for (auto& v : h.vars) v.rec = v.dims[0] == 3;
+            h.numrecs = 0;
+            h.layout();
+            std::vector<double> tv(T), yv(Y), thv(D);
+            for (size_t m = 0; m < T; m++) tv[m] = m * dtbc;
+            for (size_t j = 0; j < Y; j++) yv[j] = j * dy;
+            for (size_t i = 0; i < D; i++) thv[i] = i * dtheta + thetamin + 0.5f * dtheta;
+            std::vector<uint8_t> b = h.header();
+            if (fwrite(b.data(), 1, b.size(), F.f) != b.size()) return ncfail(std::string(path) + ": write failed");
+            const double* fixed[] = {tv.data(), yv.data(), thv.data()};
+            for (int k = 0; k < 3; k++) {
+                const Var& v = h.vars[1 + k];
+                be_doubles(fixed[k], h.slab(v), b);
+                if (fseeko(F.f, (off_t)v.begin, SEEK_SET) != 0 || fwrite(b.data(), 1, b.size(), F.f) != b.size())
+                    return ncfail(std::string(path) + ": write failed");
+            }
         }
-    NcFile F;
-    uint64_t r = 0;
-    if (append) {
-        if (int rc = F.open(path, "r+b")) return rc;
-        int dth = F.h.find_dim("theta"), dtt = F.h.find_dim("tt"), dyy = F.h.find_dim("yy");
-        if (dth < 0 || dtt < 0 || dyy < 0 || F.h.dims[dth].len != D || F.h.dims[dtt].len != T ||
-            F.h.dims[dyy].len != Y)
-            return ncfail(std::string(path) + ": not a reuse file of this block size");
-        r = F.h.numrecs;
-    } else {
-        F.f = fopen(path, "w+b");
-        if (!F.f) return ncfail(std::string(path) + " could not be created");
-        Nc3& h = F.h;
-        h.version = 2;
-        h.dims = {{"tt", T}, {"yy", Y}, {"theta", D}, {"time", 0}};
-        auto var = [](const char* n, std::vector<int> d) {
-            Var v;
-            v.name = n;
-            v.dims = std::move(d);
-            v.type = T_DOUBLE;
-            return v;
-        };
-        h.vars = {var("time", {3}),   var("tt", {0}),   var("yy", {1}),   var("theta", {2}),
-                  var("Trep", {3}),   var("Hs", {3}),   var("Tp", {3}),   var("Dp", {3}),
-                  var("ee_bnd", {3, 2, 1, 0}), var("qx_bnd", {3, 1, 0}), var("qy_bnd", {3, 1, 0})};
-        for (auto& v : h.vars) v.rec = v.dims[0] == 3;
-        h.numrecs = 0;
-        h.layout();
-        std::vector<double> tv(T), yv(Y), thv(D);
-        for (size_t m = 0; m < T; m++) tv[m] = m * dtbc;
-        for (size_t j = 0; j < Y; j++) yv[j] = j * dy;
-        for (size_t i = 0; i < D; i++) thv[i] = i * dtheta + thetamin + 0.5f * dtheta;
-        std::vector<uint8_t> b = h.header();
-        if (fwrite(b.data(), 1, b.size(), F.f) != b.size()) return ncfail(std::string(path) + ": write failed");
-        const double* fixed[] = {tv.data(), yv.data(), thv.data()};
+        const Nc3& h = F.h;
+        const char* scal[] = {"time", "Trep", "Hs", "Tp", "Dp"};
+        const double sv[] = {time, Trep, Hs, Tp, Dp};
+        for (int k = 0; k < 5; k++) {
+            const Var* v = need_var(h, scal[k], path);
+            if (!v || !v->rec) return ncfail(std::string(path) + ": variable " + scal[k]);
+            if (int rc = F.put_rec(*v, r, &sv[k])) return rc;
+        }
+        const char* big[] = {"ee_bnd", "qx_bnd", "qy_bnd"};
+        const std::vector<double>* bv[] = {&e, &x, &y};
         for (int k = 0; k < 3; k++) {
-            const Var& v = h.vars[1 + k];
-            be_doubles(fixed[k], h.slab(v), b);
-            if (fseeko(F.f, (off_t)v.begin, SEEK_SET) != 0 || fwrite(b.data(), 1, b.size(), F.f) != b.size())
-                return ncfail(std::string(path) + ": write failed");
+            const Var* v = need_var(h, big[k], path);
+            if (!v || !v->rec || h.slab(*v) != bv[k]->size()) return ncfail(std::string(path) + ": variable " + big[k]);
+            if (int rc = F.put_rec(*v, r, bv[k]->data())) return rc;
         }
+        // the record count last, so that a failed write leaves the file as it was
+        std::vector<uint8_t> nr;
+        put(nr, r + 1, h.nb());
+        if (fseeko(F.f, 4, SEEK_SET) != 0 || fwrite(nr.data(), 1, nr.size(), F.f) != nr.size())
+            return ncfail(std::string(path) + ": write failed");
+        if (fflush(F.f) != 0) return ncfail(std::string(path) + ": write failed");
+        return XBG_OK;
+    } catch (const std::exception& e) {
+        return ncfail(std::string("netCDF: ") + e.what());
     }
-    const Nc3& h = F.h;
-    const char* scal[] = {"time", "Trep", "Hs", "Tp", "Dp"};
-    const double sv[] = {time, Trep, Hs, Tp, Dp};
-    for (int k = 0; k < 5; k++) {
-        const Var* v = need_var(h, scal[k], path);
-        if (!v || !v->rec) return ncfail(std::string(path) + ": variable " + scal[k]);
-        if (int rc = F.put_rec(*v, r, &sv[k])) return rc;
-    }
-    const char* big[] = {"ee_bnd", "qx_bnd", "qy_bnd"};
-    const std::vector<double>* bv[] = {&e, &x, &y};
-    for (int k = 0; k < 3; k++) {
-        const Var* v = need_var(h, big[k], path);
-        if (!v || !v->rec || h.slab(*v) != bv[k]->size()) return ncfail(std::string(path) + ": variable " + big[k]);
-        if (int rc = F.put_rec(*v, r, bv[k]->data())) return rc;
-    }
-    // the record count last, so that a failed write leaves the file as it was
-    std::vector<uint8_t> nr;
-    put(nr, r + 1, h.nb());
-    if (fseeko(F.f, 4, SEEK_SET) != 0 || fwrite(nr.data(), 1, nr.size(), F.f) != nr.size())
-        return ncfail(std::string(path) + ": write failed");
-    if (fflush(F.f) != 0) return ncfail(std::string(path) + ": write failed");
-    return XBG_OK;
 }
 
 }  // extern "C"

@@ -25,10 +25,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <ctime>
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/xbg.h"
@@ -94,6 +96,116 @@ std::string join_dir(const std::string& dir, const std::string& f) {
     return dir + "/" + f;
 }
 
+// XBG_log.txt of the case (write_text_to_log_file, read_input.cpp:1637-1643:
+// one line per message, the file reopened in append mode each time).
+struct Log {
+    std::string path;
+    void line(const std::string& t) const {
+        if (path.empty()) return;
+        FILE* f = fopen(path.c_str(), "a");
+        if (!f) return;
+        fprintf(f, "%s\n", t.c_str());
+        fclose(f);
+    }
+};
+
+std::string d2s(double v) { return std::to_string(v); }
+
+// SaveParamtolog (read_input.cpp:1645-1755): the parameters in the
+// XBG_param.txt syntax (angles in radians as held internally).
+void save_params(const Log& L, const xbg_params& P, const xbg_inputs& IN, const std::vector<std::string>& outvars) {
+    auto kv = [&](const std::string& k, const std::string& v, const char* end = ";") { L.line(k + " = " + v + end); };
+    L.line("#################################");
+    L.line("# Bathymetry file");
+    kv("bathy", IN.bathy);
+    kv("nx", std::to_string(P.nx));
+    kv("ny", std::to_string(P.ny));
+    kv("dx", d2s(P.dx));
+    kv("grdalpha", d2s(P.grdalpha * 180.0 / PI_));
+    L.line("\n");
+    L.line("# Model controls");
+    kv("swave", std::to_string(P.swave));
+    kv("flow", std::to_string(P.flow));
+    kv("sedtrans", std::to_string(P.sedtrans));
+    kv("morphology", std::to_string(P.morphology));
+    kv("gpudevice", "0");
+    L.line("\n");
+    L.line("# Flow parameters");
+    kv("eps", d2s(P.eps));
+    kv("cfsand", d2s(P.cfsand));
+    kv("cfreef", d2s(P.cfreef));
+    kv("usesmago", std::to_string(P.usesmago));
+    if (P.usesmago == 0) kv("nuh", d2s(P.nuh));
+    else kv("smag", d2s(P.smag));
+    kv("nuhfac", d2s(P.nuhfac));
+    kv("lat", d2s(P.lat));
+    kv("fc", d2s(P.fc));
+    kv("Cd", d2s(P.Cd));
+    kv("wci", d2s(P.wci));
+    kv("hwci", d2s(P.hwci));
+    L.line("\n");
+    L.line("# Waves parameters");
+    kv("breakmodel", std::to_string(P.breakmodel));
+    kv("gamma", d2s(P.gammaa));
+    kv("n", d2s(P.n));
+    kv("alpha", d2s(P.alpha));
+    kv("gammax", d2s(P.gammax));
+    kv("beta", d2s(P.beta));
+    kv("fwsand", d2s(P.fwsand));
+    kv("fwreef", d2s(P.fwreef));
+    kv("roller", std::to_string(P.roller));
+    kv("thetamin", d2s(P.thetamin));
+    kv("thetamax", d2s(P.thetamax));
+    kv("dtheta", d2s(P.dtheta), "; ");
+    kv("ntheta", std::to_string(P.ntheta), "; ");
+    L.line("\n");
+    L.line("# Wave boundary parameters");
+    kv("dtbc", d2s(P.dtbc), "; ");
+    kv("rtlength", d2s(P.rtlength), "; ");
+    kv("sprdthr", d2s(P.sprdthr), "; ");
+    kv("random", std::to_string(P.random), "; ");
+    L.line("\n");
+    L.line("# Sediment parameters");
+    kv("D50", d2s(P.D50));
+    kv("D90", d2s(P.D90));
+    kv("rhosed", d2s(P.rhosed));
+    kv("wws", d2s(P.wws));
+    kv("drydzmax", d2s(P.drydzmax));
+    kv("wetdzmax", d2s(P.wetdzmax));
+    kv("maxslpchg", d2s(P.maxslpchg));
+    kv("por", d2s(P.por));
+    kv("morfac", d2s(P.morfac));
+    kv("sus", d2s(P.sus));
+    kv("bed", d2s(P.bed));
+    kv("facsk", d2s(P.facsk));
+    kv("facas", d2s(P.facas));
+    L.line("\n");
+    L.line("# Timekeeping parameters");
+    kv("CFL", d2s(P.CFL));
+    kv("sedstart", d2s(P.sedstart));
+    kv("outputtimestep", d2s(P.outputtimestep));
+    std::string all;
+    for (size_t k = 0; k < outvars.size(); k++) all += (k ? ", " : "") + outvars[k];
+    kv("outvars", all);
+    kv("endtime", d2s(P.endtime));
+    L.line("\n");
+    L.line("# Files");
+    kv("outfile", IN.outfile);
+    kv("SedThkfile", IN.sedthk);
+    kv("wavebndfile", IN.wavebndfile);
+    kv("wavebndtype", std::to_string(IN.wavebndtype));
+    kv("slbndfile", IN.slbndfile);
+    kv("windbndfile", IN.windfile);
+    for (int o = 0; o < IN.ntsofile && o < IN.ntsnode; o++) {
+        kv("TSOfile", IN.tsofile[o]);
+        kv("TSnode", std::to_string(IN.tsnode_i[o]) + "," + std::to_string(IN.tsnode_j[o]));
+    }
+    L.line("\n");
+    L.line("# Others");
+    kv("g", d2s(P.g));
+    kv("rho", d2s(P.rho));
+}
+
 }  // namespace
 
 extern "C" {
@@ -101,7 +213,7 @@ extern "C" {
 // readparamstr (read_input.cpp:595-1225) for one line.  Angles thetamin,
 // thetamax, dtheta arrive in degrees and are stored in radians; grdalpha
 // and lat stay in degrees (converted where the reference converts them).
-int xbg_param_line(const char* cline, xbg_params* p, xbg_inputs* in) {
+static int param_line(const char* cline, xbg_params* p, xbg_inputs* in) {
     if (!cline || !p || !in) return XBG_E_ARG;
     std::string line(cline);
     std::string v;
@@ -211,6 +323,22 @@ int xbg_param_line(const char* cline, xbg_params* p, xbg_inputs* in) {
     num("nmax", p->nmax);
     num("fcutoff", p->fcutoff);
     inum("nspr", p->nspr);
+    // read_input.cpp:1018-1040: one output file per TSOfile line, one node
+    // per TSnode line, paired in order by checkparamsanity
+    v = findparameter("TSOfile", line);
+    if (!v.empty()) {
+        if (in->ntsofile >= XBG_MAX_TS) return fail("more than " + std::to_string(XBG_MAX_TS) + " TSOfile entries");
+        copystr(in->tsofile[in->ntsofile++], sizeof(in->tsofile[0]), v);
+    }
+    v = findparameter("TSnode", line);
+    if (!v.empty()) {
+        if (in->ntsnode >= XBG_MAX_TS) return fail("more than " + std::to_string(XBG_MAX_TS) + " TSnode entries");
+        std::vector<std::string> e = split(v, ',');
+        if (e.size() < 2) return fail("TSnode needs i,j: " + v);
+        in->tsnode_i[in->ntsnode] = std::stoi(e[0]);
+        in->tsnode_j[in->ntsnode] = std::stoi(e[1]);
+        in->ntsnode++;
+    }
     return XBG_OK;
 }
 
@@ -248,11 +376,17 @@ void xbg_default_inputs(xbg_inputs* in) {
 
 const char* xbg_run_error(void) { return g_err.c_str(); }
 
-int xbg_run_case(const char* paramfile, int verbose) {
-    if (!paramfile) return fail("NULL parameter file");
-    std::string pf(paramfile), dir;
-    size_t sl = pf.find_last_of('/');
-    if (sl != std::string::npos) dir = pf.substr(0, sl);
+// The reference converts values with std::stod / std::stoi and ends on a
+// malformed number; at the C boundary that becomes an error status.
+int xbg_param_line(const char* cline, xbg_params* p, xbg_inputs* in) {
+    try {
+        return param_line(cline, p, in);
+    } catch (const std::exception& e) {
+        return fail(std::string("malformed parameter line: ") + (cline ? cline : "") + " (" + e.what() + ")");
+    }
+}
+
+static int run_case(const std::string& pf, const std::string& dir, int verbose, const Log& L) {
     xbg_params P;
     xbg_default_params(&P);
     xbg_inputs IN;
@@ -262,12 +396,14 @@ int xbg_run_case(const char* paramfile, int verbose) {
         if (fs.fail()) return fail(pf + " could not be opened");
         std::string line;
         while (std::getline(fs, line))
-            if (!line.empty() && line[0] != '#') xbg_param_line(line.c_str(), &P, &IN);
+            if (!line.empty() && line[0] != '#' && xbg_param_line(line.c_str(), &P, &IN)) return XBG_E_ARG;
     }
     // ---- bathymetry (Wave_gpu.cu:1740-1780, read_input.cpp:1508-1634)
     std::string bathy = join_dir(dir, IN.bathy);
     if (bathy.empty()) return fail("No bathymetry file specified");
     std::string ext = split(bathy, '.').back();
+    L.line("bathy: " + std::string(IN.bathy));
+    L.line("bathy extension: " + ext);
     std::vector<float> zb;
     if (ext == "md") {
         std::ifstream fs(bathy);
@@ -310,6 +446,9 @@ int xbg_run_case(const char* paramfile, int verbose) {
         return fail("bathymetry format ." + ext + " is not supported (use .md, .nc, .dep or .bot)");
     }
     P.grdalpha = P.grdalpha * PI_ / 180;  // Wave_gpu.cu:1770
+    L.line("nx=" + std::to_string(P.nx) + " ny=" + std::to_string(P.ny) + " dx=" + d2s(P.dx) +
+           " grdalpha=" + d2s(P.grdalpha * 180.0 / PI_));
+    L.line("Opening sea level bnd...");
     // ---- sea level and wind (Wave_gpu.cu:1783-1830)
     std::vector<SL> slb;
     if (IN.slbndfile[0]) {
@@ -327,6 +466,8 @@ int xbg_run_case(const char* paramfile, int verbose) {
     } else {
         slb = {{0.0, 0.0, 0.0}, {P.endtime, 0.0, 0.0}};
     }
+    L.line("done");
+    L.line("Opening wind forcing...");
     std::vector<Wind> wnd;
     if (IN.windfile[0]) {
         std::ifstream fs(join_dir(dir, IN.windfile));
@@ -343,6 +484,8 @@ int xbg_run_case(const char* paramfile, int verbose) {
         double te = std::max(P.endtime, slb.back().time);
         wnd = {{0.0, 0.0, 0.0}, {te, 0.0, 0.0}};
     }
+    L.line("done");
+    L.line("Reading wave forcing...");
     // ---- wave boundary list (Wave_gpu.cu:1833-1880)
     std::vector<xbg_wavebnd> wb;
     std::vector<std::string> wbq, wbe;  // per record: XBeach q/E files (type 2), spectrum file (type 5)
@@ -452,8 +595,23 @@ int xbg_run_case(const char* paramfile, int verbose) {
     if (outvars.empty() && P.outputtimestep > 0)
         outvars = {"zb", "zs", "uu", "vv", "H", "thetamean", "D", "urms", "ueu", "vev", "C",
                    "dzb", "Fx", "Fy", "hh", "Hmean", "uumean", "vvmean", "hhmean", "zsmean", "Cmean"};
+    // time-series outputs paired in order, nodes clamped to the grid
+    // (read_input.cpp:1367-1425)
+    int nts = std::min(IN.ntsofile, IN.ntsnode);
+    if (IN.ntsofile != IN.ntsnode) {
+        L.line("WARNING: the number of timeseries output files is not equal to the number of nodes specified");
+        if (verbose)
+            fprintf(stderr, "WARNING: the number of timeseries output files is not equal to the number of nodes "
+                            "specified\n");
+    }
+    IN.ntsofile = IN.ntsnode = nts;
+    for (int o = 0; o < nts; o++) {
+        IN.tsnode_i[o] = std::min(std::max(IN.tsnode_i[o], 0), P.nx - 1);
+        IN.tsnode_j[o] = std::min(std::max(IN.tsnode_j[o], 0), P.ny - 1);
+    }
     xbg_finalize_params(&P);
     // ---- initial state (Wave_gpu.cu:1988-2056)
+    L.line("Set initial condition...");
     const int nx = P.nx, ny = P.ny, nt = P.ntheta;
     const size_t N = (size_t)nx * ny;
     std::vector<float> zs(N), hh(N), zero(N, 0.0f), stdep(N, 5.0f);
@@ -486,11 +644,18 @@ int xbg_run_case(const char* paramfile, int verbose) {
     // the generated blocks are kept in XBG_bnd_reuse.nc for a later run with
     // wavebndtype 3 (createbndnc / writebndnc, Wavestep.cu:150-199, 560-610)
     const std::string reuse = join_dir(dir, "XBG_bnd_reuse.nc");
-    auto save_block = [&](int append, const xbg_wavebnd& w) -> int {
+    auto save_into = [&](int append, const xbg_wavebnd& w, const std::vector<double>& St,
+                         const std::vector<double>& q, double Tr, std::string& err) -> int {
         if (xbg_bndnc_write(reuse.c_str(), append, nrec_blk, ny, nt, P.dx, P.dtheta, P.thetamin, P.dtbc, w.time, w.Hs,
-                            Trep, w.Tp, w.Dp, Stfile.data(), qfile.data()))
-            return fail(xbg_nc_error());
+                            Tr, w.Tp, w.Dp, St.data(), q.data())) {
+            err = xbg_nc_error();
+            return XBG_E_ARG;
+        }
         return XBG_OK;
+    };
+    auto save_block = [&](int append, const xbg_wavebnd& w) -> int {
+        std::string e;
+        return save_into(append, w, Stfile, qfile, Trep, e) ? fail(e) : XBG_OK;
     };
     // read_reuse_bndnc (writenetcdf.cpp:1190-1286); Trep = 15 as it sets it
     auto read_block = [&](int rec) -> int {
@@ -509,28 +674,79 @@ int xbg_run_case(const char* paramfile, int verbose) {
         return XBG_OK;
     };
     // makjonswap (type 4) or readSWANSPEC (type 5), then GenWGnLBW
-    auto gen_block = [&](int k) -> int {
+    // block k into the given buffers; touches no shared state, so it also
+    // runs on the host thread that builds the next block ahead of time
+    auto gen_into = [&](int k, std::vector<double>& St, std::vector<double>& q, double& Tr, std::string& err) -> int {
         const xbg_wavebnd& w = wb[k];
         int nf, nd;
         std::vector<double> fr, di, S;
         if (IN.wavebndtype == 5) {
-            if (xbg_nc_spectrum_size(wbq[k].c_str(), &nf, &nd)) return fail(xbg_nc_error());
+            if (xbg_nc_spectrum_size(wbq[k].c_str(), &nf, &nd)) {
+                err = xbg_nc_error();
+                return XBG_E_ARG;
+            }
             fr.resize(nf);
             di.resize(nd);
             S.resize((size_t)nf * nd);
-            if (xbg_nc_spectrum(wbq[k].c_str(), P.grdalpha, fr.data(), di.data(), S.data()))
-                return fail(xbg_nc_error());
+            if (xbg_nc_spectrum(wbq[k].c_str(), P.grdalpha, fr.data(), di.data(), S.data())) {
+                err = xbg_nc_error();
+                return XBG_E_ARG;
+            }
         } else {
-            if (xbg_jonswap_size(&w, &nf, &nd)) return fail("jonswap size");
+            if (xbg_jonswap_size(&w, &nf, &nd)) {
+                err = "jonswap size";
+                return XBG_E_ARG;
+            }
             fr.resize(nf);
             di.resize(nd);
             S.resize((size_t)nf * nd);
-            if (xbg_jonswap(&w, fr.data(), di.data(), S.data())) return fail("JONSWAP spectrum: NaN");
+            if (xbg_jonswap(&w, fr.data(), di.data(), S.data())) {
+                err = "JONSWAP spectrum: NaN";
+                return XBG_E_ARG;
+            }
         }
         unsigned seed = 0;
         if (P.random == 1) seed = (unsigned)std::time(nullptr);
-        return xbg_wave_groups(&P, offdepth, nf, nd, fr.data(), di.data(), S.data(), seed, &Trep, Stfile.data(),
-                               qfile.data());
+        if (xbg_wave_groups(&P, offdepth, nf, nd, fr.data(), di.data(), S.data(), seed, &Tr, St.data(), q.data())) {
+            err = "wave-group generation failed";
+            return XBG_E_ARG;
+        }
+        return XBG_OK;
+    };
+    auto gen_block = [&](int k) -> int {
+        std::string e;
+        return gen_into(k, Stfile, qfile, Trep, e) ? fail(e) : XBG_OK;
+    };
+    // Generated boundaries (types 4, 5): block k+1 is built, and appended to
+    // the reuse file, on a host thread while the GPU steps through block k
+    // (the reference generates it at the block change, Wavestep.cu:545-610).
+    struct Ahead {
+        std::thread th;
+        int k = -1, rc = 0;
+        std::string err;
+        std::vector<double> St, q;
+        double Trep = 0.0;
+        ~Ahead() {
+            if (th.joinable()) th.join();
+        }
+    } ahead;
+    auto build_ahead = [&](int k) {
+        if ((IN.wavebndtype != 4 && IN.wavebndtype != 5) || k < 1 || k >= (int)wb.size() - 1) return;
+        if (ahead.th.joinable()) ahead.th.join();
+        ahead.k = k;
+        ahead.rc = 0;
+        ahead.err.clear();
+        ahead.St.assign(Stfile.size(), 0.0);
+        ahead.q.assign(qfile.size(), 0.0);
+        ahead.th = std::thread([&ahead, &gen_into, &save_into, &wb, k]() {
+            try {
+                ahead.rc = gen_into(k, ahead.St, ahead.q, ahead.Trep, ahead.err);
+                if (!ahead.rc) ahead.rc = save_into(1, wb[k], ahead.St, ahead.q, ahead.Trep, ahead.err);
+            } catch (const std::exception& e) {
+                ahead.err = std::string("wave boundary block: ") + e.what();
+                ahead.rc = XBG_E_ARG;
+            }
+        });
     };
     if (P.swave == 1) {
         if (IN.wavebndtype == 1) {
@@ -544,6 +760,8 @@ int xbg_run_case(const char* paramfile, int verbose) {
             if (read_block(0)) return XBG_E_ARG;
         } else if (gen_block(0) || save_block(0, wb[0])) {
             return XBG_E_ARG;
+        } else {
+            build_ahead(1);
         }
     }
     // ---- device state
@@ -565,8 +783,10 @@ int xbg_run_case(const char* paramfile, int verbose) {
     double dtf = 0, dtw = 0, dt = 0;
     if (!rc) rc = chk(xbg_cfl_dt(ctx, 1, &dtf, &dtw), "initial dt");
     dt = dtf * 0.5;  // Wave_gpu.cu:2640
-    if (!rc && dt < 0.00000001) rc = fail("timestep too small");
+    if (!rc) L.line("Initial timestep: dt=" + d2s(dt));
+    if (!rc && dt < 0.00000001) rc = fail("ERROR: timestep too small");
     // ---- output file (Wave_gpu.cu:2940-2959)
+    L.line("prepare output");
     xbg_nc* nc = nullptr;
     std::vector<float> buf;
     auto ndim = [&](const std::string& v) {
@@ -594,6 +814,60 @@ int xbg_run_case(const char* paramfile, int verbose) {
             if (!d || xbg_nc_def_var(nc, v.c_str(), ndim(v), d)) rc = fail("output variable " + v);
         }
     }
+    if (!rc) {
+        L.line("...done");
+        L.line("#################################");
+        L.line("XBeach_GPU Parameters:");
+        save_params(L, P, IN, outvars);
+        L.line("#################################");
+        L.line("Starting Computation ");
+    }
+    // ---- point time series (Wave_gpu.cu:291-310, 764-781, 875-890): the
+    // files start empty; zs and H at the nodes are gathered on the device
+    // every step and appended to the files at each output time
+    const int ts_cap = 1024;
+    std::vector<std::string> ts_path;
+    std::vector<double> ts_time;
+    std::vector<float> ts_zs, ts_H;
+    int ts_ondev = 0;
+    if (!rc && nts > 0) {
+        std::vector<int> cells(nts);
+        for (int o = 0; o < nts; o++) {
+            cells[o] = IN.tsnode_i[o] + IN.tsnode_j[o] * nx;
+            ts_path.push_back(join_dir(dir, IN.tsofile[o]));
+            FILE* f = fopen(ts_path.back().c_str(), "w");
+            if (!f) {
+                rc = fail(ts_path.back() + " could not be created");
+                break;
+            }
+            fclose(f);
+        }
+        if (!rc) rc = chk(xbg_set_points(ctx, cells.data(), nts, ts_cap), "points");
+    }
+    auto ts_drain = [&]() -> int {
+        int n = 0;
+        std::vector<float> z((size_t)ts_cap * nts), h((size_t)ts_cap * nts);
+        if (int r = chk(xbg_read_points(ctx, z.data(), h.data(), ts_cap, &n), "points")) return r;
+        ts_zs.insert(ts_zs.end(), z.begin(), z.begin() + (size_t)n * nts);
+        ts_H.insert(ts_H.end(), h.begin(), h.begin() + (size_t)n * nts);
+        ts_ondev = 0;
+        return XBG_OK;
+    };
+    auto ts_write = [&]() -> int {
+        if (nts == 0) return XBG_OK;
+        if (int r = ts_drain()) return r;
+        for (int o = 0; o < nts; o++) {
+            FILE* f = fopen(ts_path[o].c_str(), "a");
+            if (!f) return fail(ts_path[o] + " could not be opened");
+            for (size_t n = 0; n < ts_time.size(); n++)
+                fprintf(f, "%f\t%.4f\t%.4f\n", ts_time[n], (double)ts_zs[n * nts + o], (double)ts_H[n * nts + o]);
+            fclose(f);
+        }
+        ts_time.clear();
+        ts_zs.clear();
+        ts_H.clear();
+        return XBG_OK;
+    };
     double totaltime = 0.0, nextoutputtime = P.outputtimestep;
     int WAV = 1, SLs = 1, WNs = 1, nstep = 0;
     if (!rc) rc = chk(xbg_reset_means(ctx), "means");
@@ -605,6 +879,7 @@ int xbg_run_case(const char* paramfile, int verbose) {
         rc = xbg_next_dt(dtf, dtw, P.swave == 1, wdt, totaltime, nextoutputtime, P.endtime, &dt);
         if (rc) {
             g_err = "Model crashed!";
+            L.line("Model crashed!. Exiting.");
             break;
         }
         totaltime = totaltime + dt;
@@ -616,9 +891,23 @@ int xbg_run_case(const char* paramfile, int verbose) {
                 WAV++;
                 if (WAV >= (int)wb.size()) WAV = (int)wb.size() - 1;
                 if (IN.wavebndtype >= 2) {
-                    int r = IN.wavebndtype == 2   ? xb_block(WAV - 1)
+                    int r;
+                    const bool ready = ahead.k == WAV - 1;
+                    if (ahead.th.joinable()) ahead.th.join();  // a block built for another index is dropped
+                    ahead.k = -1;
+                    if (ready) {
+                        r = ahead.rc ? fail(ahead.err) : XBG_OK;
+                        if (!r) {
+                            Stfile.swap(ahead.St);
+                            qfile.swap(ahead.q);
+                            Trep = ahead.Trep;
+                        }
+                    } else {
+                        r = IN.wavebndtype == 2   ? xb_block(WAV - 1)
                             : IN.wavebndtype == 3 ? read_block(WAV - 1)
                                                   : (gen_block(WAV - 1) || save_block(1, wb[WAV - 1]));
+                    }
+                    if (!r) build_ahead(WAV);
                     if (r) {
                         rc = XBG_E_ARG;
                         break;
@@ -657,7 +946,14 @@ int xbg_run_case(const char* paramfile, int verbose) {
         }
         if ((rc = chk(xbg_step(ctx, dt, totaltime), "step"))) break;
         if ((rc = chk(xbg_accumulate_means(ctx), "means"))) break;
+        if (nts > 0) {
+            if (ts_ondev == ts_cap && (rc = ts_drain())) break;
+            if ((rc = chk(xbg_record_points(ctx), "points"))) break;
+            ts_ondev++;
+            ts_time.push_back(totaltime);
+        }
         if (nextoutputtime - totaltime <= dt * 0.001 && P.outputtimestep > 0) {
+            L.line("Writing outputs, totaltime: " + d2s(totaltime) + ", Mean dt= " + d2s(P.outputtimestep / nstep));
             nextoutputtime = nextoutputtime + P.outputtimestep;
             if ((rc = chk(xbg_divide_means(ctx, nstep), "means"))) break;
             if (nc) {
@@ -668,6 +964,7 @@ int xbg_run_case(const char* paramfile, int verbose) {
                     if (!d || xbg_nc_write_var_step(nc, v.c_str(), d)) rc = fail("output " + v);
                 }
             }
+            if (!rc) rc = ts_write();
             if (verbose) printf("Writing output, totaltime:%f s, Mean dt=%f\n", totaltime, P.outputtimestep / nstep);
             if (!rc) rc = chk(xbg_reset_means(ctx), "means");
             nstep = 0;
@@ -675,7 +972,42 @@ int xbg_run_case(const char* paramfile, int verbose) {
     }
     if (nc) xbg_nc_close(nc);
     xbg_destroy(ctx);
-    if (verbose && rc) fprintf(stderr, "%s\n", g_err.c_str());
+    return rc;
+}
+
+int xbg_run_case(const char* paramfile, int verbose) {
+    if (!paramfile) return fail("NULL parameter file");
+    g_err.clear();
+    std::string pf(paramfile), dir;
+    size_t sl = pf.find_last_of('/');
+    if (sl != std::string::npos) dir = pf.substr(0, sl);
+    // XBG_log.txt beside the parameter file, started afresh (Wave_gpu.cu:1670-1690)
+    Log L{join_dir(dir, "XBG_log.txt")};
+    if (FILE* f = fopen(L.path.c_str(), "w")) fclose(f);
+    else L.path.clear();
+    std::time_t now = std::time(nullptr);
+    char tbuf[64];
+    std::strftime(tbuf, sizeof(tbuf), "%c", std::localtime(&now));
+    L.line("#################################");
+    L.line("XBeach_GPU (libxbg, HIP)");
+    L.line("#################################");
+    L.line(std::string("model started at ") + tbuf);
+    const std::clock_t c0 = std::clock();
+    int rc;
+    try {
+        rc = run_case(pf, dir, verbose, L);
+    } catch (const std::exception& e) {
+        // a malformed number in an input file (std::stod / std::stoi)
+        rc = fail(std::string("input error: ") + e.what());
+    }
+    if (rc) {
+        L.line("ERROR: " + g_err);
+        if (verbose) fprintf(stderr, "%s\n", g_err.c_str());
+    } else {
+        L.line("End Computation");
+        L.line("#################################");
+        L.line("Total runtime= " + std::to_string((double)(std::clock() - c0) / CLOCKS_PER_SEC) + " seconds");
+    }
     return rc;
 }
 

@@ -154,6 +154,29 @@ class Domain:
     def divide_means(self, nstep: float) -> None:
         self._chk(self.L.xbg_divide_means(self._ctx, float(nstep)))
 
+    def set_points(self, cells, capacity: int = 1024) -> None:
+        """Point time series of zs and H at cells (ix + iy*nx): TSOfile /
+        TSnode of the reference."""
+        c = np.ascontiguousarray(cells, np.int32)
+        self._npt = len(c)
+        self._pt_cap = int(capacity)
+        self._chk(self.L.xbg_set_points(self._ctx, c.ctypes.data_as(C.POINTER(C.c_int)), len(c), int(capacity)))
+
+    def record_points(self) -> None:
+        self._chk(self.L.xbg_record_points(self._ctx))
+
+    def read_points(self) -> tuple[np.ndarray, np.ndarray]:
+        """(zs, H), each (records, points); empties the device buffer."""
+        n = max(getattr(self, "_npt", 0), 1)
+        cap = max(getattr(self, "_pt_cap", 0), 1)
+        zs = np.empty((cap, n), np.float32)
+        H = np.empty((cap, n), np.float32)
+        nrec = C.c_int()
+        fp = C.POINTER(C.c_float)
+        self._chk(self.L.xbg_read_points(self._ctx, zs.ctypes.data_as(fp), H.ctypes.data_as(fp), cap,
+                                         C.byref(nrec)))
+        return zs[:nrec.value, :getattr(self, "_npt", 0)], H[:nrec.value, :getattr(self, "_npt", 0)]
+
     def mass_balance(self) -> tuple[float, float]:
         a, b = C.c_double(), C.c_double()
         self._chk(self.L.xbg_mass_balance(self._ctx, C.byref(a), C.byref(b)))
