@@ -84,7 +84,9 @@ XDEV void dispersion_cell(const KP& p, float sig, float h, float& k, float& c, f
 // wave_CPU.cpp evaluates 1.5*e-0.5*e in double (lines 426-455, 524-551),
 // the theta advection in float (lines 745-786).
 XDEV float upw2d(float e1, float e2) {
-    float a = (float)(1.5 * (double)e1 - 0.5 * (double)e2);
+    // 1.5*e1 and 0.5*e2 are exact in double, so one FMA rounds the same
+    // value as the reference's multiply-multiply-subtract
+    float a = (float)__builtin_fma(1.5, (double)e1, -(0.5 * (double)e2));
     return a < 0.0f ? e1 : a;
 }
 XDEV float upw2f(float e1, float e2) {
