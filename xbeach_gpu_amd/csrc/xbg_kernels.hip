@@ -175,6 +175,7 @@ XDEV float act_rr(const KP& p, const Fld& f, int x, int y, int th) {
 struct Refr {
     float sig, dhdx, dhdy, dudx, dudy, dvdx, dvdy, w, lo, hi;
 };
+template <bool CUR>
 XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
@@ -185,11 +186,15 @@ XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
     float ddy = (float)(unsigned)(yp - ym) * p.dx;
     Refr r;
     r.dhdx = (f.hh[ir] - f.hh[il]) / ddx;
-    r.dudx = (f.uu[ir] - f.uu[il]) / ddx;
-    r.dvdx = (f.vv[ir] - f.vv[il]) / ddx;
     r.dhdy = (f.hh[it] - f.hh[ib]) / ddy;
-    r.dudy = (f.uu[it] - f.uu[ib]) / ddy;
-    r.dvdy = (f.vv[it] - f.vv[ib]) / ddy;
+    if (CUR) {
+        r.dudx = (f.uu[ir] - f.uu[il]) / ddx;
+        r.dvdx = (f.vv[ir] - f.vv[il]) / ddx;
+        r.dudy = (f.uu[it] - f.uu[ib]) / ddy;
+        r.dvdy = (f.vv[it] - f.vv[ib]) / ddy;
+    } else {
+        r.dudx = r.dvdx = r.dudy = r.dvdy = 0.0f;
+    }
     float h = f.hh[i];
     float kh = fmn(f.k[i] * h, 10.0f);
     r.sig = s.sigm / xm::sinhf(2.0f * kh);
@@ -198,9 +203,17 @@ XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
     r.hi = 0.25f * s.sigm;
     return r;
 }
+// CUR = false: wci == 0, where every current term is a product with w = 0
+// (a signed zero added to the rest; it can change only the sign of a zero
+// result, which no later step distinguishes), so it is left out.
+template <bool CUR>
 XDEV float ctheta_of(const KP& p, const float* dirs, const Refr& r, int th) {
     float cs = tab(dirs + MAXT, th), sn = tab(dirs + 2 * MAXT, th);
-    float ctht = r.sig * (r.dhdx * sn - r.dhdy * cs) + r.w * (cs * (sn * r.dudx - cs * r.dudy) + sn * (sn * r.dvdx - cs * r.dvdy));
+    float ctht;
+    if (CUR)
+        ctht = r.sig * (r.dhdx * sn - r.dhdy * cs) + r.w * (cs * (sn * r.dudx - cs * r.dudy) + sn * (sn * r.dvdx - cs * r.dvdy));
+    else
+        ctht = r.sig * (r.dhdx * sn - r.dhdy * cs);
     return fmn(fmx(ctht, r.lo), r.hi);
 }
 
@@ -211,6 +224,7 @@ XDEV float ctheta_of(const KP& p, const float* dirs, const Refr& r, int th) {
 struct AdvC {
     float sc, sxm, sxp, sym, syp, ui, uxm, uxp, vi, vym, vyp;
 };
+template <bool CUR>
 XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
@@ -221,6 +235,10 @@ XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
     a.sxp = S[xp + row];
     a.sym = S[x + (size_t)ym * nx];
     a.syp = S[x + (size_t)yp * nx];
+    if (!CUR) {
+        a.ui = a.uxm = a.uxp = a.vi = a.vym = a.vyp = 0.0f;
+        return a;
+    }
     float w = p.wci * fmn(f.hh[x + row] / p.hwci, 1.0f);
     a.ui = f.uu[x + row] * w;
     a.uxm = f.uu[xm + row] * w;
@@ -233,19 +251,20 @@ XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
 
 // x + y flux divergence of one bin from its 9-point cross stencil
 // (values at the clamped neighbour indices).
+template <bool CUR>
 XDEV float adv_xy(const KP& p, const float* dirs, const AdvC& a, float dxi, int th, float ec, float exm, float exm2, float exp_,
                   float exp2, float eym, float eym2, float eyp, float eyp2) {
     float cs = tab(dirs + MAXT, th), sn = tab(dirs + 2 * MAXT, th);
     // upwind operands are selected first so each face evaluates one
     // 2nd-order value (branch-free code would otherwise evaluate both)
-    float cgx = 0.5f * (a.sc * cs + a.ui + a.sxp * cs + a.uxp);
-    float cgxmin = 0.5f * (a.sc * cs + a.ui + a.sxm * cs + a.uxm);
+    float cgx = CUR ? 0.5f * (a.sc * cs + a.ui + a.sxp * cs + a.uxp) : 0.5f * (a.sc * cs + a.sxp * cs);
+    float cgxmin = CUR ? 0.5f * (a.sc * cs + a.ui + a.sxm * cs + a.uxm) : 0.5f * (a.sc * cs + a.sxm * cs);
     bool px = cgx > 0.0f, pxm = cgxmin > 0.0f;
     float ain = upw2d(px ? ec : exp_, px ? exm : exp2) * cgx;
     float amin = upw2d(pxm ? exm : ec, pxm ? exm2 : exp_) * cgxmin;
     float xa = (ain - amin) * dxi;
-    float cgy = 0.5f * (a.sc * sn + a.vi + a.syp * sn + a.vyp);
-    float cgymin = 0.5f * (a.sc * sn + a.vi + a.sym * sn + a.vym);
+    float cgy = CUR ? 0.5f * (a.sc * sn + a.vi + a.syp * sn + a.vyp) : 0.5f * (a.sc * sn + a.syp * sn);
+    float cgymin = CUR ? 0.5f * (a.sc * sn + a.vi + a.sym * sn + a.vym) : 0.5f * (a.sc * sn + a.sym * sn);
     bool py = cgy > 0.0f, pym = cgymin > 0.0f;
     ain = upw2d(py ? ec : eyp, py ? eym : eyp2) * cgy;
     amin = upw2d(pym ? eym : ec, pym ? eym2 : eyp) * cgymin;
@@ -280,7 +299,7 @@ XDEV float baldock_cell(const KP& p, float Trep, float Ht, float h, float k, flo
 // energyCPU + energintCPU + roelvink/baldock + dissipationCPU
 // (wave_CPU.cpp:873-1089) at cell i: e (action after Euler) -> ee after
 // dissipation, r (roller after Euler) -> rr after dissipation.
-template <int NT>
+template <int NT, bool CUR>
 XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e, float* r, float& Rout,
                       float& DRout) {
     const int nt = NT > 0 ? NT : p.nt;
@@ -318,8 +337,8 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
         if (th >= nt) break;
         float eet = e[th];
-        float cx = cc * tab(f.dirs + MAXT, th) + uui;
-        float cy = cc * tab(f.dirs + 2 * MAXT, th) + vvi;
+        float cx = CUR ? cc * tab(f.dirs + MAXT, th) + uui : cc * tab(f.dirs + MAXT, th);
+        float cy = CUR ? cc * tab(f.dirs + 2 * MAXT, th) + vvi : cc * tab(f.dirs + 2 * MAXT, th);
         float dd = eet * Dv / Emx;
         float rrt;
         if (wetc) {
@@ -494,14 +513,14 @@ XDEV void store_plane(float* pl, const SP& s, const Slots& sl, const float* raw,
 // field on exit (eulerupwindCPU, wave_CPU.cpp:810-828).  With WAVEDT the
 // largest |ctheta| over the bins, in bin order, is returned in *cmax: the
 // theta fluxes evaluate ctheta of every bin once.
-template <int NT, bool IS_EE>
+template <int NT, bool IS_EE, bool CUR>
 XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f, const AdvC& a, const Refr& rf,
                         const Slots& sl, int tid, int cx, int cy, float* ea, float* cmax) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
     const float dxi = 1.0f / p.dx;
     float e0[A];
-    float ctl = ctheta_of(p, f.dirs, rf, 0), fl = 0.0f;
+    float ctl = ctheta_of<CUR>(p, f.dirs, rf, 0), fl = 0.0f;
     float cm = 0.0f;
     if (IS_EE) {
         float v = fabsf(ctl);
@@ -516,7 +535,7 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
             ta = 0.0f;
         } else if (b < nt - 1) {
             int tm = m1(b), tp = b + 1, tp2 = p2(b, nt);
-            float ctr = ctheta_of(p, f.dirs, rf, tp);
+            float ctr = ctheta_of<CUR>(p, f.dirs, rf, tp);
             if (IS_EE) {
                 float v = fabsf(ctr);
                 cm = v > cm ? v : cm;
@@ -548,7 +567,7 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
         const float* q = pl[th & 1];
         float ec = q[c0];
         e0[th] = ec;
-        ea[th] = adv_xy(p, f.dirs, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
+        ea[th] = adv_xy<CUR>(p, f.dirs, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
                         q[c0 + PW], q[c0 + 2 * PW]);
         if (th >= 2) finish(th - 2);
         // materialise this bin's results (otherwise the arithmetic is sunk
@@ -578,7 +597,7 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
 // by k_wave_rows (rollerlatbnd copies from rows 1 and ny-2, whose
 // post-Euler bins this kernel saves).  Also the block minimum of WAVEDT.
 // Four waves per SIMD (<= 128 VGPRs): two workgroups per CU.
-template <int NT>
+template <int NT, bool CUR>
 __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
@@ -593,15 +612,15 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     const bool valid = (x0 + cx < nx) && (y0 + cy < ny);
     const uint32_t i = (uint32_t)x + (uint32_t)y * nx;
     float ea[A];
-    const Refr rf = refr_cell(p, s, f, x, y);
+    const Refr rf = refr_cell<CUR>(p, s, f, x, y);
     const Slots sl = make_slots(p, x0, y0, tid);
     // WAVEDT (Wave_kernel.cu:232-249): min over bins of CFL*dtheta/|ctheta|.
     // Rounded division is monotonic in the divisor, so this equals one
     // division by the largest |ctheta| (NaN speeds ignored as fminf does).
     float cmax;
     {
-        AdvC a = advc_cell(p, f, f.cg, x, y);
-        advect_planes<NT, true>(pl, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
+        AdvC a = advc_cell<CUR>(p, f, f.cg, x, y);
+        advect_planes<NT, true, CUR>(pl, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
     }
     float wmin = fmn(9999999.9f, p.CFL * p.dtheta / cmax);
     const int side = (y == 1) ? 0 : (y == ny - 2 ? 1 : -1);
@@ -620,8 +639,8 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     }
     float ra[A];
     if (p.roller == 1) {
-        AdvC a = advc_cell(p, f, f.c, x, y);
-        advect_planes<NT, false>(pl, p, s, f, a, rf, sl, tid, cx, cy, ra, nullptr);
+        AdvC a = advc_cell<CUR>(p, f, f.c, x, y);
+        advect_planes<NT, false, CUR>(pl, p, s, f, a, rf, sl, tid, cx, cy, ra, nullptr);
         if (valid && side >= 0) {
 #pragma unroll
             for (int th = 0; th < A; th++) {
@@ -642,7 +661,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     }
     if (valid) {
         float Rv, DRv;
-        dissip_cell<NT>(p, s, f, i, ea, ra, Rv, DRv);
+        dissip_cell<NT, CUR>(p, s, f, i, ea, ra, Rv, DRv);
         finish_cell<NT>(p, s, f, i, ea, ra, Rv, DRv);
     } else {
         wmin = 9999999.9f;
@@ -683,7 +702,7 @@ __global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
         efin[th] = f.ee_out[is + (size_t)th * N] * wet;
     }
     float Rv, DRv;
-    dissip_cell<NT>(p, s, f, i, e, r, Rv, DRv);
+    dissip_cell<NT, true>(p, s, f, i, e, r, Rv, DRv);
     finish_cell<NT>(p, s, f, i, efin, r, Rv, DRv);
 }
 
@@ -1840,17 +1859,22 @@ int launch_wave(xbg_ctx* c, const SP& s) {
     LAUNCH(c, "k_dispersion", k_dispersion, g, blk2, p, s, c->f);
     dim3 gr((p.nx + 255) / 256, 2, 1);
     dim3 gw(p.wtiles_x * p.wtiles_y, 1, 1), bw(BX, WTY, 1);
+    // wci == 0 (the reference default): the current-free kernel
+    const bool cur = p.wci != 0.0f;
     switch (p.nt) {
         case 12:
-            LAUNCH(c, "k_wave", k_wave<12>, gw, bw, p, s, c->f);
+            if (cur) LAUNCH(c, "k_wave", (k_wave<12, true>), gw, bw, p, s, c->f);
+            else LAUNCH(c, "k_wave", (k_wave<12, false>), gw, bw, p, s, c->f);
             LAUNCH(c, "k_wave_rows", k_wave_rows<12>, gr, dim3(256), p, s, c->f);
             break;
         case 24:
-            LAUNCH(c, "k_wave", k_wave<24>, gw, bw, p, s, c->f);
+            if (cur) LAUNCH(c, "k_wave", (k_wave<24, true>), gw, bw, p, s, c->f);
+            else LAUNCH(c, "k_wave", (k_wave<24, false>), gw, bw, p, s, c->f);
             LAUNCH(c, "k_wave_rows", k_wave_rows<24>, gr, dim3(256), p, s, c->f);
             break;
         default:
-            LAUNCH(c, "k_wave", k_wave<0>, gw, bw, p, s, c->f);
+            if (cur) LAUNCH(c, "k_wave", (k_wave<0, true>), gw, bw, p, s, c->f);
+            else LAUNCH(c, "k_wave", (k_wave<0, false>), gw, bw, p, s, c->f);
             LAUNCH(c, "k_wave_rows", k_wave_rows<0>, gr, dim3(256), p, s, c->f);
             break;
     }
