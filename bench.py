@@ -44,9 +44,11 @@ def alg_bytes(p, kernel="step"):
     """Algorithmic bytes per cell.  Step: compulsory read+write of the carried
     state once per step, 16*ntheta (ee, rr) + 13 2D fields * 8 B, + 16 B for
     the FP64 bed-level accumulator.  k_wave: ee and rr read and written once
-    (16*ntheta) plus the 2D inputs hh, uu, vv (12 B)."""
+    (16*ntheta), the 2D inputs hh, k, c, cg, stdep (20 B; uu and vv too when
+    wci != 0, 28 B) and the 2D outputs Sxx, Syy, Sxy, thetamean, E, H, R,
+    DR, urms, ust (40 B)."""
     if kernel == "k_wave":
-        return 16 * p.ntheta + 12
+        return 16 * p.ntheta + (28 if p.wci != 0.0 else 20) + 40
     return 16 * p.ntheta + 104 + (16 if p.zb_accum == 2 else 0)
 
 
