@@ -79,6 +79,7 @@ struct Fld {
     float* zbc_b;
     // per-block partial minima of the CFL reductions
     float *red_wave, *red_flow;
+    float* red_part;  // CFL partials, 2 per workgroup of k_cfl
     int* red_bad;
     unsigned* red_acc;  // two halves of {flow min bits, wave min bits, flag}
     double* red_sum;
@@ -320,8 +321,12 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     float fw = fwm_of(p, f.stdep[i]);
     float Dv = p.breakmodel == 1 ? roelvink_cell(p, s.Trep, E, h, kk, fw) : baldock_cell(p, s.Trep, Ht, h, kk, fw);
     float cc = f.c[i];
-    float w = p.wci * fmn(h / p.hwci, 1.0f);
-    float uui = f.uu[i] * w, vvi = f.vv[i] * w;
+    float uui = 0.0f, vvi = 0.0f;
+    if (CUR) {
+        float w = p.wci * fmn(h / p.hwci, 1.0f);
+        uui = f.uu[i] * w;
+        vvi = f.vv[i] * w;
+    }
     float RR = 0.0f, drr = 0.0f;
     bool wetc = h > p.eps;
     float Emx = fmx(E, 0.0000001f);
@@ -681,7 +686,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
 // ee/rr of rows 1 / ny-2 times the cell's wet flag, then the cell's own
 // energy..dissipation, and finally ee after dissipation copied again from
 // rows 1 / ny-2.
-template <int NT>
+template <int NT, bool CUR>
 __global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
@@ -702,7 +707,7 @@ __global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
         efin[th] = f.ee_out[is + (size_t)th * N] * wet;
     }
     float Rv, DRv;
-    dissip_cell<NT, true>(p, s, f, i, e, r, Rv, DRv);
+    dissip_cell<NT, CUR>(p, s, f, i, e, r, Rv, DRv);
     finish_cell<NT>(p, s, f, i, efin, r, Rv, DRv);
 }
 
@@ -1532,15 +1537,33 @@ __global__ __launch_bounds__(256) void k_init_depth(KP p, Fld f) {
 // atomicMin.  Non-finite or zero flow limits raise a flag (the reference's
 // crash check, Wave_gpu.cu:642-647).  acc[0..2] of the active half were
 // reset by the previous call, which also resets the other half.
-__global__ __launch_bounds__(256) void k_cfl(KP p, Fld f, float cfl, int nwave, unsigned* acc, unsigned* other) {
+// FLOWDT (Wave_kernel.cu:161-170) per cell and the minima of FLOWDT and of
+// the wave kernel's tile partials (WAVEDT), one pair per workgroup into
+// part[2*b], part[2*b+1]; the host takes the minimum of the partials (the
+// minimum does not depend on the order).  A non-finite or zero FLOWDT makes
+// the workgroup's flow partial NaN.  No atomics: every workgroup writes its
+// own pair.
+__global__ __launch_bounds__(256) void k_cfl(KP p, Fld f, float cfl, int nwave, float* part) {
     __shared__ float rf[4], rw[4];
+    __shared__ int rb[4];
     const size_t i0 = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
     float v = 3.0e38f, w = 9999999.9f;
     int bad = 0;
-    for (size_t i = i0; i < p.N; i += stride) {
-        float d = cfl * p.dx / (sqrtf(9.81f * f.hh[i]));
-        if (!(d == d) || d == INFINITY || d == 0.0f) bad = 1;
-        else v = fmn(v, d);
+    // four independent loads in flight per thread before the arithmetic
+    for (size_t b = i0; b < p.N; b += 4 * stride) {
+        float h[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            size_t i = b + (size_t)k * stride;
+            h[k] = i < p.N ? f.hh[i] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (b + (size_t)k * stride >= p.N) continue;
+            float d = cfl * p.dx / (sqrtf(9.81f * h[k]));
+            if (!(d == d) || d == INFINITY || d == 0.0f) bad = 1;
+            else v = fmn(v, d);
+        }
     }
     for (size_t j = i0; j < (size_t)nwave; j += stride) w = fmn(w, f.red_wave[j]);
     for (int o = 32; o > 0; o >>= 1) {
@@ -1551,19 +1574,14 @@ __global__ __launch_bounds__(256) void k_cfl(KP p, Fld f, float cfl, int nwave, 
     if ((threadIdx.x & 63) == 0) {
         rf[threadIdx.x >> 6] = v;
         rw[threadIdx.x >> 6] = w;
-        if (bad) atomicOr(&acc[2], 1u);
+        rb[threadIdx.x >> 6] = bad;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         v = fmn(fmn(rf[0], rf[1]), fmn(rf[2], rf[3]));
         w = fmn(fmn(rw[0], rw[1]), fmn(rw[2], rw[3]));
-        atomicMin(&acc[0], __float_as_uint(v));
-        atomicMin(&acc[1], __float_as_uint(w));
-        if (blockIdx.x == 0) {
-            other[0] = 0x7f7fffffu;
-            other[1] = 0x7f7fffffu;
-            other[2] = 0u;
-        }
+        part[2 * blockIdx.x] = (rb[0] | rb[1] | rb[2] | rb[3]) ? __builtin_nanf("") : v;
+        part[2 * blockIdx.x + 1] = w;
     }
 }
 
@@ -1693,10 +1711,12 @@ int fail(xbg_ctx* c, int code, const std::string& msg) {
             return fail(ctx, XBG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+constexpr int CFL_PARTS = 1024;  // workgroups (and partial pairs) of k_cfl
+
 dim3 grid2(const KP& p) { return dim3((p.nx + BX - 1) / BX, (p.ny + BY - 1) / BY, 1); }
 const dim3 blk2(BX, BY, 1);
 
-void prof_begin(xbg_ctx* c, const char* name) {
+void prof_begin(xbg_ctx* c, const char* name, hipStream_t st) {
     if (!c->prof) return;
     if (c->evused == c->evpool.size()) {
         xbg_ctx::Ev e;
@@ -1706,11 +1726,11 @@ void prof_begin(xbg_ctx* c, const char* name) {
         c->evpool.push_back(e);
     }
     c->evpool[c->evused].name = name;
-    hipEventRecord(c->evpool[c->evused].a, c->stream);
+    hipEventRecord(c->evpool[c->evused].a, st);
 }
-void prof_end(xbg_ctx* c) {
+void prof_end(xbg_ctx* c, hipStream_t st) {
     if (!c->prof) return;
-    hipEventRecord(c->evpool[c->evused].b, c->stream);
+    hipEventRecord(c->evpool[c->evused].b, st);
     c->evused++;
 }
 void prof_collect(xbg_ctx* c) {
@@ -1726,11 +1746,12 @@ void prof_collect(xbg_ctx* c) {
     c->evused = 0;
 }
 
-#define LAUNCH(ctx, name, kern, grid, block, ...)                              \
+#define LAUNCH(ctx, name, kern, grid, block, ...) LAUNCH_ON(ctx, (ctx)->stream, name, kern, grid, block, __VA_ARGS__)
+#define LAUNCH_ON(ctx, st, name, kern, grid, block, ...)                       \
     do {                                                                       \
-        prof_begin(ctx, name);                                                 \
-        kern<<<grid, block, 0, (ctx)->stream>>>(__VA_ARGS__);                  \
-        prof_end(ctx);                                                         \
+        prof_begin(ctx, name, st);                                             \
+        kern<<<grid, block, 0, st>>>(__VA_ARGS__);                             \
+        prof_end(ctx, st);                                                     \
         hipError_t e_ = hipGetLastError();                                     \
         if (e_ != hipSuccess)                                                  \
             return fail(ctx, XBG_E_HIP, std::string(name) + ": " + hipGetErrorString(e_)); \
@@ -1865,17 +1886,20 @@ int launch_wave(xbg_ctx* c, const SP& s) {
         case 12:
             if (cur) LAUNCH(c, "k_wave", (k_wave<12, true>), gw, bw, p, s, c->f);
             else LAUNCH(c, "k_wave", (k_wave<12, false>), gw, bw, p, s, c->f);
-            LAUNCH(c, "k_wave_rows", k_wave_rows<12>, gr, dim3(256), p, s, c->f);
+            if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<12, true>), gr, dim3(256), p, s, c->f);
+            else LAUNCH(c, "k_wave_rows", (k_wave_rows<12, false>), gr, dim3(256), p, s, c->f);
             break;
         case 24:
             if (cur) LAUNCH(c, "k_wave", (k_wave<24, true>), gw, bw, p, s, c->f);
             else LAUNCH(c, "k_wave", (k_wave<24, false>), gw, bw, p, s, c->f);
-            LAUNCH(c, "k_wave_rows", k_wave_rows<24>, gr, dim3(256), p, s, c->f);
+            if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<24, true>), gr, dim3(256), p, s, c->f);
+            else LAUNCH(c, "k_wave_rows", (k_wave_rows<24, false>), gr, dim3(256), p, s, c->f);
             break;
         default:
             if (cur) LAUNCH(c, "k_wave", (k_wave<0, true>), gw, bw, p, s, c->f);
             else LAUNCH(c, "k_wave", (k_wave<0, false>), gw, bw, p, s, c->f);
-            LAUNCH(c, "k_wave_rows", k_wave_rows<0>, gr, dim3(256), p, s, c->f);
+            if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<0, true>), gr, dim3(256), p, s, c->f);
+            else LAUNCH(c, "k_wave_rows", (k_wave_rows<0, false>), gr, dim3(256), p, s, c->f);
             break;
     }
     LAUNCH(c, "k_wave_force", k_wave_force, g, blk2, p, s, c->f);
@@ -1970,9 +1994,10 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
     c->nblk_flow = (int)((N + 255) / 256);
     c->nblk_sum = 256;
     A(&f.red_wave, c->nblk_wave); A(&f.red_flow, c->nblk_flow); A(&f.red_bad, 4); A(&f.red_acc, 8);
+    A(&f.red_part, 2 * CFL_PARTS);
     A(&f.red_sum, 2 * c->nblk_sum);
 #undef A
-    e = hipHostMalloc((void**)&c->host_red, 4 * sizeof(float), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->host_red, 2 * CFL_PARTS * sizeof(float), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_sum, 2 * c->nblk_sum * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
         g_create_error = std::string("hipHostMalloc: ") + hipGetErrorString(e);
@@ -2260,19 +2285,22 @@ int xbg_step(xbg_ctx* c, double dt, double totaltime) {
 int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
     if (!c) return XBG_E_ARG;
     float cfl = initial ? 0.5f * c->kp.CFL : c->kp.CFL;
-    unsigned* acc = c->f.red_acc + 4 * c->acc_cur;
-    unsigned* other = c->f.red_acc + 4 * (1 - c->acc_cur);
-    int nb = c->nblk_flow < 512 ? c->nblk_flow : 512;
-    LAUNCH(c, "k_cfl", k_cfl, dim3(nb), dim3(256), c->kp, c->f, cfl, c->nblk_wave, acc, other);
-    HIPC(c, hipMemcpyAsync(c->host_red, acc, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    // about 8 cells per thread, at most CFL_PARTS workgroups
+    size_t want = (c->N + 2047) / 2048;
+    int nb = (int)(want < (size_t)CFL_PARTS ? (want > 0 ? want : 1) : (size_t)CFL_PARTS);
+    LAUNCH(c, "k_cfl", k_cfl, dim3(nb), dim3(256), c->kp, c->f, cfl, c->nblk_wave, c->f.red_part);
+    HIPC(c, hipMemcpyAsync(c->host_red, c->f.red_part, 2 * (size_t)nb * sizeof(float), hipMemcpyDeviceToHost,
+                           c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    c->acc_cur = 1 - c->acc_cur;
-    unsigned* hb = (unsigned*)c->host_red;
-    float fmv, wmv;
-    std::memcpy(&fmv, &hb[0], 4);
-    std::memcpy(&wmv, &hb[1], 4);
-    double fm = (double)fmv;
-    if (hb[2] != 0u) fm = NAN;
+    float fmv = 3.0e38f, wmv = 9999999.9f;
+    bool bad = false;
+    for (int b = 0; b < nb; b++) {
+        float v = c->host_red[2 * b], w = c->host_red[2 * b + 1];
+        if (!(v == v)) bad = true;
+        else fmv = v < fmv ? v : fmv;
+        wmv = w < wmv ? w : wmv;
+    }
+    double fm = bad ? NAN : (double)fmv;
     if (dt_flow) *dt_flow = fm;
     if (dt_wave) *dt_wave = (double)wmv;
     if (!(fm == fm) || fm == 0.0 || std::isinf(fm))
