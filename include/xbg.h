@@ -201,7 +201,8 @@ int xbg_step(xbg_ctx* ctx, double dt, double totaltime);
  * Wave_kernel.cu:161-249): dt_flow = min over cells of CFL*dx/sqrt(9.81 hh),
  * dt_wave = min over cells and bins of CFL*dtheta/|ctheta| of the last wave
  * step.  Both are the true global minima (not halved); reduced on the
- * device, 2 doubles cross PCIe.  The "initial" flag uses 0.5*CFL as the
+ * device to one pair per workgroup (at most 8 KB cross PCIe) and finished
+ * on the host.  The "initial" flag uses 0.5*CFL as the
  * reference's first step does (Wave_gpu.cu:2639).
  */
 int xbg_cfl_dt(xbg_ctx* ctx, int initial, double* dt_flow, double* dt_wave);
