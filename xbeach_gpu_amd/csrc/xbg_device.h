@@ -41,6 +41,7 @@ struct KP {
     float sed_drho, sed_dester, sed_Ass, sed_ucrfac, rvr_alph, rvr_bet;
     int usesmago, breakmodel, roller, zb_accum, swave;
     int wtiles_x, wtiles_y;  // wave-kernel tile grid
+    int fold_rows;           // rows 0 / ny-1 finished inside the wave kernel
     float theta[MAXT], cxsth[MAXT], sxnth[MAXT];
 };
 

@@ -628,7 +628,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
         advect_planes<NT, true, CUR>(pl, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
     }
     float wmin = fmn(9999999.9f, p.CFL * p.dtheta / cmax);
-    const int side = (y == 1) ? 0 : (y == ny - 2 ? 1 : -1);
+    const int side = p.fold_rows && NT > 0 ? -1 : ((y == 1) ? 0 : (y == ny - 2 ? 1 : -1));
     if (valid && side >= 0) {
 #pragma unroll
         for (int th = 0; th < A; th++) {
@@ -660,13 +660,56 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
             ra[th] = f.rr_in[i + (uint32_t)th * p.N];
         }
     }
+    // Rows 0 and ny-1 (rollerlatbnd, wave_CPU.cpp:830-871): the post-Euler
+    // ee / rr of rows 1 / ny-2 times the cell's wet flag, the cell's own
+    // dissipation, then ee after dissipation copied again from rows 1 /
+    // ny-2.  In the tiles that hold those rows (a workgroup-uniform
+    // condition: the barriers below are reached by every thread or none)
+    // the source row's values come through LDS; otherwise k_wave_rows
+    // finishes the rows after this kernel.
+    const bool ftile = NT > 0 && p.fold_rows && (y0 == 0 || (ny - 1 >= y0 && ny - 1 < y0 + WTY));
+    const int bsrc = !ftile || !valid ? 0 : (y == 0 ? BX : (y == ny - 1 ? -BX : 0));
+    const bool bsink = valid && ((y0 == 0 && cy == 1) || y == ny - 2);
+    float (*rx)[BX] = reinterpret_cast<float (*)[BX]>(&pl[0][0]);  // A x BX, planes no longer needed
+    float wetb = 1.0f;
+    if (ftile) {
+        if (p.roller == 1) {
+            lds_barrier();  // the last plane reads of the rr pass are done
+            if (bsink) {
+#pragma unroll
+                for (int th = 0; th < A; th++) rx[th][cx] = ra[th];
+            }
+            lds_barrier();
+        }
+        if (bsrc != 0) wetb = f.hh[i] > p.eps ? 1.0f : 0.0f;
+    }
     if (NT > 0) {
 #pragma unroll
-        for (int th = 0; th < A; th++) ea[th] = stash[th][tid];
+        for (int th = 0; th < A; th++) ea[th] = stash[th][tid + bsrc];
+        if (bsrc != 0) {
+#pragma unroll
+            for (int th = 0; th < A; th++) ea[th] = ea[th] * wetb;
+            if (p.roller == 1) {
+#pragma unroll
+                for (int th = 0; th < A; th++) ra[th] = rx[th][cx] * wetb;
+            }
+        }
+    }
+    float Rv = 0.0f, DRv = 0.0f;
+    if (valid) dissip_cell<NT, CUR>(p, s, f, i, ea, ra, Rv, DRv);
+    if (ftile) {
+        lds_barrier();  // every thread has read its row of the stash
+        if (bsink) {
+#pragma unroll
+            for (int th = 0; th < A; th++) stash[th][tid] = ea[th];
+        }
+        lds_barrier();
+        if (bsrc != 0) {
+#pragma unroll
+            for (int th = 0; th < A; th++) ea[th] = stash[th][tid + bsrc] * wetb;
+        }
     }
     if (valid) {
-        float Rv, DRv;
-        dissip_cell<NT, CUR>(p, s, f, i, ea, ra, Rv, DRv);
         finish_cell<NT>(p, s, f, i, ea, ra, Rv, DRv);
     } else {
         wmin = 9999999.9f;
@@ -1799,6 +1842,10 @@ void make_kp(xbg_ctx* c) {
     k.rvr_bet = xm::expf(-0.6065f / 0.3539f);
     k.wtiles_x = (P.nx + BX - 1) / BX;
     k.wtiles_y = (P.ny + WTY - 1) / WTY;
+    // rows 0 and ny-1 can be finished by the wave kernel's own tiles when
+    // rows 1 and ny-2 lie in the same tile as them and the two boundary
+    // rows lie in different tiles
+    k.fold_rows = P.ny > WTY && (P.ny - 1) % WTY != 0 ? 1 : 0;
     k.usesmago = P.usesmago; k.breakmodel = P.breakmodel; k.roller = P.roller;
     k.zb_accum = P.zb_accum; k.swave = P.swave;
     xbg_theta_tables(&P, k.theta, k.cxsth, k.sxnth);
@@ -1886,14 +1933,18 @@ int launch_wave(xbg_ctx* c, const SP& s) {
         case 12:
             if (cur) LAUNCH(c, "k_wave", (k_wave<12, true>), gw, bw, p, s, c->f);
             else LAUNCH(c, "k_wave", (k_wave<12, false>), gw, bw, p, s, c->f);
-            if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<12, true>), gr, dim3(256), p, s, c->f);
-            else LAUNCH(c, "k_wave_rows", (k_wave_rows<12, false>), gr, dim3(256), p, s, c->f);
+            if (!p.fold_rows) {
+                if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<12, true>), gr, dim3(256), p, s, c->f);
+                else LAUNCH(c, "k_wave_rows", (k_wave_rows<12, false>), gr, dim3(256), p, s, c->f);
+            }
             break;
         case 24:
             if (cur) LAUNCH(c, "k_wave", (k_wave<24, true>), gw, bw, p, s, c->f);
             else LAUNCH(c, "k_wave", (k_wave<24, false>), gw, bw, p, s, c->f);
-            if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<24, true>), gr, dim3(256), p, s, c->f);
-            else LAUNCH(c, "k_wave_rows", (k_wave_rows<24, false>), gr, dim3(256), p, s, c->f);
+            if (!p.fold_rows) {
+                if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<24, true>), gr, dim3(256), p, s, c->f);
+                else LAUNCH(c, "k_wave_rows", (k_wave_rows<24, false>), gr, dim3(256), p, s, c->f);
+            }
             break;
         default:
             if (cur) LAUNCH(c, "k_wave", (k_wave<0, true>), gw, bw, p, s, c->f);
