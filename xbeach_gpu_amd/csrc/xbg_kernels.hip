@@ -106,28 +106,30 @@ XDEV float tab(const float* t, int th) {
 // The alongshore mean of uu[0,:] is taken over the pre-update column and
 // summed in row order by one lane, as a sequential sweep would.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_bnd_offshore(KP p, SP s, Fld f, int waves) {
+// ubnd1D / ubnd1Dnowaves (Flow_kernel.cu:167-242, 332) for the offshore
+// column, by one workgroup of nthr threads (tid = flat thread index).
+XDEV void offshore_wg(const KP& p, const SP& s, const Fld& f, int waves, int tid, int nthr) {
     __shared__ float col[1024];
     __shared__ float avg;
     const int nx = p.nx, ny = p.ny;
     float acc = 0.0f;
     for (int base = 0; base < ny; base += 1024) {
         int cnt = min(1024, ny - base);
-        for (int t = threadIdx.x; t < cnt; t += blockDim.x) col[t] = f.uu[(size_t)(base + t) * nx];
+        for (int t = tid; t < cnt; t += nthr) col[t] = f.uu[(size_t)(base + t) * nx];
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             // sequential row-order sum; LDS reads unrolled ahead of the adds
 #pragma unroll 16
             for (int t = 0; t < cnt; t++) acc = acc + col[t];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) avg = acc / (float)ny;
+    if (tid == 0) avg = acc / (float)ny;
     __syncthreads();
     const float uuavg = avg;
     const float factime = waves ? 1.0f / 4.0f / s.Trep * s.dt : 1.0f / 60.0f * s.dt;
     const float taper = fmn(s.totaltime / 100.0f, 1.0f);
-    for (int iy = threadIdx.x; iy < ny; iy += blockDim.x) {
+    for (int iy = tid; iy < ny; iy += nthr) {
         size_t i = (size_t)iy * nx;
         size_t ip = i + p1(0, nx);
         float zsbnd = s.zsbndi + (s.zsbndn - s.zsbndi) * ((float)iy / (float)ny);
@@ -147,11 +149,28 @@ __global__ __launch_bounds__(1024) void k_bnd_offshore(KP p, SP s, Fld f, int wa
     }
 }
 
+__global__ __launch_bounds__(1024) void k_bnd_offshore(KP p, SP s, Fld f, int waves) {
+    offshore_wg(p, s, f, waves, threadIdx.x, blockDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Wave action balance
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
+// dispersionCPU per cell.  With `offshore` set, the first row of
+// workgroups is shifted by one and its first workgroup runs the offshore
+// boundary (ubnd1D) instead: it is dispatched first and runs beside the
+// dispersion workgroups, whose inputs (hh) and outputs (k, c, cg) it does
+// not touch, and it completes before the wave kernel reads uu / vv.
+__global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offshore) {
+    int by = blockIdx.y;
+    if (offshore) {
+        if (by == 0) {
+            if (blockIdx.x == 0) offshore_wg(p, s, f, 1, threadIdx.x + threadIdx.y * BX, BX * BY);
+            return;
+        }
+        by -= 1;
+    }
+    int x = blockIdx.x * BX + threadIdx.x, y = by * BY + threadIdx.y;
     if (x >= p.nx || y >= p.ny) return;
     size_t i = (size_t)x + (size_t)y * p.nx;
     float k, c, cg;
@@ -1921,10 +1940,11 @@ const char kFieldNames[] =
     "vvold\0zsold\0dzsdt\0rolthick\0dzb\0k\0c\0cg\0H\0E\0DR\0R\0urms\0ust\0thetamean\0Sxx\0Sxy\0Syy\0"
     "Fx\0Fy\0nuh\0Sus\0Svs\0ceqsg\0umeanbnd\0Hmean\0uumean\0vvmean\0hhmean\0zsmean\0zsmax\0Cmean\0";
 
-int launch_wave(xbg_ctx* c, const SP& s) {
+int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
-    LAUNCH(c, "k_dispersion", k_dispersion, g, blk2, p, s, c->f);
+    dim3 gd(g.x, g.y + (offshore ? 1 : 0), 1);
+    LAUNCH(c, "k_dispersion", k_dispersion, gd, blk2, p, s, c->f, offshore);
     dim3 gr((p.nx + 255) / 256, 2, 1);
     dim3 gw(p.wtiles_x * p.wtiles_y, 1, 1), bw(BX, WTY, 1);
     // wci == 0 (the reference default): the current-free kernel
@@ -2324,10 +2344,12 @@ int xbg_step(xbg_ctx* c, double dt, double totaltime) {
     if (!c) return XBG_E_ARG;
     SP s = make_sp(c, dt, totaltime);
     int rc;
-    if (c->P.flow == 1) {
-        LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), c->kp, s, c->f, c->P.swave == 1 ? 1 : 0);
+    // with waves, the offshore boundary (ubnd1D) rides in the dispersion
+    // launch; without, it is the step's first kernel (ubnd1Dnowaves)
+    if (c->P.flow == 1 && c->P.swave != 1) {
+        LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), c->kp, s, c->f, 0);
     }
-    if (c->P.swave == 1 && (rc = launch_wave(c, s)) != 0) return rc;
+    if (c->P.swave == 1 && (rc = launch_wave(c, s, c->P.flow == 1 ? 1 : 0)) != 0) return rc;
     if (c->P.flow == 1 && (rc = launch_flow(c, s)) != 0) return rc;
     if (c->P.sedtrans == 1 && totaltime >= c->P.sedstart && (rc = launch_sed(c, s)) != 0) return rc;
     return XBG_OK;
