@@ -79,6 +79,7 @@ struct Fld {
     float* zbc_b;
     // per-block partial minima of the CFL reductions
     float *red_wave, *red_flow;
+    float* ua;        // Rvr's ua of each cell, formed by k_flow_uvvu for k_sed_flux
     float* red_part;  // CFL partials, 2 per workgroup of k_cfl
     int* red_bad;
     unsigned* red_acc;  // two halves of {flow min bits, wave min bits, flag}
@@ -1274,9 +1275,13 @@ XDEV void calcuvvu_at(const KP& p, const Fld& f, int x, int y, float* o) {
     o[5] = sqrtf(uev * uev + vev * vev);
 }
 
+XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i);
+
 // calcuvvu + uvlatbnd (Flow_kernel.cu:1712-1775): rows 0 / ny-1 copy rows
-// 1 / ny-2.  Also saves uuold/vvold/zsold.
-__global__ __launch_bounds__(256) void k_flow_uvvu(KP p, Fld f) {
+// 1 / ny-2.  Also saves uuold/vvold/zsold.  With want_ua, Rvr's ua of the
+// cell for the sediment step that follows (its inputs c, H, hh, urms are
+// final here): once per cell instead of per cell and halo in k_sed_flux.
+__global__ __launch_bounds__(256) void k_flow_uvvu(KP p, SP s, Fld f, int want_ua) {
     int x = gx(), y = gy();
     if (x >= p.nx || y >= p.ny) return;
     size_t i = ix2(p, x, y);
@@ -1292,6 +1297,7 @@ __global__ __launch_bounds__(256) void k_flow_uvvu(KP p, Fld f) {
     f.vev[i] = o[3];
     f.vmageu[i] = o[4];
     f.vmagev[i] = o[5];
+    if (want_ua) f.ua[i] = rvr_at(p, s, f, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1317,15 +1323,16 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
 // live output), Rvr and Susp (534-717).  ceqbg = (1-bulk)*... = 0 with
 // bulk = 1 (Sediment_kernel.cu:196, 320), so the bed-load fluxes Sub/Svb
 // are zero and only Sus/Svs are formed.
-__global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f) {
-    // Rvr's ua at the cell and at its +x / +y neighbours: evaluated once per
-    // cell into an LDS tile (+1 column, +1 row) at clamped coordinates, so
-    // tile(cx+1, cy) is ua at pplus(x) and tile(cx, cy+1) at pplus(y)
+__global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f, int ua_ready) {
+    // Rvr's ua at the cell and at its +x / +y neighbours: read from f.ua when
+    // k_flow_uvvu formed it (ua_ready), else evaluated once per cell into an
+    // LDS tile (+1 column, +1 row) at clamped coordinates, so tile(cx+1, cy)
+    // is ua at pplus(x) and tile(cx, cy+1) at pplus(y)
     __shared__ float uat_[BY + 1][BX + 1];
     const int nx = p.nx, ny = p.ny;
     const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
     const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
-    {
+    if (!ua_ready) {
         int xc = min(x0 + cx, nx - 1), yc = min(y0 + cy, ny - 1);
         uat_[cy][cx] = rvr_at(p, s, f, ix2(p, xc, yc));
         if (tid < BX + BY) {
@@ -1339,14 +1346,23 @@ __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f) {
             hy = min(y0 + ty, ny - 1);
             uat_[ty][tx] = rvr_at(p, s, f, ix2(p, hx, hy));
         }
+        __syncthreads();
     }
-    __syncthreads();
     int x = x0 + cx, y = y0 + cy;
     if (x >= nx || y >= ny) return;
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, nx), y), t = ix2(p, x, p1(y, ny));
     float rol = f.rolthick[i] + s.dt * (fabsf(f.dzsdt[i]) - p.beta * f.c[i]);
     f.rolthick[i] = fmx(rol, 0.0f);
-    float uai = uat_[cy][cx], uar = uat_[cy][cx + 1], uat = uat_[cy + 1][cx];
+    float uai, uar, uat;
+    if (ua_ready) {
+        uai = f.ua[i];
+        uar = f.ua[r];
+        uat = f.ua[t];
+    } else {
+        uai = uat_[cy][cx];
+        uar = uat_[cy][cx + 1];
+        uat = uat_[cy + 1][cx];
+    }
     float cci = f.C[i], ccr = f.C[r], cct = f.C[t];
     float tm = f.thetamean[i];
     float ctm = xm::cosf(tm), stm = xm::sinf(tm);
@@ -1979,7 +1995,8 @@ int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0) {
     return 0;
 }
 
-int launch_flow(xbg_ctx* c, const SP& s) {
+// want_ua: the sediment step follows, so k_flow_uvvu forms its ua field
+int launch_flow(xbg_ctx* c, const SP& s, int want_ua = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
     LAUNCH(c, "k_flow_nuh", k_flow_nuh, g, blk2, p, c->f);
@@ -1989,15 +2006,16 @@ int launch_flow(xbg_ctx* c, const SP& s) {
     LAUNCH(c, "k_flow_zsall", k_flow_zsall, g, blk2, p, s, c->f);
     c->zcur = 1 - c->zcur;
     sync_ptrs(c);
-    LAUNCH(c, "k_flow_uvvu", k_flow_uvvu, g, blk2, p, c->f);
+    LAUNCH(c, "k_flow_uvvu", k_flow_uvvu, g, blk2, p, s, c->f, want_ua);
     return 0;
 }
 
-int launch_sed(xbg_ctx* c, const SP& s) {
+// ua_ready: launch_flow(.., want_ua = 1) ran just before with the same s
+int launch_sed(xbg_ctx* c, const SP& s, int ua_ready = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
     int morph = p.morfac > 0.0f ? 1 : 0;
-    LAUNCH(c, "k_sed_flux", k_sed_flux, g, blk2, p, s, c->f);
+    LAUNCH(c, "k_sed_flux", k_sed_flux, g, blk2, p, s, c->f, ua_ready);
     LAUNCH(c, "k_sed_conc", k_sed_conc, g, blk2, p, s, c->f, morph);
     c->ccur = 1 - c->ccur;
     sync_ptrs(c);
@@ -2054,7 +2072,7 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
                      &f.rolthick, &f.dzb, &f.k, &f.c, &f.cg, &f.H, &f.E, &f.Dw, &f.DR, &f.R, &f.urms,
                      &f.ust_raw, &f.ust, &f.thetamean, &f.Sxx, &f.Sxy, &f.Syy, &f.Fx, &f.Fy, &f.nuh,
                      &f.uu_b, &f.vv_b, &f.Sus,
-                     &f.Svs, &f.hh_b, &f.zb_b, &f.stdep_b, &f.dzbx, &f.dzby, &f.ceqsg, &f.zbc, &f.zbc_b,
+                     &f.Svs, &f.ua, &f.hh_b, &f.zb_b, &f.stdep_b, &f.dzbx, &f.dzby, &f.ceqsg, &f.zbc, &f.zbc_b,
                      &c->Hmean, &c->uumean, &c->vvmean, &c->hhmean, &c->zsmean, &c->zsmax, &c->Cmean};
     for (auto** q : two) { A(q, N); }
     A(&f.umean, ny);
@@ -2350,8 +2368,10 @@ int xbg_step(xbg_ctx* c, double dt, double totaltime) {
         LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), c->kp, s, c->f, 0);
     }
     if (c->P.swave == 1 && (rc = launch_wave(c, s, c->P.flow == 1 ? 1 : 0)) != 0) return rc;
-    if (c->P.flow == 1 && (rc = launch_flow(c, s)) != 0) return rc;
-    if (c->P.sedtrans == 1 && totaltime >= c->P.sedstart && (rc = launch_sed(c, s)) != 0) return rc;
+    const int sed = c->P.sedtrans == 1 && totaltime >= c->P.sedstart;
+    const int ua = c->P.flow == 1 && sed;
+    if (c->P.flow == 1 && (rc = launch_flow(c, s, ua)) != 0) return rc;
+    if (sed && (rc = launch_sed(c, s, ua)) != 0) return rc;
     return XBG_OK;
 }
 
