@@ -244,6 +244,7 @@ XDEV float ctheta_of(const KP& p, const float* dirs, const Refr& r, int th) {
 // cell's wci.
 struct AdvC {
     float sc, sxm, sxp, sym, syp, ui, uxm, uxp, vi, vym, vyp;
+    float lsc, lsxm;  // speed at (x0, y0+cx) and its -x neighbour (left-face lanes)
 };
 template <bool CUR>
 XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
@@ -256,6 +257,7 @@ XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
     a.sxp = S[xp + row];
     a.sym = S[x + (size_t)ym * nx];
     a.syp = S[x + (size_t)yp * nx];
+    a.lsc = a.lsxm = 0.0f;
     if (!CUR) {
         a.ui = a.uxm = a.uxp = a.vi = a.vym = a.vyp = 0.0f;
         return a;
@@ -530,6 +532,17 @@ XDEV void store_plane(float* pl, const SP& s, const Slots& sl, const float* raw,
     }
 }
 
+// Speeds for the lanes that form the left faces of the tile's first column
+// (lane cx < WTY of the tile's second row of threads: row y0 + cx).
+XDEV bool left_lane(int cx, int cy) { return cy == 1 && cx < WTY; }
+XDEV void left_speeds(AdvC& a, const KP& p, const float* S, int x0, int y0, int cx, int cy) {
+    if (left_lane(cx, cy)) {
+        size_t row = (size_t)min(y0 + cx, p.ny - 1) * p.nx;
+        a.lsc = S[x0 + row];
+        a.lsxm = S[m1(x0) + row];
+    }
+}
+
 // x/y advection, theta advection and Euler update of all bins of one field
 // for the thread's cell.  Planes are staged through two LDS buffers (the
 // load of bin th+1 overlaps bin th).  Bin th-2 is finished as soon as the
@@ -538,9 +551,19 @@ XDEV void store_plane(float* pl, const SP& s, const Slots& sl, const float* raw,
 // field on exit (eulerupwindCPU, wave_CPU.cpp:810-828).  With WAVEDT the
 // largest |ctheta| over the bins, in bin order, is returned in *cmax: the
 // theta fluxes evaluate ctheta of every bin once.
+//
+// Without the current (CUR = false) the flux through the face between two
+// cells is the same IEEE expression in both cells' stencils (the face speed
+// 0.5*(S[a]*cs + S[b]*cs) is symmetric and the upwind operands are the same
+// clamped neighbours), so each face is formed once: every cell forms its
+// right and top faces, the tile's first row also its bottom faces and eight
+// lanes the left faces of its first column; the faces go through LDS
+// (fx / gy, one buffer per bin parity) and each cell takes its left and
+// bottom face one bin later, behind the bin's barrier.
 template <int NT, bool IS_EE, bool CUR>
-XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f, const AdvC& a, const Refr& rf,
-                        const Slots& sl, int tid, int cx, int cy, float* ea, float* cmax) {
+XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[WTY + 1][BX], const KP& p,
+                        const SP& s, const Fld& f, const AdvC& a, const Refr& rf, const Slots& sl, int tid, int cx,
+                        int cy, float* ea, float* cmax) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
     const float dxi = 1.0f / p.dx;
@@ -586,6 +609,57 @@ XDEV void advect_planes(float (*pl)[PN], const KP& p, const SP& s, const Fld& f,
     if (nt > 1) load_plane<IS_EE>(raw, p, f, sl, 1);
     lds_barrier();
     const int c0 = (cy + 2) * PW + (cx + 2);
+    if (!CUR) {
+        const int cl = (cx + 2) * PW + 2;  // left-face lanes: first cell of row cx
+        float fxo = 0.0f, gyo = 0.0f;      // this cell's right / top face of the previous bin
+#pragma unroll
+        for (int th = 0; th <= A; th++) {
+            if (th > nt) break;
+            if (th >= 1) {
+                const int b = (th - 1) & 1;
+                ea[th - 1] = (fxo - fx[b][cy][cx]) * dxi + (gyo - gy[b][cy][cx]) * dxi;
+                asm volatile("" : "+v"(ea[th - 1]) : : "memory");
+            }
+            if (th == nt) break;
+            const float* q = pl[th & 1];
+            const float cs = tab(f.dirs + MAXT, th), sn = tab(f.dirs + 2 * MAXT, th);
+            float ec = q[c0];
+            e0[th] = ec;
+            float cgx = 0.5f * (a.sc * cs + a.sxp * cs);
+            bool px = cgx > 0.0f;
+            fxo = upw2d(px ? ec : q[c0 + 1], px ? q[c0 - 1] : q[c0 + 2]) * cgx;
+            float cgy = 0.5f * (a.sc * sn + a.syp * sn);
+            bool py = cgy > 0.0f;
+            gyo = upw2d(py ? ec : q[c0 + PW], py ? q[c0 - PW] : q[c0 + 2 * PW]) * cgy;
+            fx[th & 1][cy][cx + 1] = fxo;
+            gy[th & 1][cy + 1][cx] = gyo;
+            if (cy == 0) {  // bottom faces of the tile
+                float cgymin = 0.5f * (a.sc * sn + a.sym * sn);
+                bool pym = cgymin > 0.0f;
+                gy[th & 1][0][cx] = upw2d(pym ? q[c0 - PW] : ec, pym ? q[c0 - 2 * PW] : q[c0 + PW]) * cgymin;
+            }
+            if (left_lane(cx, cy)) {  // left faces of the tile
+                float cgxmin = 0.5f * (a.lsc * cs + a.lsxm * cs);
+                bool pxm = cgxmin > 0.0f;
+                fx[th & 1][cx][0] = upw2d(pxm ? q[cl - 1] : q[cl], pxm ? q[cl - 2] : q[cl + 1]) * cgxmin;
+            }
+            if (th >= 2) finish(th - 2);
+            asm volatile("" : "+v"(fxo), "+v"(gyo) : : "memory");
+            if (th >= 2) asm volatile("" : "+v"(ea[th - 2]) : : "memory");
+            if (th + 1 < nt) {
+                store_plane<IS_EE>(pl[(th + 1) & 1], s, sl, raw, tid);
+                if (th + 2 < nt) load_plane<IS_EE>(raw, p, f, sl, th + 2);
+            }
+            lds_barrier();
+        }
+#pragma unroll
+        for (int b = 0; b < A; b++) {
+            if (b >= nt) break;
+            if (b >= nt - 2) finish(b);
+        }
+        if (IS_EE) *cmax = cm;
+        return;
+    }
 #pragma unroll
     for (int th = 0; th < A; th++) {
         if (th >= nt) break;
@@ -629,6 +703,8 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     __shared__ float pl[2][PN];
     __shared__ float wred[WTHREADS / 64];
     __shared__ float stash[NT > 0 ? NT : 1][WTHREADS];
+    __shared__ float fx[CUR ? 1 : 2][WTY][BX + 1];
+    __shared__ float gy[CUR ? 1 : 2][WTY + 1][BX];
     const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
     const int tile = xcd_tile(blockIdx.x, p.wtiles_x * p.wtiles_y);
     const int x0 = (tile % p.wtiles_x) * BX, y0 = (tile / p.wtiles_x) * WTY;
@@ -645,7 +721,8 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     float cmax;
     {
         AdvC a = advc_cell<CUR>(p, f, f.cg, x, y);
-        advect_planes<NT, true, CUR>(pl, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
+        if (!CUR) left_speeds(a, p, f.cg, x0, y0, cx, cy);
+        advect_planes<NT, true, CUR>(pl, fx, gy, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
     }
     float wmin = fmn(9999999.9f, p.CFL * p.dtheta / cmax);
     const int side = p.fold_rows && NT > 0 ? -1 : ((y == 1) ? 0 : (y == ny - 2 ? 1 : -1));
@@ -665,7 +742,8 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     float ra[A];
     if (p.roller == 1) {
         AdvC a = advc_cell<CUR>(p, f, f.c, x, y);
-        advect_planes<NT, false, CUR>(pl, p, s, f, a, rf, sl, tid, cx, cy, ra, nullptr);
+        if (!CUR) left_speeds(a, p, f.c, x0, y0, cx, cy);
+        advect_planes<NT, false, CUR>(pl, fx, gy, p, s, f, a, rf, sl, tid, cx, cy, ra, nullptr);
         if (valid && side >= 0) {
 #pragma unroll
             for (int th = 0; th < A; th++) {
