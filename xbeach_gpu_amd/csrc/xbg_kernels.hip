@@ -102,6 +102,17 @@ XDEV float tab(const float* t, int th) {
     return ((cfloat_ptr)t)[k];
 }
 
+// The wave kernel's copy of the direction table in LDS (filled at its
+// start): its lookups are then LDS reads, which complete in order with the
+// plane and face reads, where a scalar load (whose results may return out
+// of order) makes each wait on it a wait for every outstanding LDS read.
+__shared__ float s_dirs[8 * MAXT];
+template <bool L>
+XDEV float dtab(const float* g, int k, int th) {
+    if (L) return s_dirs[k * MAXT + th];
+    return tab(g + k * MAXT, th);
+}
+
 // ---------------------------------------------------------------------------
 // Flow boundary: ubnd1D (Flow_kernel.cu:167-242) / ubnd1Dnowaves (332-403).
 // The alongshore mean of uu[0,:] is taken over the pre-update column and
@@ -227,9 +238,9 @@ XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
 // CUR = false: wci == 0, where every current term is a product with w = 0
 // (a signed zero added to the rest; it can change only the sign of a zero
 // result, which no later step distinguishes), so it is left out.
-template <bool CUR>
+template <bool CUR, bool L = false>
 XDEV float ctheta_of(const KP& p, const float* dirs, const Refr& r, int th) {
-    float cs = tab(dirs + MAXT, th), sn = tab(dirs + 2 * MAXT, th);
+    float cs = dtab<L>(dirs, 1, th), sn = dtab<L>(dirs, 2, th);
     float ctht;
     if (CUR)
         ctht = r.sig * (r.dhdx * sn - r.dhdy * cs) + r.w * (cs * (sn * r.dudx - cs * r.dudy) + sn * (sn * r.dvdx - cs * r.dvdy));
@@ -274,10 +285,10 @@ XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
 
 // x + y flux divergence of one bin from its 9-point cross stencil
 // (values at the clamped neighbour indices).
-template <bool CUR>
+template <bool CUR, bool L = false>
 XDEV float adv_xy(const KP& p, const float* dirs, const AdvC& a, float dxi, int th, float ec, float exm, float exm2, float exp_,
                   float exp2, float eym, float eym2, float eyp, float eyp2) {
-    float cs = tab(dirs + MAXT, th), sn = tab(dirs + 2 * MAXT, th);
+    float cs = dtab<L>(dirs, 1, th), sn = dtab<L>(dirs, 2, th);
     // upwind operands are selected first so each face evaluates one
     // 2nd-order value (branch-free code would otherwise evaluate both)
     float cgx = CUR ? 0.5f * (a.sc * cs + a.ui + a.sxp * cs + a.uxp) : 0.5f * (a.sc * cs + a.sxp * cs);
@@ -322,7 +333,7 @@ XDEV float baldock_cell(const KP& p, float Trep, float Ht, float h, float k, flo
 // energyCPU + energintCPU + roelvink/baldock + dissipationCPU
 // (wave_CPU.cpp:873-1089) at cell i: e (action after Euler) -> ee after
 // dissipation, r (roller after Euler) -> rr after dissipation.
-template <int NT, bool CUR>
+template <int NT, bool CUR, bool L = false>
 XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e, float* r, float& Rout,
                       float& DRout) {
     const int nt = NT > 0 ? NT : p.nt;
@@ -364,8 +375,8 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
         if (th >= nt) break;
         float eet = e[th];
-        float cx = CUR ? cc * tab(f.dirs + MAXT, th) + uui : cc * tab(f.dirs + MAXT, th);
-        float cy = CUR ? cc * tab(f.dirs + 2 * MAXT, th) + vvi : cc * tab(f.dirs + 2 * MAXT, th);
+        float cx = CUR ? cc * dtab<L>(f.dirs, 1, th) + uui : cc * dtab<L>(f.dirs, 1, th);
+        float cy = CUR ? cc * dtab<L>(f.dirs, 2, th) + vvi : cc * dtab<L>(f.dirs, 2, th);
         float dd = eet * Dv / Emx;
         float rrt;
         if (wetc) {
@@ -390,7 +401,7 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
 // meandirCPU (wave_CPU.cpp:1092-1116), radstressCPU (1118-1157) and
 // breakerdelayCPU (1334-1374, called with R) at cell i on the final ee/rr;
 // stores the directional state of the cell.
-template <int NT>
+template <int NT, bool L = false>
 XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const float* efin, const float* r, float Rv,
                       float DRv) {
     const int nt = NT > 0 ? NT : p.nt;
@@ -403,13 +414,13 @@ XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const fl
         if (th >= nt) break;
         float ev = efin[th], rv = r[th];
         sume = sume + ev;
-        sumethet = sumethet + ev * tab(f.dirs, th);
-        txx = txx + (tab(f.dirs + 3 * MAXT, th) * ev);   // (1 + cs*cs) * ev
-        tyy = tyy + (tab(f.dirs + 4 * MAXT, th) * ev);   // (1 + sn*sn) * ev
-        txy = txy + (tab(f.dirs + 5 * MAXT, th) * ev);   // sn*cs*ev
-        rxx = rxx + (tab(f.dirs + 6 * MAXT, th) * rv);   // (cs*cs) * rv
-        ryy = ryy + (tab(f.dirs + 7 * MAXT, th) * rv);   // (sn*sn) * rv
-        rxy = rxy + (tab(f.dirs + 5 * MAXT, th) * rv);   // sn*cs*rv
+        sumethet = sumethet + ev * dtab<L>(f.dirs, 0, th);
+        txx = txx + (dtab<L>(f.dirs, 3, th) * ev);   // (1 + cs*cs) * ev
+        tyy = tyy + (dtab<L>(f.dirs, 4, th) * ev);   // (1 + sn*sn) * ev
+        txy = txy + (dtab<L>(f.dirs, 5, th) * ev);   // sn*cs*ev
+        rxx = rxx + (dtab<L>(f.dirs, 6, th) * rv);   // (cs*cs) * rv
+        ryy = ryy + (dtab<L>(f.dirs, 7, th) * rv);   // (sn*sn) * rv
+        rxy = rxy + (dtab<L>(f.dirs, 5, th) * rv);   // sn*cs*rv
         f.ee_out[i + (size_t)th * N] = ev;
         f.rr_out[i + (size_t)th * N] = rv;
     }
@@ -568,7 +579,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
     const int nt = NT > 0 ? NT : p.nt;
     const float dxi = 1.0f / p.dx;
     float e0[A];
-    float ctl = ctheta_of<CUR>(p, f.dirs, rf, 0), fl = 0.0f;
+    float ctl = ctheta_of<CUR, true>(p, f.dirs, rf, 0), fl = 0.0f;
     float cm = 0.0f;
     if (IS_EE) {
         float v = fabsf(ctl);
@@ -583,7 +594,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
             ta = 0.0f;
         } else if (b < nt - 1) {
             int tm = m1(b), tp = b + 1, tp2 = p2(b, nt);
-            float ctr = ctheta_of<CUR>(p, f.dirs, rf, tp);
+            float ctr = ctheta_of<CUR, true>(p, f.dirs, rf, tp);
             if (IS_EE) {
                 float v = fabsf(ctr);
                 cm = v > cm ? v : cm;
@@ -622,7 +633,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
             }
             if (th == nt) break;
             const float* q = pl[th & 1];
-            const float cs = tab(f.dirs + MAXT, th), sn = tab(f.dirs + 2 * MAXT, th);
+            const float cs = dtab<true>(f.dirs, 1, th), sn = dtab<true>(f.dirs, 2, th);
             float ec = q[c0];
             e0[th] = ec;
             float cgx = 0.5f * (a.sc * cs + a.sxp * cs);
@@ -666,7 +677,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
         const float* q = pl[th & 1];
         float ec = q[c0];
         e0[th] = ec;
-        ea[th] = adv_xy<CUR>(p, f.dirs, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
+        ea[th] = adv_xy<CUR, true>(p, f.dirs, a, dxi, th, ec, q[c0 - 1], q[c0 - 2], q[c0 + 1], q[c0 + 2], q[c0 - PW], q[c0 - 2 * PW],
                         q[c0 + PW], q[c0 + 2 * PW]);
         if (th >= 2) finish(th - 2);
         // materialise this bin's results (otherwise the arithmetic is sunk
@@ -713,8 +724,10 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     const bool valid = (x0 + cx < nx) && (y0 + cy < ny);
     const uint32_t i = (uint32_t)x + (uint32_t)y * nx;
     float ea[A];
+    for (int l = tid; l < 8 * MAXT; l += WTHREADS) s_dirs[l] = f.dirs[l];
     const Refr rf = refr_cell<CUR>(p, s, f, x, y);
     const Slots sl = make_slots(p, x0, y0, tid);
+    __syncthreads();
     // WAVEDT (Wave_kernel.cu:232-249): min over bins of CFL*dtheta/|ctheta|.
     // Rounded division is monotonic in the divisor, so this equals one
     // division by the largest |ctheta| (NaN speeds ignored as fminf does).
@@ -794,7 +807,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
         }
     }
     float Rv = 0.0f, DRv = 0.0f;
-    if (valid) dissip_cell<NT, CUR>(p, s, f, i, ea, ra, Rv, DRv);
+    if (valid) dissip_cell<NT, CUR, true>(p, s, f, i, ea, ra, Rv, DRv);
     if (ftile) {
         lds_barrier();  // every thread has read its row of the stash
         if (bsink) {
@@ -808,7 +821,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
         }
     }
     if (valid) {
-        finish_cell<NT>(p, s, f, i, ea, ra, Rv, DRv);
+        finish_cell<NT, true>(p, s, f, i, ea, ra, Rv, DRv);
     } else {
         wmin = 9999999.9f;
     }
