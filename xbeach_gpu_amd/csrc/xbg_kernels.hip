@@ -1110,23 +1110,26 @@ __global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
 
 // wrkuu2Ocorr (1960-2044), wrkvv2Ocorr (2046-2128), uu2Ocorr (2210-2245),
 // vv2Ocorr (2247-2282) and depthhu / depthhv (653-750, uumin = 0) on the
-// corrected velocities (Flow_kernel.cu), in one pass over a 64 x 4 tile:
+// corrected velocities (Flow_kernel.cu), in one pass over a 64 x CY tile:
 //   1. zs, zb, qx, qy, u, uold, v, vold on the tile + 3-cell halo into LDS
 //      (slots hold the values at the clamped coordinates, which equal the
 //      mminus/pplus/mminus2/pplus2 neighbours);
 //   2. the four correction fluxes on the tile + 1-cell halo, into LDS;
 //   3. the corrected u, v of the cell and its flux depths.
-constexpr int DW = BX + 6, DH = BY + 6;  // data tile:  x0-3 .. x0+BX+2, y0-3 .. y0+BY+2
-constexpr int KW = BX + 2, KH = BY + 2;  // flux tile:  x0-1 .. x0+BX,   y0-1 .. y0+BY
+// Tile height of the two corrector kernels: their halos reach 3 rows up
+// and down, so a taller tile stages fewer halo rows per cell.
+constexpr int CY = 8;
+constexpr int DW = BX + 6, DH = CY + 6;  // data tile:  x0-3 .. x0+BX+2, y0-3 .. y0+CY+2
+constexpr int KW = BX + 2, KH = CY + 2;  // flux tile:  x0-1 .. x0+BX,   y0-1 .. y0+CY
 enum { T_ZS, T_ZB, T_QX, T_QY, T_U, T_UO, T_V, T_VO, T_N };
-__global__ __launch_bounds__(256) void k_flow_uvcorr(KP p, SP s, Fld f) {
+__global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
     __shared__ float dt_[T_N][DH][DW];
     __shared__ float kt[4][KH][KW];
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * CY;
     const float* src[T_N] = {f.zs, f.zb, f.qx, f.qy, f.uu_b, f.uuold, f.vv_b, f.vvold};
-    for (int l = tid; l < DW * DH; l += BX * BY) {
+    for (int l = tid; l < DW * DH; l += BX * CY) {
         int r = l / DW, c = l - r * DW;
         int gxx = min(max(x0 - 3 + c, 0), nx - 1), gyy = min(max(y0 - 3 + r, 0), ny - 1);
         size_t g = ix2(p, gxx, gyy);
@@ -1137,7 +1140,7 @@ __global__ __launch_bounds__(256) void k_flow_uvcorr(KP p, SP s, Fld f) {
     // field k at unclamped (ux, uy)
     auto D = [&](int k, int ux, int uy) { return dt_[k][uy - (y0 - 3)][ux - (x0 - 3)]; };
     const float two_eps = p.eps * 2.0f;
-    for (int l = tid; l < KW * KH; l += BX * BY) {
+    for (int l = tid; l < KW * KH; l += BX * CY) {
         int r = l / KW, c = l - r * KW;
         int x = x0 - 1 + c, y = y0 - 1 + r;
         float w1u = 0.0f, w2u = 0.0f, w1v = 0.0f, w2v = 0.0f;
@@ -1259,7 +1262,7 @@ XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, f
 }
 
 // Continuity, wrkzs2Ocorr (Flow_kernel.cu:2130-2208), zs2Ocorr (2284-2319),
-// latbnd(zs) and hsbnd (1683-1710) in one pass over a 64 x 4 tile:
+// latbnd(zs) and hsbnd (1683-1710) in one pass over a 64 x CY tile:
 //   1. water level after continuity on the tile + (-2..+1, -3..+1) halo
 //      (the halo the correction fluxes of step 2 reach), into LDS;
 //   2. correction fluxes w1/w2 on the tile + (-1, -2..-1) halo, into LDS;
@@ -1268,27 +1271,27 @@ XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, f
 // Halo slots hold the values at the clamped coordinates, which equal the
 // mminus/pplus/mminus2/pplus2 neighbours.  The new level goes to the other
 // zs buffer: other tiles still read the old one in step 1.
-constexpr int ZW = BX + 4, ZH = BY + 5;  // level tile: x0-2 .. x0+BX+1, y0-3 .. y0+BY+1
-constexpr int FW = BX + 1, FH = BY + 2;  // flux tile:  x0-1 .. x0+BX-1, y0-2 .. y0+BY-1
-__global__ __launch_bounds__(256) void k_flow_zsall(KP p, SP s, Fld f) {
+constexpr int ZW = BX + 4, ZH = CY + 5;  // level tile: x0-2 .. x0+BX+1, y0-3 .. y0+CY+1
+constexpr int FW = BX + 1, FH = CY + 2;  // flux tile:  x0-1 .. x0+BX-1, y0-2 .. y0+CY-1
+__global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
     __shared__ float zt[ZH][ZW];
     __shared__ float w1t[FH][FW], w2t[FH][FW];
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
-    for (int l = tid; l < ZW * ZH; l += BX * BY) {
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * CY;
+    for (int l = tid; l < ZW * ZH; l += BX * CY) {
         int r = l / ZW, c = l - r * ZW;
         int ux = x0 - 2 + c, uy = y0 - 3 + r;
         int gxx = min(max(ux, 0), nx - 1), gyy = min(max(uy, 0), ny - 1);
         float dzdt = 0.0f;
         float z = continuity_at(p, s, f, gxx, gyy, &dzdt);
         zt[r][c] = z;
-        if (ux == gxx && uy == gyy && ux >= x0 && ux < x0 + BX && uy >= y0 && uy < y0 + BY && gxx > 0)
+        if (ux == gxx && uy == gyy && ux >= x0 && ux < x0 + BX && uy >= y0 && uy < y0 + CY && gxx > 0)
             f.dzsdt[ix2(p, gxx, gyy)] = dzdt;
     }
     __syncthreads();
     const float two_eps = p.eps * 2.0f;
-    for (int l = tid; l < FW * FH; l += BX * BY) {
+    for (int l = tid; l < FW * FH; l += BX * CY) {
         int r = l / FW, c = l - r * FW;
         int wx = x0 - 1 + c, wy = y0 - 2 + r;
         float w1 = 0.0f, w2 = 0.0f;
@@ -2093,8 +2096,9 @@ int launch_flow(xbg_ctx* c, const SP& s, int want_ua = 0) {
     LAUNCH(c, "k_flow_nuh", k_flow_nuh, g, blk2, p, c->f);
     LAUNCH(c, "k_flow_u", k_flow_u, g, blk2, p, s, c->f);
     LAUNCH(c, "k_flow_v", k_flow_v, g, blk2, p, s, c->f);
-    LAUNCH(c, "k_flow_uvcorr", k_flow_uvcorr, g, blk2, p, s, c->f);
-    LAUNCH(c, "k_flow_zsall", k_flow_zsall, g, blk2, p, s, c->f);
+    const dim3 gc((p.nx + BX - 1) / BX, (p.ny + CY - 1) / CY, 1), bc(BX, CY, 1);
+    LAUNCH(c, "k_flow_uvcorr", k_flow_uvcorr, gc, bc, p, s, c->f);
+    LAUNCH(c, "k_flow_zsall", k_flow_zsall, gc, bc, p, s, c->f);
     c->zcur = 1 - c->zcur;
     sync_ptrs(c);
     LAUNCH(c, "k_flow_uvvu", k_flow_uvvu, g, blk2, p, s, c->f, want_ua);
