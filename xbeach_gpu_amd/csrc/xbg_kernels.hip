@@ -1118,7 +1118,7 @@ __global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
 //   3. the corrected u, v of the cell and its flux depths.
 // Tile height of the two corrector kernels: their halos reach 3 rows up
 // and down, so a taller tile stages fewer halo rows per cell.
-constexpr int CY = 8;
+constexpr int CY = 16;
 constexpr int DW = BX + 6, DH = CY + 6;  // data tile:  x0-3 .. x0+BX+2, y0-3 .. y0+CY+2
 constexpr int KW = BX + 2, KH = CY + 2;  // flux tile:  x0-1 .. x0+BX,   y0-1 .. y0+CY
 enum { T_ZS, T_ZB, T_QX, T_QY, T_U, T_UO, T_V, T_VO, T_N };
