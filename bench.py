@@ -53,7 +53,7 @@ def alg_bytes(p, kernel="step"):
 
 
 def run_gpu(p, steps, warmup, t0=300.0):
-    from xbeach_gpu_amd import Domain, next_dt
+    from xbeach_gpu_amd import Domain
 
     st, frc = build_domain(p)
     d = Domain(p)
@@ -67,14 +67,9 @@ def run_gpu(p, steps, warmup, t0=300.0):
     state = {"t": t0, "dt": 0.05}
 
     def one_step():
-        dtf, dtw = d.cfl(False)
-        dt = next_dt(dtf, dtw, 1, state["dt"], state["t"], 1e30, 1e30)
-        t = state["t"] + dt
-        rec = int((t - t0) // frc.dtbc)
-        tsl = t - (t0 + rec * frc.dtbc)
-        d.use_bnd_record(rec, tsl, frc.dtbc, frc.Trep)
-        d.step(dt, t)
-        state["t"], state["dt"] = t, dt
+        # cfl -> next_dt -> record of the new time -> step, as the host main
+        # loop (mainloopGPU) runs them, in one library call
+        state["t"], state["dt"] = d.advance_resident(t0, frc.dtbc, frc.Trep, state["t"], state["dt"])
 
     for _ in range(warmup):
         one_step()

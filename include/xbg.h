@@ -207,6 +207,19 @@ int xbg_step(xbg_ctx* ctx, double dt, double totaltime);
  */
 int xbg_cfl_dt(xbg_ctx* ctx, int initial, double* dt_flow, double* dt_wave);
 
+/*
+ * One iteration of the main loop (mainloopGPU, Wave_gpu.cu:610-728) on a
+ * resident record series (xbg_set_bnd_series with records dtbc apart from
+ * time t0): xbg_cfl_dt, xbg_next_dt with *dt as the previous step, the
+ * record of the new time (rec = floor((t - t0)/dtbc), timesincelast =
+ * t - (rec*dtbc + t0), no tide or wind), then xbg_step.  On return
+ * *totaltime and *dt hold the new time and step.  The same calls a host
+ * loop would make, without a return to the caller between the CFL read-back
+ * and the launches of the step.
+ */
+int xbg_advance_resident(xbg_ctx* ctx, double t0, double dtbc, double Trep, double nextoutputtime,
+                         double endtime, double* totaltime, double* dt);
+
 /* Running means and max of the output fields (Wave_gpu.cu:731-757, a61). */
 int xbg_accumulate_means(xbg_ctx* ctx);
 int xbg_reset_means(xbg_ctx* ctx);

@@ -153,6 +153,37 @@ def test_resident_boundary_records_match_host_interpolation():
     d.close()
 
 
+def test_advance_resident_matches_host_loop():
+    """xbg_advance_resident runs cfl -> next_dt -> record -> step as a host
+    loop of the separate calls does (mainloopGPU, Wave_gpu.cu:610-728)."""
+    from xbeach_gpu_amd import next_dt
+
+    p, d, o, frc = pair(nx=96, ny=40)
+    p2, d2, o2, _ = pair(nx=96, ny=40)
+    t0 = 150.0
+    S, Q = frc.records(t0, 12)
+    d.set_bnd_series(S, Q)
+    d2.set_bnd_series(S, Q)
+    t, dt = t0, 0.05
+    t2, dt2 = t0, 0.05
+    for _ in range(12):
+        dtf, dtw = d.cfl(False)
+        dt = next_dt(dtf, dtw, 1, dt, t, 1e30, 1e30)
+        t = t + dt
+        rec = int(math.floor((t - t0) / frc.dtbc))
+        tsl = t - (rec * frc.dtbc + t0)
+        d.use_bnd_record(rec, tsl, frc.dtbc, frc.Trep)
+        d.step(dt, t)
+        t2, dt2 = d2.advance_resident(t0, frc.dtbc, frc.Trep, t2, dt2)
+        assert (t2, dt2) == (t, dt)
+    for k in MAIN + MORE:
+        assert np.array_equal(d.get(k), d2.get(k)), k
+    with pytest.raises(Exception):
+        d2.advance_resident(t0, frc.dtbc, frc.Trep, t0 + 12 * frc.dtbc, dt2)  # past the last record
+    d.close()
+    d2.close()
+
+
 def test_cfl_limits_match_oracle():
     p, d, o, frc = pair()
     assert d.cfl(initial=True)[0] == o.cfl(initial=True)[0]

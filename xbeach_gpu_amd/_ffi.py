@@ -91,6 +91,7 @@ PROTOTYPES = {
     "xbg_sed_step": (C.c_int, [_CTX, C.c_double]),
     "xbg_step": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xbg_cfl_dt": (C.c_int, [_CTX, C.c_int, _D, _D]),
+    "xbg_advance_resident": (C.c_int, [_CTX, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _D, _D]),
     "xbg_accumulate_means": (C.c_int, [_CTX]),
     "xbg_reset_means": (C.c_int, [_CTX]),
     "xbg_divide_means": (C.c_int, [_CTX, C.c_double]),

@@ -2362,6 +2362,28 @@ int xbg_use_bnd_record(xbg_ctx* c, int rec, double timesincelast, double timenex
     return XBG_OK;
 }
 
+// One iteration of the main loop (Wave_gpu.cu:610-728) on a resident record
+// series: CFL limits, the time-step rule, the record of the new time and the
+// step, without returning to the caller between the dt read-back and the
+// launches of the step.
+int xbg_advance_resident(xbg_ctx* c, double t0, double dtbc, double Trep, double nextoutputtime, double endtime,
+                         double* totaltime, double* dt) {
+    if (!c || !totaltime || !dt || !(dtbc > 0.0)) return XBG_E_ARG;
+    double dtf, dtw, dtn;
+    int rc = xbg_cfl_dt(c, 0, &dtf, &dtw);
+    if (rc) return rc;
+    if ((rc = xbg_next_dt(dtf, dtw, c->P.swave == 1, *dt, *totaltime, nextoutputtime, endtime, &dtn)) != XBG_OK)
+        return fail(c, rc, "Model crashed: non-finite or zero flow time step");
+    const double t = *totaltime + dtn;
+    const int rec = (int)std::floor((t - t0) / dtbc);
+    const double tsl = t - (rec * dtbc + t0);
+    if ((rc = xbg_use_bnd_record(c, rec, tsl, dtbc, Trep, 0.0, 0.0, 0.0, 0.0)) != XBG_OK) return rc;
+    if ((rc = xbg_step(c, dtn, t)) != XBG_OK) return rc;
+    *totaltime = t;
+    *dt = dtn;
+    return XBG_OK;
+}
+
 }  // extern "C"
 
 namespace {

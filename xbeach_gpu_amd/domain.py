@@ -145,6 +145,16 @@ class Domain:
         self._chk(self.L.xbg_cfl_dt(self._ctx, 1 if initial else 0, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def advance_resident(self, t0: float, dtbc: float, Trep: float, totaltime: float, prev_dt: float,
+                         nextoutputtime: float = 1e30, endtime: float = 1e30) -> tuple[float, float]:
+        """One main-loop iteration on the resident record series (cfl, next_dt,
+        use_bnd_record of the new time, step) in one library call; returns the
+        new (totaltime, dt)."""
+        t, d = C.c_double(totaltime), C.c_double(prev_dt)
+        self._chk(self.L.xbg_advance_resident(self._ctx, float(t0), float(dtbc), float(Trep), float(nextoutputtime),
+                                              float(endtime), C.byref(t), C.byref(d)))
+        return t.value, d.value
+
     def accumulate_means(self) -> None:
         self._chk(self.L.xbg_accumulate_means(self._ctx))
 
