@@ -23,8 +23,8 @@
 //   k_flow_uvvu     calcuvvu + uvlatbnd
 //   k_sed_flux      longturb + Rvr + Susp
 //   k_sed_conc      Sbvr + Erosus + Conc + CClatbnd + bedupdate + zblatbnd
-//   k_sed_aval      avalanching (gather form, no scatter race)
-//   k_sed_update    updatezb + zblatbnd
+//   k_sed_morph     avalanching (gather form, no scatter race) + updatezb
+//                   + zblatbnd
 //
 // In-place boundary copies of the reference (rollerlatbnd, latbnd,
 // vvlatbnd, uvlatbnd, CClatbnd, zblatbnd, twodimbnd) are folded into the
@@ -74,7 +74,7 @@ struct Fld {
     float *uu_b, *vv_b;
     float* zs_out;  // new water level of the zs pass (the other zs buffer)
     // sediment intermediates
-    float *Sus, *Svs, *C_out, *hh_b, *zb_b, *stdep_b, *dzbx, *dzby, *ceqsg;
+    float *Sus, *Svs, *C_out, *hh_b, *zb_b, *stdep_b, *ceqsg;
     double* zb64_b;
     float* zbc_b;
     // per-block partial minima of the CFL reductions
@@ -1603,11 +1603,9 @@ __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) 
     }
 }
 
-// avalanching (Sediment_kernel.cu:1020-1137): slope-limited exchange with the
-// +x and +y neighbours, stored per cell and gathered by k_sed_update.
-__global__ __launch_bounds__(256) void k_sed_aval(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
+// avalanching (Sediment_kernel.cu:1020-1137) at (x, y): slope-limited
+// exchange with the +x and +y neighbours.
+XDEV void aval_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& bxo, float& byo) {
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
     const float *zb = f.zb_b, *sd = f.stdep_b;
     float dzmax = f.hh_b[i] > p.eps ? p.wetdzmax : p.drydzmax;
@@ -1637,28 +1635,48 @@ __global__ __launch_bounds__(256) void k_sed_aval(KP p, SP s, Fld f) {
             by = fmx(by, -1 * sd[t]);
         }
     }
-    f.dzbx[i] = bx;
-    f.dzby[i] = by;
+    bxo = bx;
+    byo = by;
 }
 
-XDEV float ddzb_at(const KP& p, const Fld& f, int x, int y) {
+// avalanching + updatezb (Sediment_kernel.cu:1020-1159) + zblatbnd(zb),
+// zblatbnd(stdep) in one pass over a 64 x BY tile.  The exchanges are formed
+// into LDS for the tile and the halo its gather reaches: one column to the
+// left (the -x exchange) and two rows below (the -y exchange, and row ny-1
+// takes the change of row ny-2, whose gather reads row ny-3).  The reference
+// scatters the exchange into both cells without atomics; the change of each
+// cell is gathered here as (dzbx + dzby)_i - dzbx_{i-1} - dzby_{i-nx}.
+constexpr int MW = BX + 1, MH = BY + 2;  // exchange tile: x0-1 .. x0+BX-1, y0-2 .. y0+BY-1
+__global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
+    __shared__ float ex[MH][MW], ey[MH][MW];
+    const int nx = p.nx, ny = p.ny;
+    const int tid = threadIdx.x + threadIdx.y * BX;
+    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    for (int l = tid; l < MW * MH; l += BX * BY) {
+        int r = l / MW, c = l - r * MW;
+        int hx = x0 - 1 + c, hy = y0 - 2 + r;
+        float bx = 0.0f, by = 0.0f;
+        if (hx >= 0 && hy >= 0 && hx < nx && hy < ny) aval_at(p, s, f, hx, hy, bx, by);
+        ex[r][c] = bx;
+        ey[r][c] = by;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= nx || y >= ny) return;
+    // bed change of cell (ux, uy) of the tile or its halo
+    auto ddzb = [&](int ux, int uy) {
+        int c = ux - (x0 - 1), r = uy - (y0 - 2);
+        float d = ex[r][c] + ey[r][c];
+        if (ux > 0) d = d - ex[r][c - 1];
+        if (uy > 0) d = d - ey[r - 1][c];
+        return d;
+    };
     size_t i = ix2(p, x, y);
-    float d = f.dzbx[i] + f.dzby[i];
-    if (x > 0) d = d - f.dzbx[i - 1];
-    if (y > 0) d = d - f.dzby[ix2(p, x, y - 1)];
-    return d;
-}
-
-// updatezb (Sediment_kernel.cu:1139-1159) + zblatbnd(zb), zblatbnd(stdep).
-__global__ __launch_bounds__(256) void k_sed_update(KP p, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
-    size_t i = ix2(p, x, y);
-    float d = ddzb_at(p, f, x, y);
+    float d = ddzb(x, y);
     int sx, sy;
     zsrc(p, x, y, sx, sy);
     size_t is = ix2(p, sx, sy);
-    float ds = (is == i) ? d : ddzb_at(p, f, sx, sy);
+    float ds = (is == i) ? d : ddzb(sx, sy);
     f.dzb[i] = f.dzb[i] - d;
     f.zs[i] = f.zs[i] - d;
     f.hh[i] = f.hh_b[i] - d;
@@ -2115,8 +2133,7 @@ int launch_sed(xbg_ctx* c, const SP& s, int ua_ready = 0) {
     c->ccur = 1 - c->ccur;
     sync_ptrs(c);
     if (morph) {
-        LAUNCH(c, "k_sed_aval", k_sed_aval, g, blk2, p, s, c->f);
-        LAUNCH(c, "k_sed_update", k_sed_update, g, blk2, p, c->f);
+        LAUNCH(c, "k_sed_morph", k_sed_morph, g, blk2, p, s, c->f);
     }
     return 0;
 }
@@ -2167,7 +2184,7 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
                      &f.rolthick, &f.dzb, &f.k, &f.c, &f.cg, &f.H, &f.E, &f.Dw, &f.DR, &f.R, &f.urms,
                      &f.ust_raw, &f.ust, &f.thetamean, &f.Sxx, &f.Sxy, &f.Syy, &f.Fx, &f.Fy, &f.nuh,
                      &f.uu_b, &f.vv_b, &f.Sus,
-                     &f.Svs, &f.ua, &f.hh_b, &f.zb_b, &f.stdep_b, &f.dzbx, &f.dzby, &f.ceqsg, &f.zbc, &f.zbc_b,
+                     &f.Svs, &f.ua, &f.hh_b, &f.zb_b, &f.stdep_b, &f.ceqsg, &f.zbc, &f.zbc_b,
                      &c->Hmean, &c->uumean, &c->vvmean, &c->hhmean, &c->zsmean, &c->zsmax, &c->Cmean};
     for (auto** q : two) { A(q, N); }
     A(&f.umean, ny);
