@@ -882,9 +882,7 @@ XDEV void wavforce_at(const KP& p, const Fld& f, int x, int y, float& Fxv, float
 // wavforce + twodimbndnoix(Fx) + twodimbnd(Fy) + twodimbnd(ust)
 // (Wavestep.cu:1099-1125; CPU twins wave_CPU.cpp:1224-1331: the neighbour
 // value read is the pre-update one and the ix==0 rule wins).
-__global__ __launch_bounds__(256) void k_wave_force(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
-    if (x >= p.nx || y >= p.ny) return;
+XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     size_t i = (size_t)x + (size_t)y * nx;
     float wet = f.hh[i] > p.eps ? 1.0f : 0.0f;
@@ -909,6 +907,11 @@ __global__ __launch_bounds__(256) void k_wave_force(KP p, SP s, Fld f) {
     f.Fy[i] = fy;
     f.ust[i] = ust;
 }
+__global__ __launch_bounds__(256) void k_wave_force(KP p, Fld f) {
+    int x = gx(), y = gy();
+    if (x >= p.nx || y >= p.ny) return;
+    wave_force_cell(p, f, x, y);
+}
 
 // ---------------------------------------------------------------------------
 // Shallow-water flow
@@ -925,10 +928,13 @@ XDEV float wetu_at(const KP& p, const Fld& f, int x, int y) { return (float)(f.w
 XDEV float wetv_at(const KP& p, const Fld& f, int x, int y) { return (float)((f.wet[ix2(p, x, y)] >> 1) & 1); }
 
 // smago (Flow_kernel.cu:1282-1317) and the wet masks of
-// udepthmomcont/vdepthmomcont from the previous step's hu/hv.
-__global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f) {
+// udepthmomcont/vdepthmomcont from the previous step's hu/hv.  With force
+// set, also the wave forces of the step (k_wave_force's work: neither reads
+// what the other writes), one launch where a whole step runs both.
+__global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
     int x = gx(), y = gy();
     if (x >= p.nx || y >= p.ny) return;
+    if (force) wave_force_cell(p, f, x, y);
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     size_t i = ix2(p, x, y);
@@ -2068,7 +2074,9 @@ const char kFieldNames[] =
     "vvold\0zsold\0dzsdt\0rolthick\0dzb\0k\0c\0cg\0H\0E\0DR\0R\0urms\0ust\0thetamean\0Sxx\0Sxy\0Syy\0"
     "Fx\0Fy\0nuh\0Sus\0Svs\0ceqsg\0umeanbnd\0Hmean\0uumean\0vvmean\0hhmean\0zsmean\0zsmax\0Cmean\0";
 
-int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0) {
+// defer_force: the flow step follows and forms the wave forces in its first
+// launch
+int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0, int defer_force = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
     dim3 gd(g.x, g.y + (offshore ? 1 : 0), 1);
@@ -2101,17 +2109,18 @@ int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0) {
             else LAUNCH(c, "k_wave_rows", (k_wave_rows<0, false>), gr, dim3(256), p, s, c->f);
             break;
     }
-    LAUNCH(c, "k_wave_force", k_wave_force, g, blk2, p, s, c->f);
+    if (!defer_force) LAUNCH(c, "k_wave_force", k_wave_force, g, blk2, p, c->f);
     c->cur = 1 - c->cur;
     sync_ptrs(c);
     return 0;
 }
 
-// want_ua: the sediment step follows, so k_flow_uvvu forms its ua field
-int launch_flow(xbg_ctx* c, const SP& s, int want_ua = 0) {
+// want_ua: the sediment step follows, so k_flow_uvvu forms its ua field;
+// force: the wave step deferred its forces to this step's first launch
+int launch_flow(xbg_ctx* c, const SP& s, int want_ua = 0, int force = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
-    LAUNCH(c, "k_flow_nuh", k_flow_nuh, g, blk2, p, c->f);
+    LAUNCH(c, "k_flow_nuh", k_flow_nuh, g, blk2, p, c->f, force);
     LAUNCH(c, "k_flow_u", k_flow_u, g, blk2, p, s, c->f);
     LAUNCH(c, "k_flow_v", k_flow_v, g, blk2, p, s, c->f);
     const dim3 gc((p.nx + BX - 1) / BX, (p.ny + CY - 1) / CY, 1), bc(BX, CY, 1);
@@ -2501,10 +2510,11 @@ int xbg_step(xbg_ctx* c, double dt, double totaltime) {
     if (c->P.flow == 1 && c->P.swave != 1) {
         LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), c->kp, s, c->f, 0);
     }
-    if (c->P.swave == 1 && (rc = launch_wave(c, s, c->P.flow == 1 ? 1 : 0)) != 0) return rc;
+    const int both = c->P.swave == 1 && c->P.flow == 1;
+    if (c->P.swave == 1 && (rc = launch_wave(c, s, c->P.flow == 1 ? 1 : 0, both)) != 0) return rc;
     const int sed = c->P.sedtrans == 1 && totaltime >= c->P.sedstart;
     const int ua = c->P.flow == 1 && sed;
-    if (c->P.flow == 1 && (rc = launch_flow(c, s, ua)) != 0) return rc;
+    if (c->P.flow == 1 && (rc = launch_flow(c, s, ua, both)) != 0) return rc;
     if (sed && (rc = launch_sed(c, s, ua)) != 0) return rc;
     return XBG_OK;
 }
