@@ -3,24 +3,26 @@
 // One step of the reference (Wave_gpu.cu:702-728) is ~92 launches, each a
 // 16x16-block stencil followed by a device synchronisation, plus ~25
 // cudaMalloc/cudaFree pairs and two full-field D2H copies for the time step.
-// Here the same step is 14 stream-ordered launches over buffers allocated
-// once:
+// Here a whole step is 11 stream-ordered launches over buffers allocated
+// once (plus the record interpolation and the CFL pass around it):
 //
-//   k_bnd_offshore  ubnd1D                                  (1 workgroup)
-//   k_dispersion    dispersion -> k, c, cg
+//   k_dispersion    dispersion -> k, c, cg; its first workgroup runs ubnd1D
+//                   (k_bnd_offshore alone when waves are off)
 //   k_wave<NT>      offshorebndWav .. breakerdelay fused per cell: the
 //                   direction planes of ee and rr stream through LDS tiles,
 //                   the bins of a cell stay in registers; one read and one
-//                   write of each directional field per step
-//   k_wave_force    wavforce + twodimbnd(noix) for Fx, Fy, ust
-//   k_flow_nuh      smago + wetu/wetv masks
+//                   write of each directional field per step (k_wave_rows
+//                   finishes rows 0 / ny-1 where a tile cannot)
+//   k_flow_nuh      smago + wetu/wetv masks, and in a whole step the wave
+//                   forces: wavforce + twodimbnd(noix) for Fx, Fy, ust
+//                   (k_wave_force alone for the wave sub-step)
 //   k_flow_u        u momentum (wlevslopes .. eulerustep, latbnd)
 //   k_flow_v        v momentum (vdvdy_adv .. eulervstep, vvlatbnd)
 //   k_flow_uvcorr   wrkuu2Ocorr + wrkvv2Ocorr + uu2Ocorr + vv2Ocorr +
 //                   depthhu/depthhv (LDS tile with halo)
 //   k_flow_zsall    CalcQFlow + continuity + wrkzs2Ocorr + zs2Ocorr +
 //                   latbnd(zs) + hsbnd (LDS tile with halo)
-//   k_flow_uvvu     calcuvvu + uvlatbnd
+//   k_flow_uvvu     calcuvvu + uvlatbnd, and Rvr's ua when sediment follows
 //   k_sed_flux      longturb + Rvr + Susp
 //   k_sed_conc      Sbvr + Erosus + Conc + CClatbnd + bedupdate + zblatbnd
 //   k_sed_morph     avalanching (gather form, no scatter race) + updatezb
@@ -703,9 +705,10 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
 // The fused wave step, one thread per cell of a 64 x WTY tile:
 // offshorebndWav, sanity, action, x/y/theta advection + Euler of ee and rr,
 // energy, energint, roelvink/baldock, dissipation, meandir, radstress,
-// breakerdelay (Wavestep.cu:781-1125 order).  Rows 0 and ny-1 are finished
-// by k_wave_rows (rollerlatbnd copies from rows 1 and ny-2, whose
-// post-Euler bins this kernel saves).  Also the block minimum of WAVEDT.
+// breakerdelay (Wavestep.cu:781-1125 order).  Rows 0 and ny-1
+// (rollerlatbnd copies from rows 1 and ny-2) are finished in the tile when
+// it also holds the source row (fold_rows), else by k_wave_rows from the
+// post-Euler bins this kernel saves.  Also the block minimum of WAVEDT.
 // Four waves per SIMD (<= 128 VGPRs): two workgroups per CU.
 template <int NT, bool CUR>
 __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
