@@ -79,6 +79,8 @@ def test_one_step_small():
     dict(morfac=10.0),
     dict(sedtrans=0),
     dict(nx=130, ny=37),               # neither multiple of the tile
+    dict(nx=96, ny=41),                # ny-1 a multiple of the wave tile: boundary rows in their own launch
+    dict(nx=96, ny=33, dtheta=5.0),    # the same with 24 bins
     dict(nx=5, ny=6),                  # smallest grids
     dict(nx=64, ny=4),
     dict(usesmago=0, nuh=0.5),
