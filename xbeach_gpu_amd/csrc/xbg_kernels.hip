@@ -184,7 +184,12 @@ __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offsh
         }
         by -= 1;
     }
-    int x = blockIdx.x * BX + threadIdx.x, y = by * BY + threadIdx.y;
+    // each wavefront takes a 16 x BY patch of the block's 64 x BY cells: the
+    // iteration count follows the depth, which varies less over 16 cells
+    // across shore than over 64, and a wavefront runs its slowest lane's
+    static_assert(BX == 64 && BY == 4, "four 16 x 4 patches per 64 x 4 block");
+    const int tid = threadIdx.x + threadIdx.y * BX, w = tid >> 6, l = tid & 63;
+    int x = blockIdx.x * BX + 16 * w + (l & 15), y = by * BY + (l >> 4);
     if (x >= p.nx || y >= p.ny) return;
     size_t i = (size_t)x + (size_t)y * p.nx;
     float k, c, cg;
