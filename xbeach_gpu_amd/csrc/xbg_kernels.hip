@@ -175,6 +175,8 @@ __global__ __launch_bounds__(1024) void k_bnd_offshore(KP p, SP s, Fld f, int wa
 // boundary (ubnd1D) instead: it is dispatched first and runs beside the
 // dispersion workgroups, whose inputs (hh) and outputs (k, c, cg) it does
 // not touch, and it completes before the wave kernel reads uu / vv.
+constexpr int DPW = 8, DPH = 8;  // dispersion patch of one wavefront
+static_assert(DPW * DPH == 64 && BX * BY == 256, "four one-wavefront patches per workgroup");
 __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offshore) {
     int by = blockIdx.y;
     if (offshore) {
@@ -184,12 +186,12 @@ __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offsh
         }
         by -= 1;
     }
-    // each wavefront takes a 16 x BY patch of the block's 64 x BY cells: the
-    // iteration count follows the depth, which varies less over 16 cells
-    // across shore than over 64, and a wavefront runs its slowest lane's
-    static_assert(BX == 64 && BY == 4, "four 16 x 4 patches per 64 x 4 block");
+    // each wavefront takes a DPW x DPH patch (the block four of them side by
+    // side): the iteration count follows the depth, which varies less over
+    // a compact patch than along a 64-cell cross-shore row, and a wavefront
+    // runs its slowest lane's
     const int tid = threadIdx.x + threadIdx.y * BX, w = tid >> 6, l = tid & 63;
-    int x = blockIdx.x * BX + 16 * w + (l & 15), y = by * BY + (l >> 4);
+    int x = blockIdx.x * (4 * DPW) + DPW * w + (l % DPW), y = by * DPH + l / DPW;
     if (x >= p.nx || y >= p.ny) return;
     size_t i = (size_t)x + (size_t)y * p.nx;
     float k, c, cg;
@@ -2087,7 +2089,7 @@ const char kFieldNames[] =
 int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0, int defer_force = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
-    dim3 gd(g.x, g.y + (offshore ? 1 : 0), 1);
+    dim3 gd((p.nx + 4 * DPW - 1) / (4 * DPW), (p.ny + DPH - 1) / DPH + (offshore ? 1 : 0), 1);
     LAUNCH(c, "k_dispersion", k_dispersion, gd, blk2, p, s, c->f, offshore);
     dim3 gr((p.nx + 255) / 256, 2, 1);
     dim3 gw(p.wtiles_x * p.wtiles_y, 1, 1), bw(BX, WTY, 1);
