@@ -175,7 +175,7 @@ __global__ __launch_bounds__(1024) void k_bnd_offshore(KP p, SP s, Fld f, int wa
 // boundary (ubnd1D) instead: it is dispatched first and runs beside the
 // dispersion workgroups, whose inputs (hh) and outputs (k, c, cg) it does
 // not touch, and it completes before the wave kernel reads uu / vv.
-constexpr int DPW = 8, DPH = 8;  // dispersion patch of one wavefront
+constexpr int DPW = 4, DPH = 16;  // dispersion patch of one wavefront
 static_assert(DPW * DPH == 64 && BX * BY == 256, "four one-wavefront patches per workgroup");
 __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offshore) {
     int by = blockIdx.y;
