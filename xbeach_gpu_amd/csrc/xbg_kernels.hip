@@ -1573,8 +1573,14 @@ XDEV void zsrc(const KP& p, int x, int y, int& sx, int& sy) {
 
 // Conc + CClatbnd (Sediment_kernel.cu:782-822) + bedupdate + zblatbnd(zb),
 // zblatbnd(stdep).  Writes the *_b buffers.
+//
+// Each wavefront takes a 16 x 4 patch of the block's 64 x 4 cells: the
+// mobilised-sediment branch and the boundary re-evaluations of column 0 are
+// then taken by fewer wavefronts.
 __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) {
-    int x = gx(), y = gy();
+    static_assert(BX == 64 && BY == 4, "four 16 x 4 patches per 64 x 4 block");
+    const int tid = threadIdx.x + threadIdx.y * BX, w = tid >> 6, l = tid & 63;
+    int x = blockIdx.x * BX + 16 * w + (l & 15), y = blockIdx.y * BY + (l >> 4);
     if (x >= p.nx || y >= p.ny) return;
     const int ny = p.ny;
     size_t i = ix2(p, x, y);
