@@ -1574,13 +1574,14 @@ XDEV void zsrc(const KP& p, int x, int y, int& sx, int& sy) {
 // Conc + CClatbnd (Sediment_kernel.cu:782-822) + bedupdate + zblatbnd(zb),
 // zblatbnd(stdep).  Writes the *_b buffers.
 //
-// Each wavefront takes a 16 x 4 patch of the block's 64 x 4 cells: the
-// mobilised-sediment branch and the boundary re-evaluations of column 0 are
-// then taken by fewer wavefronts.
+// Each wavefront takes an SPW x SPH patch (four side by side per
+// workgroup): the mobilised-sediment branch and the boundary re-evaluations
+// of column 0 are then taken by fewer wavefronts.
+constexpr int SPW = 16, SPH = 4;
 __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) {
-    static_assert(BX == 64 && BY == 4, "four 16 x 4 patches per 64 x 4 block");
+    static_assert(SPW * SPH == 64 && BX * BY == 256, "four one-wavefront patches per workgroup");
     const int tid = threadIdx.x + threadIdx.y * BX, w = tid >> 6, l = tid & 63;
-    int x = blockIdx.x * BX + 16 * w + (l & 15), y = blockIdx.y * BY + (l >> 4);
+    int x = blockIdx.x * (4 * SPW) + SPW * w + (l % SPW), y = blockIdx.y * SPH + l / SPW;
     if (x >= p.nx || y >= p.ny) return;
     const int ny = p.ny;
     size_t i = ix2(p, x, y);
@@ -2154,7 +2155,8 @@ int launch_sed(xbg_ctx* c, const SP& s, int ua_ready = 0) {
     dim3 g = grid2(p);
     int morph = p.morfac > 0.0f ? 1 : 0;
     LAUNCH(c, "k_sed_flux", k_sed_flux, g, blk2, p, s, c->f, ua_ready);
-    LAUNCH(c, "k_sed_conc", k_sed_conc, g, blk2, p, s, c->f, morph);
+    LAUNCH(c, "k_sed_conc", k_sed_conc, dim3((p.nx + 4 * SPW - 1) / (4 * SPW), (p.ny + SPH - 1) / SPH, 1), blk2,
+           p, s, c->f, morph);
     c->ccur = 1 - c->ccur;
     sync_ptrs(c);
     if (morph) {
