@@ -42,6 +42,8 @@ struct KP {
     int usesmago, breakmodel, roller, zb_accum, swave;
     int wtiles_x, wtiles_y;  // wave-kernel tile grid
     int fold_rows;           // rows 0 / ny-1 finished inside the wave kernel
+    double rdtheta;          // 1/dtheta rounded once in double (div_uniform)
+    int dth_recip;           // 0 < dtheta < 2: divide through rdtheta
     float theta[MAXT], cxsth[MAXT], sxnth[MAXT];
 };
 
@@ -50,7 +52,21 @@ struct SP {
     float dt, totaltime, Trep, sigm, taper;   // taper of offshorebndWavCPU
     float tsl, tnext, zsbndi, zsbndn, windu, windv;
     int signwu, signwv;
+    double rsigm;                             // 1/sigm rounded once in double (div_uniform)
+    int sigm_recip;                           // 0 < sigm < 2: divide through rsigm
 };
+
+// a / b (IEEE float quotient) for a divisor 0 < b < 2 through rb = 1/b
+// correctly rounded in double: (double)a * rb is within 2^-52 (relative) of
+// a/b, while a quotient of two floats that is not itself a float lies more
+// than 2^-49 from every midpoint between floats (an exact tie needs a
+// subnormal quotient and b >= 2), so rounding the product once to float
+// gives the quotient's own rounding; zeros, infinities and overflow follow.
+// Three VALU instructions instead of the eleven of an IEEE division.
+XDEV float div_uniform(float a, float b, double rb, int use_rb) {
+    if (use_rb) return (float)((double)a * rb);
+    return a / b;
+}
 
 // Friction maps derived from the sediment thickness, updatezom
 // (Sediment_kernel.cu:1162-1189 == Sediment_CPU.cpp:13-40).

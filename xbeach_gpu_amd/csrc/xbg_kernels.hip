@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offsh
 XDEV float act_ee(const KP& p, const SP& s, const Fld& f, int x, int y, int th) {
     float v = (x == 0) ? f.St[y + th * p.ny] * s.taper
                        : f.ee_in[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
-    return fmx(v, 0.000f) / s.sigm;
+    return div_uniform(fmx(v, 0.000f), s.sigm, s.rsigm, s.sigm_recip);
 }
 XDEV float act_rr(const KP& p, const Fld& f, int x, int y, int th) {
     return f.rr_in[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
@@ -545,7 +545,7 @@ XDEV void store_plane(float* pl, const SP& s, const Slots& sl, const float* raw,
             float v = raw[it];
             if (IS_EE) {
                 if (sl.code[it] < 0) v = v * s.taper;  // offshorebndWav (wave_CPU.cpp:24-56)
-                v = fmx(v, 0.000f) / s.sigm;           // sanity + action (wave_CPU.cpp:58-82, 289-309)
+                v = div_uniform(fmx(v, 0.000f), s.sigm, s.rsigm, s.sigm_recip);  // sanity + action (wave_CPU.cpp:58-82, 289-309)
             }
             pl[l] = v;
         }
@@ -612,11 +612,11 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
             bool pc = cb > 0.0f;
             float eu = upw2f(pc ? e0[b] : e0[tp], pc ? e0[tm] : e0[tp2]);
             float fr = cb * eu;
-            ta = b == 0 ? (fr - 0.0f) / p.dtheta : (fr - fl) / p.dtheta;
+            ta = div_uniform(b == 0 ? fr - 0.0f : fr - fl, p.dtheta, p.rdtheta, p.dth_recip);
             fl = fr;
             ctl = ctr;
         } else {
-            ta = (0.0f - fl) / p.dtheta;
+            ta = div_uniform(0.0f - fl, p.dtheta, p.rdtheta, p.dth_recip);
         }
         ea[b] = e0[b] - s.dt * (ea[b] + ta);
     };
@@ -2016,6 +2016,8 @@ void make_kp(xbg_ctx* c) {
     // rows 1 and ny-2 lie in the same tile as them and the two boundary
     // rows lie in different tiles
     k.fold_rows = P.ny > WTY && (P.ny - 1) % WTY != 0 ? 1 : 0;
+    k.rdtheta = 1.0 / (double)k.dtheta;
+    k.dth_recip = k.dtheta > 0.0f && k.dtheta < 2.0f ? 1 : 0;
     k.usesmago = P.usesmago; k.breakmodel = P.breakmodel; k.roller = P.roller;
     k.zb_accum = P.zb_accum; k.swave = P.swave;
     xbg_theta_tables(&P, k.theta, k.cxsth, k.sxnth);
@@ -2027,6 +2029,8 @@ SP make_sp(xbg_ctx* c, double dt, double totaltime) {
     s.totaltime = (float)totaltime;
     s.Trep = c->Trep;
     s.sigm = (float)(2 * PI_ / (double)c->Trep);
+    s.rsigm = 1.0 / (double)s.sigm;
+    s.sigm_recip = s.sigm > 0.0f && s.sigm < 2.0f ? 1 : 0;
     // wave_CPU.cpp:29: taper = min(totaltime/100.0, 1.0) with DECNUM totaltime
     double tp = (double)s.totaltime / 100.0;
     s.taper = (float)(tp < 1.0 ? tp : 1.0);
