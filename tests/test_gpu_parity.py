@@ -268,9 +268,9 @@ def test_lake_at_rest_full_size():
 
 @pytest.mark.parametrize("kw", [dict(), dict(dtheta=5.0, zb_accum=2), dict(roller=0, breakmodel=2),
                                 dict(wci=1.0, hwci=0.5, morfac=10.0), dict(nx=130, ny=37, dtheta=120.0 / 7),
-                                dict(nx=96, ny=33, dtheta=5.0)],
+                                dict(nx=96, ny=33, dtheta=5.0), dict(dx=2.0, dtheta=5.0, zb_accum=2)],
                          ids=["default", "24bins-fp64zb", "noroller-baldock", "wci-morfac10", "130x37-7bins",
-                              "96x33-24bins-rowlaunch"])
+                              "96x33-24bins-rowlaunch", "dx2-24bins"])
 def test_bitwise_identical_to_oracle(kw):
     """The kernels and the oracle evaluate every elementary function with the
     same deterministic implementation and keep the reference's operation
