@@ -536,7 +536,10 @@ XDEV void load_plane(float* raw, const KP& p, const Fld& f, const Slots& sl, int
         raw[it] = *src;
     }
 }
-template <bool IS_EE>
+// RQ: the divisions by sigm and dtheta go through their double reciprocals
+// unconditionally (the host launches it only when both divisors are below 2,
+// where that gives the IEEE quotient: div_uniform).
+template <bool IS_EE, bool RQ>
 XDEV void store_plane(float* pl, const SP& s, const Slots& sl, const float* raw, int tid) {
 #pragma unroll
     for (int it = 0; it < NSLOT; it++) {
@@ -545,7 +548,8 @@ XDEV void store_plane(float* pl, const SP& s, const Slots& sl, const float* raw,
             float v = raw[it];
             if (IS_EE) {
                 if (sl.code[it] < 0) v = v * s.taper;  // offshorebndWav (wave_CPU.cpp:24-56)
-                v = div_uniform(fmx(v, 0.000f), s.sigm, s.rsigm, s.sigm_recip);  // sanity + action (wave_CPU.cpp:58-82, 289-309)
+                v = fmx(v, 0.000f);  // sanity + action (wave_CPU.cpp:58-82, 289-309)
+                v = RQ ? (float)((double)v * s.rsigm) : div_uniform(v, s.sigm, s.rsigm, s.sigm_recip);
             }
             pl[l] = v;
         }
@@ -580,7 +584,7 @@ XDEV void left_speeds(AdvC& a, const KP& p, const float* S, int x0, int y0, int 
 // lanes the left faces of its first column; the faces go through LDS
 // (fx / gy, one buffer per bin parity) and each cell takes its left and
 // bottom face one bin later, behind the bin's barrier.
-template <int NT, bool IS_EE, bool CUR>
+template <int NT, bool IS_EE, bool CUR, bool RQ>
 XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[WTY + 1][BX], const KP& p,
                         const SP& s, const Fld& f, const AdvC& a, const Refr& rf, const Slots& sl, int tid, int cx,
                         int cy, float* ea, float* cmax) {
@@ -612,11 +616,12 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
             bool pc = cb > 0.0f;
             float eu = upw2f(pc ? e0[b] : e0[tp], pc ? e0[tm] : e0[tp2]);
             float fr = cb * eu;
-            ta = div_uniform(b == 0 ? fr - 0.0f : fr - fl, p.dtheta, p.rdtheta, p.dth_recip);
+            const float num = b == 0 ? fr - 0.0f : fr - fl;
+            ta = RQ ? (float)((double)num * p.rdtheta) : div_uniform(num, p.dtheta, p.rdtheta, p.dth_recip);
             fl = fr;
             ctl = ctr;
         } else {
-            ta = div_uniform(0.0f - fl, p.dtheta, p.rdtheta, p.dth_recip);
+            ta = RQ ? (float)((double)(0.0f - fl) * p.rdtheta) : div_uniform(0.0f - fl, p.dtheta, p.rdtheta, p.dth_recip);
         }
         ea[b] = e0[b] - s.dt * (ea[b] + ta);
     };
@@ -625,7 +630,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
     // flight from HBM
     float raw[NSLOT];
     load_plane<IS_EE>(raw, p, f, sl, 0);
-    store_plane<IS_EE>(pl[0], s, sl, raw, tid);
+    store_plane<IS_EE, RQ>(pl[0], s, sl, raw, tid);
     if (nt > 1) load_plane<IS_EE>(raw, p, f, sl, 1);
     lds_barrier();
     const int c0 = (cy + 2) * PW + (cx + 2);
@@ -667,7 +672,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
             asm volatile("" : "+v"(fxo), "+v"(gyo) : : "memory");
             if (th >= 2) asm volatile("" : "+v"(ea[th - 2]) : : "memory");
             if (th + 1 < nt) {
-                store_plane<IS_EE>(pl[(th + 1) & 1], s, sl, raw, tid);
+                store_plane<IS_EE, RQ>(pl[(th + 1) & 1], s, sl, raw, tid);
                 if (th + 2 < nt) load_plane<IS_EE>(raw, p, f, sl, th + 2);
             }
             lds_barrier();
@@ -696,7 +701,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
         asm volatile("" : "+v"(ea[th]) : : "memory");
         if (th >= 2) asm volatile("" : "+v"(ea[th - 2]) : : "memory");
         if (th + 1 < nt) {
-            store_plane<IS_EE>(pl[(th + 1) & 1], s, sl, raw, tid);
+            store_plane<IS_EE, RQ>(pl[(th + 1) & 1], s, sl, raw, tid);
             if (th + 2 < nt) load_plane<IS_EE>(raw, p, f, sl, th + 2);
         }
         lds_barrier();
@@ -717,7 +722,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
 // it also holds the source row (fold_rows), else by k_wave_rows from the
 // post-Euler bins this kernel saves.  Also the block minimum of WAVEDT.
 // Four waves per SIMD (<= 128 VGPRs): two workgroups per CU.
-template <int NT, bool CUR>
+template <int NT, bool CUR, bool RQ>
 __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
@@ -745,7 +750,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     {
         AdvC a = advc_cell<CUR>(p, f, f.cg, x, y);
         if (!CUR) left_speeds(a, p, f.cg, x0, y0, cx, cy);
-        advect_planes<NT, true, CUR>(pl, fx, gy, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
+        advect_planes<NT, true, CUR, RQ>(pl, fx, gy, p, s, f, a, rf, sl, tid, cx, cy, ea, &cmax);
     }
     float wmin = fmn(9999999.9f, p.CFL * p.dtheta / cmax);
     const int side = p.fold_rows && NT > 0 ? -1 : ((y == 1) ? 0 : (y == ny - 2 ? 1 : -1));
@@ -766,7 +771,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     if (p.roller == 1) {
         AdvC a = advc_cell<CUR>(p, f, f.c, x, y);
         if (!CUR) left_speeds(a, p, f.c, x0, y0, cx, cy);
-        advect_planes<NT, false, CUR>(pl, fx, gy, p, s, f, a, rf, sl, tid, cx, cy, ra, nullptr);
+        advect_planes<NT, false, CUR, RQ>(pl, fx, gy, p, s, f, a, rf, sl, tid, cx, cy, ra, nullptr);
         if (valid && side >= 0) {
 #pragma unroll
             for (int th = 0; th < A; th++) {
@@ -2097,6 +2102,13 @@ const char kFieldNames[] =
 
 // defer_force: the flow step follows and forms the wave forces in its first
 // launch
+template <int NT, bool CUR>
+int launch_k_wave(xbg_ctx* c, const SP& s, bool rq, dim3 gw, dim3 bw) {
+    if (rq) LAUNCH(c, "k_wave", (k_wave<NT, CUR, true>), gw, bw, c->kp, s, c->f);
+    else LAUNCH(c, "k_wave", (k_wave<NT, CUR, false>), gw, bw, c->kp, s, c->f);
+    return 0;
+}
+
 int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0, int defer_force = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
@@ -2106,26 +2118,28 @@ int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0, int defer_force = 0) 
     dim3 gw(p.wtiles_x * p.wtiles_y, 1, 1), bw(BX, WTY, 1);
     // wci == 0 (the reference default): the current-free kernel
     const bool cur = p.wci != 0.0f;
+    const bool rq = s.sigm_recip && p.dth_recip;
+    int rc = 0;
     switch (p.nt) {
         case 12:
-            if (cur) LAUNCH(c, "k_wave", (k_wave<12, true>), gw, bw, p, s, c->f);
-            else LAUNCH(c, "k_wave", (k_wave<12, false>), gw, bw, p, s, c->f);
+            rc = cur ? launch_k_wave<12, true>(c, s, rq, gw, bw) : launch_k_wave<12, false>(c, s, rq, gw, bw);
+            if (rc) return rc;
             if (!p.fold_rows) {
                 if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<12, true>), gr, dim3(256), p, s, c->f);
                 else LAUNCH(c, "k_wave_rows", (k_wave_rows<12, false>), gr, dim3(256), p, s, c->f);
             }
             break;
         case 24:
-            if (cur) LAUNCH(c, "k_wave", (k_wave<24, true>), gw, bw, p, s, c->f);
-            else LAUNCH(c, "k_wave", (k_wave<24, false>), gw, bw, p, s, c->f);
+            rc = cur ? launch_k_wave<24, true>(c, s, rq, gw, bw) : launch_k_wave<24, false>(c, s, rq, gw, bw);
+            if (rc) return rc;
             if (!p.fold_rows) {
                 if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<24, true>), gr, dim3(256), p, s, c->f);
                 else LAUNCH(c, "k_wave_rows", (k_wave_rows<24, false>), gr, dim3(256), p, s, c->f);
             }
             break;
         default:
-            if (cur) LAUNCH(c, "k_wave", (k_wave<0, true>), gw, bw, p, s, c->f);
-            else LAUNCH(c, "k_wave", (k_wave<0, false>), gw, bw, p, s, c->f);
+            rc = cur ? launch_k_wave<0, true>(c, s, rq, gw, bw) : launch_k_wave<0, false>(c, s, rq, gw, bw);
+            if (rc) return rc;
             if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<0, true>), gr, dim3(256), p, s, c->f);
             else LAUNCH(c, "k_wave_rows", (k_wave_rows<0, false>), gr, dim3(256), p, s, c->f);
             break;
