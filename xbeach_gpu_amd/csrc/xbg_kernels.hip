@@ -1876,6 +1876,7 @@ struct xbg_ctx {
     xbg_params P;
     KP kp;
     hipStream_t stream = nullptr;
+    hipEvent_t cfl_done = nullptr;  // the CFL partials have reached the host
     size_t N = 0, ND = 0;
     std::vector<void*> allocs;
     Fld f;
@@ -2210,8 +2211,10 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
     c->ND = c->N * P.ntheta;
     make_kp(c);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->cfl_done, hipEventDisableTiming);
     if (e != hipSuccess) {
         g_create_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        if (c->stream) hipStreamDestroy(c->stream);
         delete c;
         return XBG_E_HIP;
     }
@@ -2292,6 +2295,7 @@ int xbg_destroy(xbg_ctx* c) {
     if (c->pt_buf) hipFree(c->pt_buf);
     if (c->host_red) hipHostFree(c->host_red);
     if (c->host_sum) hipHostFree(c->host_sum);
+    if (c->cfl_done) hipEventDestroy(c->cfl_done);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return XBG_OK;
@@ -2564,7 +2568,13 @@ int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
     LAUNCH(c, "k_cfl", k_cfl, dim3(nb), dim3(256), c->kp, c->f, cfl, c->nblk_wave, c->f.red_part);
     HIPC(c, hipMemcpyAsync(c->host_red, c->f.red_part, 2 * (size_t)nb * sizeof(float), hipMemcpyDeviceToHost,
                            c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
+    // the step cannot be launched before dt is known: poll for the copy
+    // rather than block, so the next launches go out as soon as it lands
+    HIPC(c, hipEventRecord(c->cfl_done, c->stream));
+    hipError_t q;
+    while ((q = hipEventQuery(c->cfl_done)) == hipErrorNotReady) {
+    }
+    HIPC(c, q);
     float fmv = 3.0e38f, wmv = 9999999.9f;
     bool bad = false;
     for (int b = 0; b < nb; b++) {
