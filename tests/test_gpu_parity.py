@@ -284,6 +284,28 @@ def test_bitwise_identical_to_oracle(kw):
     d.close()
 
 
+def test_bitwise_with_short_period_forcing():
+    """Trep = 2.5 s makes sigm = 2 pi / Trep > 2: the wave kernel then keeps
+    the IEEE divisions by sigm instead of the double reciprocal (which equals
+    the IEEE quotient only for divisors below 2)."""
+    p, d, o, frc = pair(nx=96, ny=40, dtheta=5.0)
+    Trep = 2.5
+    d.init_state(Trep)
+    o.init_state(Trep)
+    t, dt = 150.0, 0.05
+    for _ in range(30):
+        t += dt
+        St, qo, qn = frc.St(t), frc.qbnd(t - 0.5), frc.qbnd(t + 0.5)
+        d.set_wave_forcing(St, Trep)
+        d.set_flow_forcing(qo, qn, 0.5, 1.0)
+        o.set_forcing(St, qo, qn, Trep, 0.5, 1.0)
+        d.step(dt, t)
+        o.step(dt, t)
+    for k in MAIN + MORE:
+        assert np.array_equal(d.get(k), o.get(k)), k
+    d.close()
+
+
 def test_point_time_series_match_the_fields():
     """xbg_record_points gathers zs and H at the listed cells after each
     step (TSOfile / TSnode); the records equal the full fields read back
