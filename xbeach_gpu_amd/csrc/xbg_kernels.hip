@@ -1419,6 +1419,19 @@ __global__ __launch_bounds__(256) void k_flow_uvvu(KP p, SP s, Fld f, int want_u
 // Sediment transport and bed update
 // ---------------------------------------------------------------------------
 
+// xm::powf(x, y) for a base x > 0 (finite or +inf) whose double log lx =
+// xm::log((double)x) is given: the special cases in xm::pow's order, then
+// its exp(y * log(x)); two powers of one base share the log.
+XDEV float powf_pos(float x, float y, double lx) {
+    const double a = (double)x, yy = (double)y;
+    if (yy == 0.0 || a == 1.0) return 1.0f;
+    if (yy != yy) return (float)xm::nan_();
+    if (yy == 2.0) return (float)(a * a);
+    if (yy == 3.0) return (float)(a * a * a);
+    if (a == xm::inf_()) return yy > 0.0 ? (float)xm::inf_() : 0.0f;
+    return (float)(1.0 * xm::exp(yy * lx));
+}
+
 // Rvr (Sediment_kernel.cu:477-510) at cell i
 XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
     const float m1c = 0.0f, m2c = 0.7939f, m5c = 0.6373f, m6c = 0.5995f;
@@ -1427,6 +1440,14 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
     float kk = (float)(2 * PI_ / (double)(f.c[i] * s.Trep));
     float Ur = 3.0f / 8.0f * sqrtf(2.0f) * f.H[i] * kk / xm::powf(kk * f.hh[i], 3.0f);
     Ur = fmx(Ur, 0.00001f);
+    if (Ur == Ur) {  // Ur >= 1e-5 (or +inf): both powers from one log
+        const double lu = xm::log((double)Ur);
+        float Bm = m1c + (m2c - m1c) / (1.0f + bet * powf_pos(Ur, alph, lu));
+        float B1 = (float)((double)(-90.0f + 90.0f * xm::tanhf(m5c / powf_pos(Ur, m6c, lu))) * PI_ / 180.0);
+        float Sk = Bm * xm::cosf(B1);
+        float As = Bm * xm::sinf(B1);
+        return (p.facsk * Sk - p.facas * As) * f.urms[i];
+    }
     float Bm = m1c + (m2c - m1c) / (1.0f + bet * xm::powf(Ur, alph));
     float B1 = (float)((double)(-90.0f + 90.0f * xm::tanhf(m5c / xm::powf(Ur, m6c))) * PI_ / 180.0);
     float Sk = Bm * xm::cosf(B1);
