@@ -212,7 +212,8 @@ int xbg_cfl_dt(xbg_ctx* ctx, int initial, double* dt_flow, double* dt_wave);
  * resident record series (xbg_set_bnd_series with records dtbc apart from
  * time t0): xbg_cfl_dt, xbg_next_dt with *dt as the previous step, the
  * record of the new time (rec = floor((t - t0)/dtbc), timesincelast =
- * t - (rec*dtbc + t0), no tide or wind), then xbg_step.  On return
+ * t - (rec*dtbc + t0); the offshore sea level and the wind stay as the
+ * last xbg_set_flow_forcing / xbg_use_bnd_record set them), then xbg_step.  On return
  * *totaltime and *dt hold the new time and step.  The same calls a host
  * loop would make, without a return to the caller between the CFL read-back
  * and the launches of the step.
@@ -305,6 +306,60 @@ int xbg_hilbert(int m, const double* x, double* re, double* im);
  * Hermitian spectrum from its non-zero coefficients at 0 < idx < N/2
  * (exposed for tests). */
 int xbg_bnd_synth(int N, int nk, const int* idx, const double* gre, const double* gim, double* out);
+
+/* ------------------------------------------------------------------------
+ * Batched ensemble of independent domains (storm-scenario ensembles).  No
+ * counterpart in the reference, which steps one domain per process
+ * (mainloopGPU, Wave_gpu.cu:610-728, fixed 16x16 blocks at 615-619): here
+ * every phase of the step is one launch over all domains (tile maps of
+ * (domain, tile) entries walked by persistent workgroups), each domain's
+ * time step (CFL minima, the rule of xbg_next_dt, the boundary record of
+ * the new time) is chosen on the device, and a whole step is one hipGraph
+ * replay.  Results equal xbg_advance_resident loops on each domain bit
+ * for bit.
+ * ---------------------------------------------------------------------- */
+typedef struct xbg_ens xbg_ens;
+
+/* Run setup of one domain of an ensemble. */
+typedef struct xbg_ens_domain {
+    double t_start;   /* model time at the start */
+    double dt_prev;   /* previous time step (the +-10 % rate limit) */
+    double endtime;   /* the domain stops at this time (steps are aligned on it) */
+    double t0, dtbc;  /* resident records (xbg_set_bnd_series): record 0 at t0, dtbc apart */
+    double Trep;      /* representative period of the records */
+} xbg_ens_domain;
+
+/*
+ * Build an ensemble over ndom contexts, each loaded (fields, xbg_init_state,
+ * xbg_set_bnd_series; sea level and wind as xbg_set_flow_forcing left them)
+ * and requiring swave = flow = 1 and sedstart <= t_start.  The contexts
+ * stay owned by the caller and must outlive the ensemble; between
+ * xbg_ens_step and xbg_ens_status their buffers belong to the ensemble.
+ */
+int xbg_ens_create(int ndom, xbg_ctx* const* ctxs, const xbg_ens_domain* run, xbg_ens** out);
+/* Enqueue nsteps batched steps (graph replays); returns without waiting.
+ * A domain whose time reached its endtime stops (status 1); a crash
+ * (status 2) or a time past its last record (status 3) stops it too. */
+int xbg_ens_step(xbg_ens* e, int nsteps);
+int xbg_ens_sync(xbg_ens* e);
+/* Per-domain time, last dt, steps taken and status (arrays of ndom, each
+ * may be NULL); waits for the ensemble and points every context at the
+ * buffers its last step wrote, so xbg_get_field reads the current state. */
+int xbg_ens_status(xbg_ens* e, double* t, double* dt, int* steps, int* status);
+/* Autotuner: times every offered launch shape (persistent grid per kernel
+ * class, patch shape of the dispersion and concentration kernels) over
+ * rounds x 12 whole steps, keeps the fastest of each and rebuilds the step
+ * graph.  The timed steps advance the ensemble (a launch shape never
+ * changes a result).  report (may be NULL) receives the measured table. */
+int xbg_ens_tune(xbg_ens* e, int rounds, char* report, size_t n);
+/* Average duration (us) of each kernel class over nsteps steps run with
+ * events (these steps advance the ensemble); us has XBG_ENS_NKERNELS entries
+ * named by xbg_ens_kernel_name. */
+#define XBG_ENS_NKERNELS 13
+int xbg_ens_kernel_times(xbg_ens* e, int nsteps, double* us);
+const char* xbg_ens_kernel_name(int k);
+const char* xbg_ens_last_error(const xbg_ens* e);
+int xbg_ens_destroy(xbg_ens* e);
 
 /* ------------------------------------------------------------------------
  * netCDF output (host code, no netCDF library; writenetcdf.cpp:32-239)

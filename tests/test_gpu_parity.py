@@ -13,8 +13,9 @@ from xbeach_gpu_amd import Domain, ModelCrashed, setups
 pytestmark = pytest.mark.gpu
 
 MAIN = ("zs", "uu", "vv", "ee", "H", "C", "zb")
-MORE = ("rr", "hh", "stdep", "k", "c", "cg", "E", "DR", "R", "urms", "ust", "Sxx", "Syy", "hu", "hv", "qx",
-        "qy", "ueu", "vev", "uv", "vu", "dzb", "zsold", "umeanbnd")
+MORE = ("rr", "hh", "stdep", "k", "c", "cg", "E", "DR", "R", "urms", "ust", "Sxx", "Syy", "Sxy", "hu", "hv",
+        "qx", "qy", "ueu", "vev", "uv", "vu", "dzb", "zsold", "umeanbnd", "rolthick", "Fx", "Fy", "thetamean",
+        "nuh", "dzsdt", "Sus", "Svs", "ceqsg")
 
 
 def rel(a, b):
@@ -41,13 +42,15 @@ def pair(nx=200, ny=50, spun=True, seed=1234, bathy="small", **kw):
     return p, d, o, frc
 
 
-def advance(d, o, frc, n, t=150.0, dt=0.15):
+def advance(d, o, frc, n, t=150.0, dt=0.15, **flow):
+    """n steps of both sides with the same boundary records; flow may set
+    zsbndi, zsbndn (offshore sea level at iy = 0 and ny) and windu, windv."""
     for _ in range(n):
         t += dt
         St, qo, qn = frc.St(t), frc.qbnd(t - 0.5), frc.qbnd(t + 0.5)
         d.set_wave_forcing(St, frc.Trep)
-        d.set_flow_forcing(qo, qn, 0.5, 1.0)
-        o.set_forcing(St, qo, qn, frc.Trep, 0.5, 1.0)
+        d.set_flow_forcing(qo, qn, 0.5, 1.0, **flow)
+        o.set_forcing(St, qo, qn, frc.Trep, 0.5, 1.0, **flow)
         d.step(dt, t)
         o.step(dt, t)
     return t
@@ -166,6 +169,9 @@ def test_advance_resident_matches_host_loop():
     S, Q = frc.records(t0, 12)
     d.set_bnd_series(S, Q)
     d2.set_bnd_series(S, Q)
+    # sea level and wind set once are kept by the one-call loop
+    flow = dict(zsbndi=0.1, zsbndn=0.25, windu=-6.0, windv=4.0)
+    d2.set_flow_forcing(Q[0], Q[1], 0.0, frc.dtbc, **flow)
     t, dt = t0, 0.05
     t2, dt2 = t0, 0.05
     for _ in range(12):
@@ -174,7 +180,7 @@ def test_advance_resident_matches_host_loop():
         t = t + dt
         rec = int(math.floor((t - t0) / frc.dtbc))
         tsl = t - (rec * frc.dtbc + t0)
-        d.use_bnd_record(rec, tsl, frc.dtbc, frc.Trep)
+        d.use_bnd_record(rec, tsl, frc.dtbc, frc.Trep, **flow)
         d.step(dt, t)
         t2, dt2 = d2.advance_resident(t0, frc.dtbc, frc.Trep, t2, dt2)
         assert (t2, dt2) == (t, dt)
@@ -227,7 +233,8 @@ def test_full_size_large_domain():
     p = setups.large_params()
     elev = setups.barred_bathy(p.nx, p.ny, p.dx)
     st = setups.spun_up_state(p, elev, seed=5, Hrms=3.0 / math.sqrt(2))
-    frc = setups.GroupForcing(p, 3.0, 12.0, dir_deg=10.0, s=20.0)
+    frc = setups.JonswapForcing(p, 3.0, 12.0, dir_deg=10.0, s=20.0, seed=0, offdepth=20.0, rtlength=600.0,
+                                t0=250.0)
     d = Domain(p)
     o = Oracle(p, 4)
     for k, v in st.items():
@@ -279,6 +286,39 @@ def test_bitwise_identical_to_oracle(kw):
     nx, ny = kw.pop("nx", 200), kw.pop("ny", 50)
     p, d, o, frc = pair(nx=nx, ny=ny, **kw)
     advance(d, o, frc, 50)
+    for k in MAIN + MORE:
+        assert np.array_equal(d.get(k), o.get(k)), k
+    d.close()
+
+
+@pytest.mark.parametrize("lat", [0.0, 35.0])
+def test_bitwise_with_tide_gradient_wind_and_coriolis(lat):
+    """Non-zero offshore sea level varying alongshore (zsbndi != zsbndn),
+    wind in both directions and, at lat 35, the Coriolis terms: 50 steps
+    bit for bit against the oracle."""
+    p, d, o, frc = pair(nx=120, ny=44, lat=lat)
+    assert (p.fc != 0.0) == (lat != 0.0)
+    advance(d, o, frc, 50, zsbndi=0.3, zsbndn=-0.15, windu=11.0, windv=-7.0)
+    for k in MAIN + MORE:
+        assert np.array_equal(d.get(k), o.get(k)), k
+    d.close()
+
+
+def test_bitwise_with_jonswap_wave_groups():
+    """The seeded JONSWAP wave-group records of the boundary generator
+    (xbg_wave_groups, seed 0) on the small configuration: 50 steps bit for
+    bit against the oracle."""
+    p = setups.small_params()
+    elev = setups.small_bathy(p.nx, p.ny, p.dx)
+    st = setups.spun_up_state(p, elev, seed=21)
+    frc = setups.JonswapForcing(p, 1.5, 10.0, seed=0, offdepth=8.0, rtlength=300.0, t0=100.0)
+    d, o = Domain(p), Oracle(p, 4)
+    for k, v in st.items():
+        d.set(k, v)
+        o.set(k, v)
+    d.init_state(frc.Trep)
+    o.init_state(frc.Trep)
+    advance(d, o, frc, 50, t=110.0, dt=0.17)
     for k in MAIN + MORE:
         assert np.array_equal(d.get(k), o.get(k)), k
     d.close()
@@ -342,7 +382,8 @@ def test_morph_config_size():
     p = setups.large_params(nx=1024, ny=512, dx=3.0, morfac=10.0)
     elev = setups.barred_bathy(p.nx, p.ny, p.dx)
     st = setups.spun_up_state(p, elev, seed=9, Hrms=2.0)
-    frc = setups.GroupForcing(p, 2.8, 11.0, dir_deg=-10.0, s=15.0, seed=3)
+    frc = setups.JonswapForcing(p, 2.8, 11.0, dir_deg=-10.0, s=15.0, seed=0, offdepth=20.0, rtlength=600.0,
+                                t0=450.0)
     d = Domain(p)
     o = Oracle(p, 4)
     for k, v in st.items():

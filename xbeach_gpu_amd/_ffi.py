@@ -130,7 +130,23 @@ PROTOTYPES = {
     "xbg_param_line": (C.c_int, [C.c_char_p, _P, C.c_void_p]),
     "xbg_run_case": (C.c_int, [C.c_char_p, C.c_int]),
     "xbg_run_error": (C.c_char_p, []),
+    "xbg_ens_create": (C.c_int, [C.c_int, C.POINTER(_CTX), C.c_void_p, C.POINTER(C.c_void_p)]),
+    "xbg_ens_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "xbg_ens_sync": (C.c_int, [C.c_void_p]),
+    "xbg_ens_status": (C.c_int, [C.c_void_p, _D, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "xbg_ens_tune": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t]),
+    "xbg_ens_kernel_times": (C.c_int, [C.c_void_p, C.c_int, _D]),
+    "xbg_ens_kernel_name": (C.c_char_p, [C.c_int]),
+    "xbg_ens_last_error": (C.c_char_p, [C.c_void_p]),
+    "xbg_ens_destroy": (C.c_int, [C.c_void_p]),
 }
+
+
+class XbgEnsDomain(C.Structure):
+    """Mirror of ``xbg_ens_domain`` (run setup of one ensemble domain)."""
+
+    _fields_ = [("t_start", C.c_double), ("dt_prev", C.c_double), ("endtime", C.c_double),
+                ("t0", C.c_double), ("dtbc", C.c_double), ("Trep", C.c_double)]
 
 
 MAX_TS = 64  # XBG_MAX_TS

@@ -39,6 +39,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -88,8 +90,13 @@ struct Fld {
     double* red_sum;
 };
 
-XDEV int gx() { return blockIdx.x * BX + threadIdx.x; }
-XDEV int gy() { return blockIdx.y * BY + threadIdx.y; }
+// Block (tile) coordinates of a kernel body: blockIdx in a single-domain
+// launch, the tile-map entry in a batched ensemble launch.
+struct BI {
+    int x, y;
+};
+XDEV int gx(BI bi) { return bi.x * BX + threadIdx.x; }
+XDEV int gy(BI bi) { return bi.y * BY + threadIdx.y; }
 
 // Direction-table entry read
 // through an opaque scalar index: a scalar load at each use instead of
@@ -176,22 +183,23 @@ __global__ __launch_bounds__(1024) void k_bnd_offshore(KP p, SP s, Fld f, int wa
 // dispersion workgroups, whose inputs (hh) and outputs (k, c, cg) it does
 // not touch, and it completes before the wave kernel reads uu / vv.
 constexpr int DPW = 4, DPH = 16;  // dispersion patch of one wavefront
-static_assert(DPW * DPH == 64 && BX * BY == 256, "four one-wavefront patches per workgroup");
-__global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offshore) {
-    int by = blockIdx.y;
+template <int QW = DPW, int QH = DPH>
+XDEV void dispersion_body(const KP& p, const SP& s, const Fld& f, int offshore, BI bi) {
+    static_assert(QW * QH == 64 && BX * BY == 256, "four one-wavefront patches per workgroup");
+    int by = bi.y;
     if (offshore) {
         if (by == 0) {
-            if (blockIdx.x == 0) offshore_wg(p, s, f, 1, threadIdx.x + threadIdx.y * BX, BX * BY);
+            if (bi.x == 0) offshore_wg(p, s, f, 1, threadIdx.x + threadIdx.y * BX, BX * BY);
             return;
         }
         by -= 1;
     }
-    // each wavefront takes a DPW x DPH patch (the block four of them side by
+    // each wavefront takes a QW x QH patch (the block four of them side by
     // side): the iteration count follows the depth, which varies less over
     // a compact patch than along a 64-cell cross-shore row, and a wavefront
     // runs its slowest lane's
     const int tid = threadIdx.x + threadIdx.y * BX, w = tid >> 6, l = tid & 63;
-    int x = blockIdx.x * (4 * DPW) + DPW * w + (l % DPW), y = by * DPH + l / DPW;
+    int x = bi.x * (4 * QW) + QW * w + (l % QW), y = by * QH + l / QW;
     if (x >= p.nx || y >= p.ny) return;
     size_t i = (size_t)x + (size_t)y * p.nx;
     float k, c, cg;
@@ -199,6 +207,9 @@ __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offsh
     f.k[i] = k;
     f.c[i] = c;
     f.cg[i] = cg;
+}
+__global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offshore) {
+    dispersion_body(p, s, f, offshore, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // Wave action after offshorebndWav + sanity + action (wave_CPU.cpp:24-309).
@@ -469,7 +480,7 @@ constexpr int WTHREADS = BX * WTY;
 #ifndef XBG_XCD_REMAP
 #define XBG_XCD_REMAP 1
 #endif
-XDEV int xcd_tile(int b, int ntiles) {
+__host__ __device__ inline int xcd_tile(int b, int ntiles) {
 #if XBG_XCD_REMAP
     const int NX = 8;
     int per = (ntiles + NX - 1) / NX;
@@ -723,7 +734,7 @@ XDEV void advect_planes(float (*pl)[PN], float (*fx)[WTY][BX + 1], float (*gy)[W
 // post-Euler bins this kernel saves.  Also the block minimum of WAVEDT.
 // Four waves per SIMD (<= 128 VGPRs): two workgroups per CU.
 template <int NT, bool CUR, bool RQ>
-__global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
+XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
     __shared__ float pl[2][PN];
@@ -732,7 +743,7 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
     __shared__ float fx[CUR ? 1 : 2][WTY][BX + 1];
     __shared__ float gy[CUR ? 1 : 2][WTY + 1][BX];
     const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
-    const int tile = xcd_tile(blockIdx.x, p.wtiles_x * p.wtiles_y);
+    const int tile = bi.x;  // tile of the domain, after the XCD remap
     const int x0 = (tile % p.wtiles_x) * BX, y0 = (tile / p.wtiles_x) * WTY;
     const int nx = p.nx, ny = p.ny;
     const int x = min(x0 + cx, nx - 1), y = min(y0 + cy, ny - 1);
@@ -850,17 +861,21 @@ __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
         f.red_wave[tile] = m;
     }
 }
+template <int NT, bool CUR, bool RQ>
+__global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
+    wave_body<NT, CUR, RQ>(p, s, f, BI{xcd_tile(blockIdx.x, p.wtiles_x * p.wtiles_y), 0});
+}
 
 // Rows 0 and ny-1 (rollerlatbnd, wave_CPU.cpp:830-871): the post-Euler
 // ee/rr of rows 1 / ny-2 times the cell's wet flag, then the cell's own
 // energy..dissipation, and finally ee after dissipation copied again from
 // rows 1 / ny-2.
 template <int NT, bool CUR>
-__global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
+XDEV void wave_rows_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     constexpr int A = NT > 0 ? NT : MAXT;
     const int nt = NT > 0 ? NT : p.nt;
-    int x = blockIdx.x * 256 + threadIdx.x;
-    int side = blockIdx.y;
+    int x = bi.x * 256 + threadIdx.x;
+    int side = bi.y;
     if (x >= p.nx) return;
     const int nx = p.nx, ny = p.ny;
     const size_t N = p.N;
@@ -878,6 +893,10 @@ __global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
     float Rv, DRv;
     dissip_cell<NT, CUR>(p, s, f, i, e, r, Rv, DRv);
     finish_cell<NT>(p, s, f, i, efin, r, Rv, DRv);
+}
+template <int NT, bool CUR>
+__global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
+    wave_rows_body<NT, CUR>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // wavforceCPU (wave_CPU.cpp:1159-1223) at (x, y)
@@ -923,7 +942,7 @@ XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
     f.ust[i] = ust;
 }
 __global__ __launch_bounds__(256) void k_wave_force(KP p, Fld f) {
-    int x = gx(), y = gy();
+    int x = gx(BI{(int)blockIdx.x, (int)blockIdx.y}), y = gy(BI{(int)blockIdx.x, (int)blockIdx.y});
     if (x >= p.nx || y >= p.ny) return;
     wave_force_cell(p, f, x, y);
 }
@@ -946,8 +965,8 @@ XDEV float wetv_at(const KP& p, const Fld& f, int x, int y) { return (float)((f.
 // udepthmomcont/vdepthmomcont from the previous step's hu/hv.  With force
 // set, also the wave forces of the step (k_wave_force's work: neither reads
 // what the other writes), one launch where a whole step runs both.
-__global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
-    int x = gx(), y = gy();
+XDEV void flow_nuh_body(const KP& p, const Fld& f, int force, BI bi) {
+    int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     if (force) wave_force_cell(p, f, x, y);
     const int nx = p.nx, ny = p.ny;
@@ -973,6 +992,9 @@ __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
     } else {
         f.nuh[i] = fmx(p.nuh, turb);
     }
+}
+__global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
+    flow_nuh_body(p, f, force, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // eulerustep (Flow_kernel.cu:1487-1544) at (x, y) with ududx_adv (752-823),
@@ -1040,11 +1062,14 @@ XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
 
 // eulerustep + latbnd(uu) (Flow_kernel.cu:1778-1809): rows 0 / ny-1 copy
 // the updated rows 1 / ny-2.
-__global__ __launch_bounds__(256) void k_flow_u(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
+XDEV void flow_u_body(const KP& p, const SP& s, const Fld& f, BI bi) {
+    int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
     f.uu_b[ix2(p, x, y)] = eulerustep_at(p, s, f, x, ys);
+}
+__global__ __launch_bounds__(256) void k_flow_u(KP p, SP s, Fld f) {
+    flow_u_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // viscov1 (Flow_kernel.cu:1399-1429) at (x, y)
@@ -1122,11 +1147,14 @@ XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
 
 // eulervstep + vvlatbnd (Flow_kernel.cu:1812-1844): rows 0 / ny-2 copy the
 // updated rows 1 / ny-3.
-__global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
-    int x = gx(), y = gy();
+XDEV void flow_v_body(const KP& p, const SP& s, const Fld& f, BI bi) {
+    int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     int ys = (y == 0) ? 1 : (y == p.ny - 2 ? p.ny - 3 : y);
     f.vv_b[ix2(p, x, y)] = eulervstep_at(p, s, f, x, ys);
+}
+__global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
+    flow_v_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // wrkuu2Ocorr (1960-2044), wrkvv2Ocorr (2046-2128), uu2Ocorr (2210-2245),
@@ -1143,12 +1171,12 @@ constexpr int CY = 16;
 constexpr int DW = BX + 6, DH = CY + 6;  // data tile:  x0-3 .. x0+BX+2, y0-3 .. y0+CY+2
 constexpr int KW = BX + 2, KH = CY + 2;  // flux tile:  x0-1 .. x0+BX,   y0-1 .. y0+CY
 enum { T_ZS, T_ZB, T_QX, T_QY, T_U, T_UO, T_V, T_VO, T_N };
-__global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
+XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     __shared__ float dt_[T_N][DH][DW];
     __shared__ float kt[4][KH][KW];
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * CY;
+    const int x0 = bi.x * BX, y0 = bi.y * CY;
     const float* src[T_N] = {f.zs, f.zb, f.qx, f.qy, f.uu_b, f.uuold, f.vv_b, f.vvold};
     for (int l = tid; l < DW * DH; l += BX * CY) {
         int r = l / DW, c = l - r * DW;
@@ -1265,6 +1293,9 @@ __global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
     else h = fmx(fmx(zsi, zst) - hb, p.eps);
     f.hv[i] = fmx(h, 0.0f);
 }
+__global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
+    flow_uvcorr_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+}
 
 // CalcQFlow (Flow_kernel.cu:1917-1958) + continuity (1626-1681) at (x, y):
 // the water level after continuity; *dzdt_out gets dzsdt.
@@ -1294,12 +1325,12 @@ XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, f
 // zs buffer: other tiles still read the old one in step 1.
 constexpr int ZW = BX + 4, ZH = CY + 5;  // level tile: x0-2 .. x0+BX+1, y0-3 .. y0+CY+1
 constexpr int FW = BX + 1, FH = CY + 2;  // flux tile:  x0-1 .. x0+BX-1, y0-2 .. y0+CY-1
-__global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
+XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     __shared__ float zt[ZH][ZW];
     __shared__ float w1t[FH][FW], w2t[FH][FW];
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * CY;
+    const int x0 = bi.x * BX, y0 = bi.y * CY;
     for (int l = tid; l < ZW * ZH; l += BX * CY) {
         int r = l / ZW, c = l - r * ZW;
         int ux = x0 - 2 + c, uy = y0 - 3 + r;
@@ -1363,6 +1394,9 @@ __global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
     f.qy[i] = qyv;
     f.hh[i] = fmx(f.zb[i] + z, p.eps);
 }
+__global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
+    flow_zsall_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+}
 
 // calcuvvu (Flow_kernel.cu:467-541) at (x, y)
 XDEV void calcuvvu_at(const KP& p, const Fld& f, int x, int y, float* o) {
@@ -1396,8 +1430,8 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i);
 // 1 / ny-2.  Also saves uuold/vvold/zsold.  With want_ua, Rvr's ua of the
 // cell for the sediment step that follows (its inputs c, H, hh, urms are
 // final here): once per cell instead of per cell and halo in k_sed_flux.
-__global__ __launch_bounds__(256) void k_flow_uvvu(KP p, SP s, Fld f, int want_ua) {
-    int x = gx(), y = gy();
+XDEV void flow_uvvu_body(const KP& p, const SP& s, const Fld& f, int want_ua, BI bi) {
+    int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     size_t i = ix2(p, x, y);
     int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
@@ -1413,6 +1447,9 @@ __global__ __launch_bounds__(256) void k_flow_uvvu(KP p, SP s, Fld f, int want_u
     f.vmageu[i] = o[4];
     f.vmagev[i] = o[5];
     if (want_ua) f.ua[i] = rvr_at(p, s, f, i);
+}
+__global__ __launch_bounds__(256) void k_flow_uvvu(KP p, SP s, Fld f, int want_ua) {
+    flow_uvvu_body(p, s, f, want_ua, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // ---------------------------------------------------------------------------
@@ -1459,7 +1496,7 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
 // live output), Rvr and Susp (534-717).  ceqbg = (1-bulk)*... = 0 with
 // bulk = 1 (Sediment_kernel.cu:196, 320), so the bed-load fluxes Sub/Svb
 // are zero and only Sus/Svs are formed.
-__global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f, int ua_ready) {
+XDEV void sed_flux_body(const KP& p, const SP& s, const Fld& f, int ua_ready, BI bi) {
     // Rvr's ua at the cell and at its +x / +y neighbours: read from f.ua when
     // k_flow_uvvu formed it (ua_ready), else evaluated once per cell into an
     // LDS tile (+1 column, +1 row) at clamped coordinates, so tile(cx+1, cy)
@@ -1467,7 +1504,7 @@ __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f, int ua_read
     __shared__ float uat_[BY + 1][BX + 1];
     const int nx = p.nx, ny = p.ny;
     const int cx = threadIdx.x, cy = threadIdx.y, tid = cx + cy * BX;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    const int x0 = bi.x * BX, y0 = bi.y * BY;
     if (!ua_ready) {
         int xc = min(x0 + cx, nx - 1), yc = min(y0 + cy, ny - 1);
         uat_[cy][cx] = rvr_at(p, s, f, ix2(p, xc, yc));
@@ -1527,6 +1564,9 @@ __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f, int ua_read
     const float facsl = 1.6f;
     f.Sus[i] = p.sus * (cu * urep * hu - Dc * hu * dcsdx - facsl * cu * vmagu * hu * dzbdx) * wu;
     f.Svs[i] = p.sus * (cv * vrep * hv - Dc * hv * dcsdy - facsl * cv * vmagv * hv * dzbdy) * wv;
+}
+__global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f, int ua_ready) {
+    sed_flux_body(p, s, f, ua_ready, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // Sbvr (Sediment_kernel.cu:167-326) + Erosus (512-531) + Conc (721-779) at
@@ -1604,10 +1644,11 @@ XDEV void zsrc(const KP& p, int x, int y, int& sx, int& sy) {
 // workgroup): the mobilised-sediment branch and the boundary re-evaluations
 // of column 0 are then taken by fewer wavefronts.
 constexpr int SPW = 16, SPH = 4;
-__global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) {
-    static_assert(SPW * SPH == 64 && BX * BY == 256, "four one-wavefront patches per workgroup");
+template <int QW = SPW, int QH = SPH>
+XDEV void sed_conc_body(const KP& p, const SP& s, const Fld& f, int morph, BI bi) {
+    static_assert(QW * QH == 64 && BX * BY == 256, "four one-wavefront patches per workgroup");
     const int tid = threadIdx.x + threadIdx.y * BX, w = tid >> 6, l = tid & 63;
-    int x = blockIdx.x * (4 * SPW) + SPW * w + (l % SPW), y = blockIdx.y * SPH + l / SPW;
+    int x = bi.x * (4 * QW) + QW * w + (l % QW), y = bi.y * QH + l / QW;
     if (x >= p.nx || y >= p.ny) return;
     const int ny = p.ny;
     size_t i = ix2(p, x, y);
@@ -1650,6 +1691,9 @@ __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) 
     } else {
         f.zb_b[i] = f.zb[is] + dzs;
     }
+}
+__global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) {
+    sed_conc_body(p, s, f, morph, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // avalanching (Sediment_kernel.cu:1020-1137) at (x, y): slope-limited
@@ -1696,11 +1740,11 @@ XDEV void aval_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& b
 // scatters the exchange into both cells without atomics; the change of each
 // cell is gathered here as (dzbx + dzby)_i - dzbx_{i-1} - dzby_{i-nx}.
 constexpr int MW = BX + 1, MH = BY + 2;  // exchange tile: x0-1 .. x0+BX-1, y0-2 .. y0+BY-1
-__global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
+XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     __shared__ float ex[MH][MW], ey[MH][MW];
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
-    const int x0 = blockIdx.x * BX, y0 = blockIdx.y * BY;
+    const int x0 = bi.x * BX, y0 = bi.y * BY;
     for (int l = tid; l < MW * MH; l += BX * BY) {
         int r = l / MW, c = l - r * MW;
         int hx = x0 - 1 + c, hy = y0 - 2 + r;
@@ -1744,6 +1788,9 @@ __global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
         f.zb[i] = f.zb_b[is] + ds;
     }
 }
+__global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
+    sed_morph_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+}
 
 // ---------------------------------------------------------------------------
 // Init, reductions, output averages
@@ -1751,7 +1798,7 @@ __global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
 
 // depthhu / depthhv on the initial state (Wave_gpu.cu init order).
 __global__ __launch_bounds__(256) void k_init_depth(KP p, Fld f) {
-    int x = gx(), y = gy();
+    int x = gx(BI{(int)blockIdx.x, (int)blockIdx.y}), y = gy(BI{(int)blockIdx.x, (int)blockIdx.y});
     if (x >= p.nx || y >= p.ny) return;
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
     const float *zs = f.zs, *zb = f.zb;
@@ -2452,7 +2499,8 @@ int xbg_use_bnd_record(xbg_ctx* c, int rec, double timesincelast, double timenex
 // One iteration of the main loop (Wave_gpu.cu:610-728) on a resident record
 // series: CFL limits, the time-step rule, the record of the new time and the
 // step, without returning to the caller between the dt read-back and the
-// launches of the step.
+// launches of the step.  The offshore sea level and the wind stay as the
+// last xbg_set_flow_forcing / xbg_use_bnd_record left them.
 int xbg_advance_resident(xbg_ctx* c, double t0, double dtbc, double Trep, double nextoutputtime, double endtime,
                          double* totaltime, double* dt) {
     if (!c || !totaltime || !dt || !(dtbc > 0.0)) return XBG_E_ARG;
@@ -2464,7 +2512,8 @@ int xbg_advance_resident(xbg_ctx* c, double t0, double dtbc, double Trep, double
     const double t = *totaltime + dtn;
     const int rec = (int)std::floor((t - t0) / dtbc);
     const double tsl = t - (rec * dtbc + t0);
-    if ((rc = xbg_use_bnd_record(c, rec, tsl, dtbc, Trep, 0.0, 0.0, 0.0, 0.0)) != XBG_OK) return rc;
+    if ((rc = xbg_use_bnd_record(c, rec, tsl, dtbc, Trep, c->zsbndi, c->zsbndn, c->windu, c->windv)) != XBG_OK)
+        return rc;
     if ((rc = xbg_step(c, dtn, t)) != XBG_OK) return rc;
     *totaltime = t;
     *dt = dtn;
@@ -2685,6 +2734,849 @@ int xbg_profile_report(xbg_ctx* c, char* buf, size_t n) {
     size_t m = out.size() < n - 1 ? out.size() : n - 1;
     std::memcpy(buf, out.data(), m);
     buf[m] = 0;
+    return XBG_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched ensemble of independent domains
+//
+// The domains of a storm-scenario ensemble (uneven grids, own forcing, own
+// CFL time step) advance together: every phase of the step is ONE launch
+// over all domains, whose workgroups walk a tile map (domain, tile) with a
+// grid stride (persistent workgroups when the tuned grid is smaller than
+// the map).  Each workgroup reads its domain's constants, step scalars and
+// buffer pointers from a descriptor table in device memory.  The time step
+// of every domain (its CFL minima, the rule of mainloopGPU, the boundary
+// record of the new time) is chosen on the device by k_ens_dt, so whole
+// steps are enqueued without a host round trip and captured in a hipGraph.
+// The per-cell work is the single-domain kernels' bodies: results equal a
+// loop of xbg_advance_resident calls on each domain bit for bit.
+// ===========================================================================
+namespace {
+
+// Phases of a step: which ping-pong halves a kernel sees.  0: step start,
+// 1: after the wave kernel (ee, rr flipped), 2: after the water-level pass
+// (zs flipped), 3: after the concentration pass (C flipped) = the next
+// step's start.
+enum { PH_START = 0, PH_WAVE = 1, PH_ZS = 2, PH_END = 3 };
+
+enum { ENS_ACTIVE = 0, ENS_DONE = 1, ENS_CRASH = 2, ENS_NOREC = 3 };
+
+struct DomDesc {
+    KP p;
+    SP s;                  // written by k_ens_dt for the step being run
+    Fld f[2][4];           // [parity of the state at step start][phase]
+    const double* Stfile;  // resident boundary records
+    const double* qfile;
+    int nrec, nst, nq, sed, morph, with_wave;
+    double t0, dtbc, endtime, nextout;
+    float Trep, zsbndi, zsbndn, windu, windv;
+    int cfl_first, cfl_count, nwave;
+    // run state, written by k_ens_dt
+    double t, dt;
+    int steps, par, status, pad_;
+};
+
+struct TileEnt {
+    int dom, bx, by, pad_;
+};
+
+typedef const DomDesc* __restrict__ DTab;
+typedef const TileEnt* __restrict__ TMap;
+
+// grid-stride walk over the tile map; the barrier keeps a workgroup's next
+// tile from overwriting LDS its previous tile still reads
+#define ENS_FOR_TILES(map, ntiles)                                         \
+    for (int t_ = blockIdx.x; t_ < (ntiles); t_ += gridDim.x)
+#define ENS_TILE_END __syncthreads()
+
+template <int QW, int QH>
+__global__ __launch_bounds__(256) void kb_dispersion(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) dispersion_body<QW, QH>(D.p, D.s, D.f[D.par][PH_START], 1, BI{e.bx, e.by});
+        ENS_TILE_END;
+    }
+}
+
+// Entries with bx < 0 pad a domain's run of tiles to a multiple of 8 (so
+// that map position modulo 8, the XCD of the workgroup, is the XCD the
+// single-domain launch gives the tile) and are skipped.  One tile per
+// workgroup: the wave kernel sits at the 128-VGPR limit of its two
+// workgroups per CU, and a tile loop around it spills.
+template <int NT, bool CUR, bool RQ>
+__global__ __launch_bounds__(WTHREADS, 4) void kb_wave(DTab tab, TMap map, int ntiles) {
+    const TileEnt e = map[blockIdx.x];
+    if (e.bx < 0) return;
+    const DomDesc& D = tab[e.dom];
+    if (D.status == ENS_ACTIVE) wave_body<NT, CUR, RQ>(D.p, D.s, D.f[D.par][PH_START], BI{e.bx, 0});
+}
+
+template <int NT, bool CUR>
+__global__ __launch_bounds__(256) void kb_wave_rows(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) wave_rows_body<NT, CUR>(D.p, D.s, D.f[D.par][PH_START], BI{e.bx, e.by});
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_flow_nuh(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) flow_nuh_body(D.p, D.f[D.par][PH_WAVE], 1, BI{e.bx, e.by});
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_flow_u(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) flow_u_body(D.p, D.s, D.f[D.par][PH_WAVE], BI{e.bx, e.by});
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_flow_v(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) flow_v_body(D.p, D.s, D.f[D.par][PH_WAVE], BI{e.bx, e.by});
+    }
+}
+
+__global__ __launch_bounds__(BX * CY) void kb_flow_uvcorr(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) flow_uvcorr_body(D.p, D.s, D.f[D.par][PH_WAVE], BI{e.bx, e.by});
+        ENS_TILE_END;
+    }
+}
+
+__global__ __launch_bounds__(BX * CY) void kb_flow_zsall(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) flow_zsall_body(D.p, D.s, D.f[D.par][PH_WAVE], BI{e.bx, e.by});
+        ENS_TILE_END;
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_flow_uvvu(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) flow_uvvu_body(D.p, D.s, D.f[D.par][PH_ZS], D.sed, BI{e.bx, e.by});
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_sed_flux(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE && D.sed) sed_flux_body(D.p, D.s, D.f[D.par][PH_ZS], 1, BI{e.bx, e.by});
+        ENS_TILE_END;
+    }
+}
+
+template <int QW, int QH>
+__global__ __launch_bounds__(256) void kb_sed_conc(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE && D.sed)
+            sed_conc_body<QW, QH>(D.p, D.s, D.f[D.par][PH_ZS], D.morph, BI{e.bx, e.by});
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_sed_morph(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE && D.sed && D.morph)
+            sed_morph_body(D.p, D.s, D.f[D.par][PH_END], BI{e.bx, e.by});
+        ENS_TILE_END;
+    }
+}
+
+// FLOWDT (Wave_kernel.cu:161-170) of the state at step start, one partial
+// minimum per tile of 2048 cells (NaN when a cell gives a non-finite or
+// zero step), written to part[tile].
+constexpr int CFL_TILE = 2048;
+__global__ __launch_bounds__(256) void kb_cfl(DTab tab, TMap map, int ntiles, float* part) {
+    __shared__ float rf[4];
+    __shared__ int rb[4];
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        if (D.status == ENS_ACTIVE) {
+            const KP& p = D.p;
+            const float* hh = D.f[D.par][PH_END].hh;
+            const size_t b0 = (size_t)e.bx * CFL_TILE + threadIdx.x;
+            float h[CFL_TILE / 256];
+#pragma unroll
+            for (int k = 0; k < CFL_TILE / 256; k++) {
+                size_t i = b0 + (size_t)k * 256;
+                h[k] = i < p.N ? hh[i] : 0.0f;
+            }
+            float v = 3.0e38f;
+            int bad = 0;
+#pragma unroll
+            for (int k = 0; k < CFL_TILE / 256; k++) {
+                if (b0 + (size_t)k * 256 >= p.N) continue;
+                float d = p.CFL * p.dx / (sqrtf(9.81f * h[k]));
+                if (!(d == d) || d == INFINITY || d == 0.0f) bad = 1;
+                else v = fmn(v, d);
+            }
+            for (int o = 32; o > 0; o >>= 1) v = fmn(v, __shfl_xor(v, o, 64));
+            bad = __any(bad);
+            if ((threadIdx.x & 63) == 0) {
+                rf[threadIdx.x >> 6] = v;
+                rb[threadIdx.x >> 6] = bad;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                v = fmn(fmn(rf[0], rf[1]), fmn(rf[2], rf[3]));
+                part[t_] = (rb[0] | rb[1] | rb[2] | rb[3]) ? __builtin_nanf("") : v;
+            }
+        }
+        ENS_TILE_END;
+    }
+}
+
+// The time step of each domain (one workgroup per domain): the minima of
+// its flow partials and of its wave kernel's tile minima (xbg_cfl_dt), the
+// rule of xbg_next_dt (Wave_gpu.cu:621-698, the same float/double
+// operations), the boundary record of the new time with St interpolated in
+// double and qbnd rounded to float (k_bnd_interp, Wavestep.cu:722-745), and
+// the step scalars of make_sp.  Domains that reached their end time stop.
+__global__ __launch_bounds__(256) void k_ens_dt(DomDesc* tab, const float* part) {
+    DomDesc& D = tab[blockIdx.x];
+    __shared__ float rf[4], rw[4];
+    __shared__ int rb[4];
+    __shared__ int go, rec_s, par_s;
+    __shared__ double tsl_s;
+    const int tid = threadIdx.x;
+    if (D.status != ENS_ACTIVE) return;
+    const Fld& f = D.f[D.par ^ 1][PH_START];  // the state after the last step
+    float v = 3.0e38f, w = 9999999.9f;
+    int bad = 0;
+    for (int j = tid; j < D.cfl_count; j += 256) {
+        float a = part[D.cfl_first + j];
+        if (!(a == a)) bad = 1;
+        else v = fmn(v, a);
+    }
+    for (int j = tid; j < D.nwave; j += 256) w = fmn(w, f.red_wave[j]);
+    for (int o = 32; o > 0; o >>= 1) {
+        v = fmn(v, __shfl_xor(v, o, 64));
+        w = fmn(w, __shfl_xor(w, o, 64));
+    }
+    bad = __any(bad);
+    if ((tid & 63) == 0) {
+        rf[tid >> 6] = v;
+        rw[tid >> 6] = w;
+        rb[tid >> 6] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        go = 0;
+        const double t = D.t;
+        if (!(t < D.endtime)) {
+            D.status = ENS_DONE;
+        } else {
+            float fmv = fmn(fmn(rf[0], rf[1]), fmn(rf[2], rf[3]));
+            float wmv = fmn(fmn(rw[0], rw[1]), fmn(rw[2], rw[3]));
+            double dtflow = (rb[0] | rb[1] | rb[2] | rb[3]) ? (double)__builtin_nanf("") : (double)fmv;
+            // xbg_next_dt
+            float wdt = (float)D.dt;
+            double dt = (float)dtflow * 0.5;
+            if (dt == 0.0 || isinf(dt) || isnan(dt)) {
+                D.status = ENS_CRASH;
+            } else {
+                if (D.with_wave && t > 0.0) {
+                    double dtwave = (float)(double)wmv * 0.5;
+                    dt = dt < dtwave ? dt : dtwave;
+                }
+                float diffdt = (float)(dt - wdt);
+                if (fabsf(diffdt) / wdt > 0.1) dt = wdt * (1 + (diffdt) / fabsf(diffdt) * 0.1);
+                double nextstop = D.nextout < D.endtime ? D.nextout : D.endtime;
+                if (ceil((nextstop - t) / dt) > 0.0) dt = (nextstop - t) / ceil((nextstop - t) / dt);
+                const double tn = t + dt;
+                const int rec = (int)floor((tn - D.t0) / D.dtbc);
+                const double tsl = tn - (rec * D.dtbc + D.t0);
+                if (rec < 0 || rec + 1 >= D.nrec) {
+                    D.status = ENS_NOREC;
+                } else {
+                    SP& s = D.s;
+                    s.dt = (float)dt;
+                    s.totaltime = (float)tn;
+                    s.Trep = D.Trep;
+                    s.sigm = (float)(2 * PI_ / (double)D.Trep);
+                    s.rsigm = 1.0 / (double)s.sigm;
+                    s.sigm_recip = s.sigm > 0.0f && s.sigm < 2.0f ? 1 : 0;
+                    double tp = (double)s.totaltime / 100.0;
+                    s.taper = (float)(tp < 1.0 ? tp : 1.0);
+                    s.tsl = (float)tsl;
+                    s.tnext = (float)D.dtbc;
+                    s.zsbndi = D.zsbndi;
+                    s.zsbndn = D.zsbndn;
+                    s.windu = D.windu;
+                    s.windv = D.windv;
+                    s.signwu = ((double)D.windu > 0.0) - ((double)D.windu < 0.0);
+                    s.signwv = ((double)D.windv > 0.0) - ((double)D.windv < 0.0);
+                    D.t = tn;
+                    D.dt = dt;
+                    D.par = D.steps & 1;
+                    par_s = D.steps & 1;
+                    D.steps = D.steps + 1;
+                    rec_s = rec;
+                    tsl_s = tsl;
+                    go = 1;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (!go) return;
+    // boundary record of the new time into this step's St, qbndold, qbndnew
+    const Fld& g = D.f[par_s][PH_START];
+    const int rec = rec_s;
+    const double tsl = tsl_s, tnext = D.dtbc;
+    float* St = (float*)g.St;
+    float* qo = (float*)g.qbo;
+    float* qn = (float*)g.qbn;
+    for (int j = tid; j < D.nst; j += 256) {
+        double so = D.Stfile[(size_t)rec * D.nst + j];
+        double sn = D.Stfile[(size_t)(rec + 1) * D.nst + j];
+        St[j] = (float)(so + (tsl) / (tnext) * (sn - so));
+    }
+    for (int j = tid; j < D.nq; j += 256) {
+        qo[j] = (float)D.qfile[(size_t)rec * D.nq + j];
+        qn[j] = (float)D.qfile[(size_t)(rec + 1) * D.nq + j];
+    }
+}
+
+// Kernels of a step whose launch shape the tuner chooses.
+enum EnsK { EK_CFL, EK_DISP, EK_WAVE, EK_ROWS, EK_NUH, EK_U, EK_V, EK_UVCORR, EK_ZSALL, EK_UVVU, EK_SFLUX, EK_SCONC,
+            EK_SMORPH, EK_N };
+const char* const kEnsName[EK_N] = {"kb_cfl", "kb_dispersion", "kb_wave", "kb_wave_rows", "kb_flow_nuh",
+                                    "kb_flow_u", "kb_flow_v", "kb_flow_uvcorr", "kb_flow_zsall", "kb_flow_uvvu",
+                                    "kb_sed_flux", "kb_sed_conc", "kb_sed_morph"};
+// persistent grid sizes offered to the tuner, in workgroups per CU (0: one
+// workgroup per tile)
+const int kGridPerCU[] = {0, 2, 4, 8};
+constexpr int N_GRID = 4;
+// patch shapes (one wavefront) offered for the dispersion and concentration
+// kernels
+const int kShapeW[] = {4, 8, 16};
+constexpr int N_SHAPE = 3;
+
+struct TileMap {
+    TileEnt* d = nullptr;  // device
+    int n = 0;
+};
+
+struct WaveGroup {
+    int nt;
+    bool cur, rq;
+    TileMap map;
+};
+
+}  // namespace
+
+struct xbg_ens {
+    int nd = 0;
+    std::vector<xbg_ctx*> ctx;
+    std::vector<DomDesc> h;       // host copy of the descriptors (as built)
+    std::vector<int> base_cur, base_zcur, base_ccur;
+    DomDesc* dtab = nullptr;
+    hipStream_t stream = nullptr;
+    TileMap m_cfl, m_2d, m_corr, m_rows[3];  // rows per nt class (12, 24, other)
+    TileMap m_disp[N_SHAPE], m_conc[N_SHAPE];
+    std::vector<WaveGroup> waves;
+    std::vector<WaveGroup> rows;
+    float* part = nullptr;
+    int ncu = 256;
+    // launch choices (tuned)
+    int grid_sel[EK_N] = {};   // index into kGridPerCU
+    int disp_shape = 0, conc_shape = 2;  // 4x16 dispersion, 16x4 concentration (the single-domain shapes)
+    // graph of one step for the current choices
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool prof = false;
+    std::vector<hipEvent_t> ev;
+    std::string err;
+};
+
+namespace {
+
+int efail(xbg_ens* e, int code, const std::string& msg) {
+    if (e) e->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define EHIP(e, call)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return efail(e, XBG_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+Fld fld_state(xbg_ctx* c, int cur, int zcur, int ccur) {
+    Fld f = c->f;
+    f.ee_in = c->ee[cur];
+    f.ee_out = c->ee[1 - cur];
+    f.rr_in = c->rr[cur];
+    f.rr_out = c->rr[1 - cur];
+    f.C = c->Cb[ccur];
+    f.C_out = c->Cb[1 - ccur];
+    f.zs = c->zsb[zcur];
+    f.zs_out = c->zsb[1 - zcur];
+    f.St = c->St;
+    f.dirs = c->dirs;
+    f.qbo = c->qbo;
+    f.qbn = c->qbn;
+    return f;
+}
+
+int upload_map(xbg_ens* e, TileMap& m, const std::vector<TileEnt>& v) {
+    if (m.d) hipFree(m.d);
+    m.d = nullptr;
+    m.n = (int)v.size();
+    if (v.empty()) return 0;
+    EHIP(e, hipMalloc((void**)&m.d, v.size() * sizeof(TileEnt)));
+    EHIP(e, hipMemcpy(m.d, v.data(), v.size() * sizeof(TileEnt), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// tiles of a (gx x gy) grid of every listed domain, domain after domain
+std::vector<TileEnt> grid_map(const std::vector<int>& doms, const std::vector<int>& gxv,
+                              const std::vector<int>& gyv) {
+    std::vector<TileEnt> v;
+    for (size_t k = 0; k < doms.size(); k++)
+        for (int by = 0; by < gyv[k]; by++)
+            for (int bx = 0; bx < gxv[k]; bx++) v.push_back(TileEnt{doms[k], bx, by, 0});
+    return v;
+}
+
+int grid_of(const xbg_ens* e, int k, int ntiles) {
+    int per = kGridPerCU[e->grid_sel[k]];
+    if (per == 0) return ntiles;
+    int g = per * e->ncu;
+    return g < ntiles ? g : ntiles;
+}
+
+template <int NT, bool CUR>
+void launch_wave_group(xbg_ens* e, const WaveGroup& w, int grid) {
+    grid = w.map.n;  // one workgroup per tile (kb_wave has no tile loop)
+    if (w.rq) kb_wave<NT, CUR, true><<<dim3(grid), dim3(BX, WTY), 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+    else kb_wave<NT, CUR, false><<<dim3(grid), dim3(BX, WTY), 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+}
+
+void ens_ev(xbg_ens* e, int slot) {
+    if (e->prof) hipEventRecord(e->ev[slot], e->stream);
+}
+
+// One batched step: the launches, in the order of xbg_cfl_dt + xbg_step.
+// With profiling on, events bracket each kernel class (slots 2k, 2k+1).
+int ens_launch_step(xbg_ens* e) {
+    const dim3 b2(BX, BY), bc(BX, CY);
+    if (e->m_cfl.n) {
+        ens_ev(e, 2 * EK_CFL);
+        kb_cfl<<<dim3(grid_of(e, EK_CFL, e->m_cfl.n)), dim3(256), 0, e->stream>>>(e->dtab, e->m_cfl.d, e->m_cfl.n,
+                                                                                 e->part);
+        ens_ev(e, 2 * EK_CFL + 1);
+    }
+    k_ens_dt<<<dim3(e->nd), dim3(256), 0, e->stream>>>(e->dtab, e->part);
+    {
+        const TileMap& m = e->m_disp[e->disp_shape];
+        int g = grid_of(e, EK_DISP, m.n);
+        ens_ev(e, 2 * EK_DISP);
+        switch (e->disp_shape) {
+            case 0: kb_dispersion<4, 16><<<dim3(g), b2, 0, e->stream>>>(e->dtab, m.d, m.n); break;
+            case 1: kb_dispersion<8, 8><<<dim3(g), b2, 0, e->stream>>>(e->dtab, m.d, m.n); break;
+            default: kb_dispersion<16, 4><<<dim3(g), b2, 0, e->stream>>>(e->dtab, m.d, m.n); break;
+        }
+        ens_ev(e, 2 * EK_DISP + 1);
+    }
+    ens_ev(e, 2 * EK_WAVE);
+    for (const WaveGroup& w : e->waves) {
+        int g = grid_of(e, EK_WAVE, w.map.n);
+        if (w.nt == 12) {
+            if (w.cur) launch_wave_group<12, true>(e, w, g);
+            else launch_wave_group<12, false>(e, w, g);
+        } else if (w.nt == 24) {
+            if (w.cur) launch_wave_group<24, true>(e, w, g);
+            else launch_wave_group<24, false>(e, w, g);
+        } else {
+            if (w.cur) launch_wave_group<0, true>(e, w, g);
+            else launch_wave_group<0, false>(e, w, g);
+        }
+    }
+    ens_ev(e, 2 * EK_WAVE + 1);
+    ens_ev(e, 2 * EK_ROWS);
+    for (const WaveGroup& w : e->rows) {
+        int g = grid_of(e, EK_ROWS, w.map.n);
+        dim3 b(256);
+        if (w.nt == 12) {
+            if (w.cur) kb_wave_rows<12, true><<<dim3(g), b, 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+            else kb_wave_rows<12, false><<<dim3(g), b, 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+        } else if (w.nt == 24) {
+            if (w.cur) kb_wave_rows<24, true><<<dim3(g), b, 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+            else kb_wave_rows<24, false><<<dim3(g), b, 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+        } else {
+            if (w.cur) kb_wave_rows<0, true><<<dim3(g), b, 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+            else kb_wave_rows<0, false><<<dim3(g), b, 0, e->stream>>>(e->dtab, w.map.d, w.map.n);
+        }
+    }
+    ens_ev(e, 2 * EK_ROWS + 1);
+    const TileMap& m2 = e->m_2d;
+    const TileMap& mc = e->m_corr;
+#define EL(k, kern, blk, m)                                                                                 \
+    do {                                                                                                    \
+        ens_ev(e, 2 * (k));                                                                                 \
+        kern<<<dim3(grid_of(e, (k), (m).n)), blk, 0, e->stream>>>(e->dtab, (m).d, (m).n);                   \
+        ens_ev(e, 2 * (k) + 1);                                                                             \
+    } while (0)
+    EL(EK_NUH, kb_flow_nuh, b2, m2);
+    EL(EK_U, kb_flow_u, b2, m2);
+    EL(EK_V, kb_flow_v, b2, m2);
+    EL(EK_UVCORR, kb_flow_uvcorr, bc, mc);
+    EL(EK_ZSALL, kb_flow_zsall, bc, mc);
+    EL(EK_UVVU, kb_flow_uvvu, b2, m2);
+    EL(EK_SFLUX, kb_sed_flux, b2, m2);
+    {
+        const TileMap& m = e->m_conc[e->conc_shape];
+        int g = grid_of(e, EK_SCONC, m.n);
+        ens_ev(e, 2 * EK_SCONC);
+        switch (e->conc_shape) {
+            case 0: kb_sed_conc<4, 16><<<dim3(g), b2, 0, e->stream>>>(e->dtab, m.d, m.n); break;
+            case 1: kb_sed_conc<8, 8><<<dim3(g), b2, 0, e->stream>>>(e->dtab, m.d, m.n); break;
+            default: kb_sed_conc<16, 4><<<dim3(g), b2, 0, e->stream>>>(e->dtab, m.d, m.n); break;
+        }
+        ens_ev(e, 2 * EK_SCONC + 1);
+    }
+    EL(EK_SMORPH, kb_sed_morph, b2, m2);
+#undef EL
+    hipError_t q = hipGetLastError();
+    if (q != hipSuccess) return efail(e, XBG_E_HIP, std::string("ensemble launch: ") + hipGetErrorString(q));
+    return 0;
+}
+
+void drop_graph(xbg_ens* e) {
+    if (e->gexec) hipGraphExecDestroy(e->gexec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->gexec = nullptr;
+    e->graph = nullptr;
+}
+
+int build_graph(xbg_ens* e) {
+    drop_graph(e);
+    EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ens_launch_step(e);
+    hipGraph_t g = nullptr;
+    hipError_t q = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (q != hipSuccess) return efail(e, XBG_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(q));
+    e->graph = g;
+    EHIP(e, hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xbg_ens_create(int ndom, xbg_ctx* const* ctxs, const xbg_ens_domain* run, xbg_ens** out) {
+    if (!out || !ctxs || !run || ndom < 1) return efail(nullptr, XBG_E_ARG, "xbg_ens_create: bad argument");
+    *out = nullptr;
+    xbg_ens* e = new xbg_ens();
+    e->nd = ndom;
+    auto bail = [&](int rc) {
+        xbg_ens_destroy(e);
+        return rc;
+    };
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+        e->ncu = prop.multiProcessorCount;
+    // the single-domain launch shapes by default: one workgroup per tile
+    for (int k = 0; k < EK_N; k++) e->grid_sel[k] = 0;
+    std::vector<int> doms_all, gx2, gy2, gxc, gyc, gxcfl, gycfl;
+    std::vector<TileEnt> cfl;
+    std::map<std::tuple<int, bool, bool>, std::vector<TileEnt>> wave_tiles, row_tiles;
+    e->h.resize(ndom);
+    for (int d = 0; d < ndom; d++) {
+        xbg_ctx* c = ctxs[d];
+        if (!c) return bail(efail(e, XBG_E_ARG, "xbg_ens_create: NULL context"));
+        const xbg_params& P = c->P;
+        const xbg_ens_domain& r = run[d];
+        std::string who = "xbg_ens_create: domain " + std::to_string(d) + ": ";
+        if (P.swave != 1 || P.flow != 1) return bail(efail(e, XBG_E_ARG, who + "needs swave = 1 and flow = 1"));
+        if (!c->Stfile || c->nrec < 2) return bail(efail(e, XBG_E_STATE, who + "no resident boundary records"));
+        if (P.sedtrans == 1 && P.sedstart > r.t_start)
+            return bail(efail(e, XBG_E_ARG, who + "sedstart after the start time is not supported"));
+        if (!(r.dtbc > 0.0) || !(r.dt_prev > 0.0))
+            return bail(efail(e, XBG_E_ARG, who + "dtbc and dt_prev must be positive"));
+        if (hipStreamSynchronize(c->stream) != hipSuccess)
+            return bail(efail(e, XBG_E_HIP, who + "context stream failed"));
+        e->ctx.push_back(c);
+        e->base_cur.push_back(c->cur);
+        e->base_zcur.push_back(c->zcur);
+        e->base_ccur.push_back(c->ccur);
+        DomDesc& D = e->h[d];
+        std::memset(&D, 0, sizeof(D));
+        D.p = c->kp;
+        D.sed = P.sedtrans == 1 ? 1 : 0;
+        D.morph = c->kp.morfac > 0.0f ? 1 : 0;
+        D.with_wave = 1;
+        for (int par = 0; par < 2; par++) {
+            int c0 = c->cur ^ par, z0 = c->zcur ^ par, cc0 = c->ccur ^ (D.sed ? par : 0);
+            D.f[par][PH_START] = fld_state(c, c0, z0, cc0);
+            D.f[par][PH_WAVE] = fld_state(c, c0 ^ 1, z0, cc0);
+            D.f[par][PH_ZS] = fld_state(c, c0 ^ 1, z0 ^ 1, cc0);
+            D.f[par][PH_END] = fld_state(c, c0 ^ 1, z0 ^ 1, cc0 ^ D.sed);
+        }
+        D.Stfile = c->Stfile;
+        D.qfile = c->qfile;
+        D.nrec = c->nrec;
+        D.nst = P.ny * P.ntheta;
+        D.nq = 4 * P.ny;
+        D.t0 = r.t0;
+        D.dtbc = r.dtbc;
+        D.endtime = r.endtime;
+        D.nextout = r.endtime;
+        D.Trep = (float)r.Trep;
+        D.zsbndi = c->zsbndi;
+        D.zsbndn = c->zsbndn;
+        D.windu = c->windu;
+        D.windv = c->windv;
+        D.nwave = c->nblk_wave;
+        D.t = r.t_start;
+        D.dt = r.dt_prev;
+        D.steps = 0;
+        D.par = 1;  // the state at the start is the end state of parity 1
+        D.status = ENS_ACTIVE;
+        // tile maps
+        const KP& p = c->kp;
+        doms_all.push_back(d);
+        gx2.push_back((p.nx + BX - 1) / BX);
+        gy2.push_back((p.ny + BY - 1) / BY);
+        gxc.push_back((p.nx + BX - 1) / BX);
+        gyc.push_back((p.ny + CY - 1) / CY);
+        int ncfl = (int)((c->N + CFL_TILE - 1) / CFL_TILE);
+        D.cfl_first = (int)cfl.size();
+        D.cfl_count = ncfl;
+        for (int k = 0; k < ncfl; k++) cfl.push_back(TileEnt{d, k, 0, 0});
+        const int nt = (p.nt == 12 || p.nt == 24) ? p.nt : 0;
+        const bool cur = p.wci != 0.0f;
+        float sigm = (float)(2 * PI_ / (double)(float)r.Trep);
+        const bool rq = (sigm > 0.0f && sigm < 2.0f) && p.dth_recip;
+        // wave tiles in the XCD order of the single-domain launch: the
+        // domain's run starts on a multiple of 8 entries, padded with
+        // entries of an idle domain slot (-1 -> skipped below)
+        auto& wv = wave_tiles[std::make_tuple(nt, cur, rq)];
+        while (wv.size() % 8) wv.push_back(TileEnt{d, -1, 0, 0});
+        const int ntw = p.wtiles_x * p.wtiles_y;
+        for (int b = 0; b < ntw; b++) wv.push_back(TileEnt{d, xcd_tile(b, ntw), 0, 0});
+        if (!p.fold_rows || nt == 0) {
+            auto& rv = row_tiles[std::make_tuple(nt, cur, false)];
+            for (int side = 0; side < 2; side++)
+                for (int bx = 0; bx < (p.nx + 255) / 256; bx++) rv.push_back(TileEnt{d, bx, side, 0});
+        }
+    }
+    int rc = 0;
+    EHIP(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    EHIP(e, hipMalloc((void**)&e->dtab, sizeof(DomDesc) * ndom));
+    EHIP(e, hipMemcpy(e->dtab, e->h.data(), sizeof(DomDesc) * ndom, hipMemcpyHostToDevice));
+    EHIP(e, hipMalloc((void**)&e->part, sizeof(float) * (cfl.size() > 0 ? cfl.size() : 1)));
+    if ((rc = upload_map(e, e->m_cfl, cfl))) return bail(rc);
+    if ((rc = upload_map(e, e->m_2d, grid_map(doms_all, gx2, gy2)))) return bail(rc);
+    if ((rc = upload_map(e, e->m_corr, grid_map(doms_all, gxc, gyc)))) return bail(rc);
+    for (int sh = 0; sh < N_SHAPE; sh++) {
+        const int qw = kShapeW[sh], qh = 64 / qw;
+        std::vector<TileEnt> dv, cv;
+        for (int d = 0; d < ndom; d++) {
+            const KP& p = e->h[d].p;
+            // dispersion: row 0 holds the offshore boundary workgroup
+            dv.push_back(TileEnt{d, 0, 0, 0});
+            for (int by = 0; by < (p.ny + qh - 1) / qh; by++)
+                for (int bx = 0; bx < (p.nx + 4 * qw - 1) / (4 * qw); bx++) dv.push_back(TileEnt{d, bx, by + 1, 0});
+            for (int by = 0; by < (p.ny + qh - 1) / qh; by++)
+                for (int bx = 0; bx < (p.nx + 4 * qw - 1) / (4 * qw); bx++) cv.push_back(TileEnt{d, bx, by, 0});
+        }
+        if ((rc = upload_map(e, e->m_disp[sh], dv))) return bail(rc);
+        if ((rc = upload_map(e, e->m_conc[sh], cv))) return bail(rc);
+    }
+    for (auto& kv : wave_tiles) {
+        WaveGroup w{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), TileMap{}};
+        if ((rc = upload_map(e, w.map, kv.second))) return bail(rc);
+        e->waves.push_back(w);
+    }
+    for (auto& kv : row_tiles) {
+        WaveGroup w{std::get<0>(kv.first), std::get<1>(kv.first), false, TileMap{}};
+        if ((rc = upload_map(e, w.map, kv.second))) return bail(rc);
+        e->rows.push_back(w);
+    }
+    e->ev.resize(2 * EK_N);
+    for (auto& q : e->ev) EHIP(e, hipEventCreate(&q));
+    if ((rc = build_graph(e))) return bail(rc);
+    *out = e;
+    return XBG_OK;
+}
+
+int xbg_ens_step(xbg_ens* e, int nsteps) {
+    if (!e || nsteps < 0) return XBG_E_ARG;
+    for (int k = 0; k < nsteps; k++) EHIP(e, hipGraphLaunch(e->gexec, e->stream));
+    return XBG_OK;
+}
+
+int xbg_ens_sync(xbg_ens* e) {
+    if (!e) return XBG_E_ARG;
+    EHIP(e, hipStreamSynchronize(e->stream));
+    return XBG_OK;
+}
+
+int xbg_ens_status(xbg_ens* e, double* t, double* dt, int* steps, int* status) {
+    if (!e) return XBG_E_ARG;
+    EHIP(e, hipStreamSynchronize(e->stream));
+    std::vector<DomDesc> h(e->nd);
+    EHIP(e, hipMemcpy(h.data(), e->dtab, sizeof(DomDesc) * e->nd, hipMemcpyDeviceToHost));
+    for (int d = 0; d < e->nd; d++) {
+        if (t) t[d] = h[d].t;
+        if (dt) dt[d] = h[d].dt;
+        if (steps) steps[d] = h[d].steps;
+        if (status) status[d] = h[d].status;
+        // the contexts read the halves the last step wrote
+        xbg_ctx* c = e->ctx[d];
+        int par = h[d].steps & 1;
+        c->cur = e->base_cur[d] ^ par;
+        c->zcur = e->base_zcur[d] ^ par;
+        c->ccur = e->base_ccur[d] ^ (h[d].sed ? par : 0);
+        sync_ptrs(c);
+    }
+    return XBG_OK;
+}
+
+int xbg_ens_tune(xbg_ens* e, int rounds, char* report, size_t n) {
+    if (!e || rounds < 1) return XBG_E_ARG;
+    // every candidate of every kernel class is timed over whole steps (the
+    // steps are real: a launch shape changes which thread computes which
+    // cell, never a result)
+    const int ncand_grid = N_GRID;
+    std::vector<std::vector<double>> tg(EK_N, std::vector<double>(ncand_grid, 0.0));
+    std::vector<std::vector<int>> cg(EK_N, std::vector<int>(ncand_grid, 0));
+    std::vector<double> td(N_SHAPE, 0.0), tc(N_SHAPE, 0.0);
+    std::vector<int> cd(N_SHAPE, 0), cc(N_SHAPE, 0);
+    e->prof = true;
+    int rc = 0;
+    // warm-up step (first-launch costs) with the current choices
+    if ((rc = ens_launch_step(e))) { e->prof = false; return rc; }
+    EHIP(e, hipStreamSynchronize(e->stream));
+    const int total = rounds * ncand_grid * N_SHAPE;
+    for (int r = 0; r < total; r++) {
+        for (int k = 0; k < EK_N; k++) e->grid_sel[k] = r % ncand_grid;
+        e->disp_shape = (r / ncand_grid) % N_SHAPE;
+        e->conc_shape = (r / ncand_grid) % N_SHAPE;
+        if ((rc = ens_launch_step(e))) { e->prof = false; return rc; }
+        EHIP(e, hipStreamSynchronize(e->stream));
+        for (int k = 0; k < EK_N; k++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e->ev[2 * k], e->ev[2 * k + 1]) != hipSuccess) continue;
+            tg[k][r % ncand_grid] += ms;
+            cg[k][r % ncand_grid] += 1;
+            if (k == EK_DISP) { td[e->disp_shape] += ms; cd[e->disp_shape]++; }
+            if (k == EK_SCONC) { tc[e->conc_shape] += ms; cc[e->conc_shape]++; }
+        }
+    }
+    e->prof = false;
+    std::string out;
+    char line[256];
+    for (int k = 0; k < EK_N; k++) {
+        int best = 0;
+        double bt = 1e300;
+        for (int j = 0; j < ncand_grid; j++) {
+            double a = cg[k][j] ? tg[k][j] / cg[k][j] : 1e300;
+            snprintf(line, sizeof(line), "%s grid=%s%d us=%.2f\n", kEnsName[k], kGridPerCU[j] ? "" : "tiles:",
+                     kGridPerCU[j] ? kGridPerCU[j] * e->ncu : 0, 1000.0 * a);
+            out += line;
+            if (a < bt) { bt = a; best = j; }
+        }
+        e->grid_sel[k] = best;
+    }
+    int bd = 0, bc = 0;
+    for (int s = 0; s < N_SHAPE; s++) {
+        double a = cd[s] ? td[s] / cd[s] : 1e300, b = cc[s] ? tc[s] / cc[s] : 1e300;
+        snprintf(line, sizeof(line), "kb_dispersion patch=%dx%d us=%.2f\nkb_sed_conc patch=%dx%d us=%.2f\n",
+                 kShapeW[s], 64 / kShapeW[s], 1000.0 * a, kShapeW[s], 64 / kShapeW[s], 1000.0 * b);
+        out += line;
+        if (a < td[bd] / (cd[bd] ? cd[bd] : 1)) bd = s;
+        if (b < tc[bc] / (cc[bc] ? cc[bc] : 1)) bc = s;
+    }
+    e->disp_shape = bd;
+    e->conc_shape = bc;
+    out += "chosen:";
+    for (int k = 0; k < EK_N; k++) {
+        snprintf(line, sizeof(line), " %s=%d", kEnsName[k], kGridPerCU[e->grid_sel[k]] * e->ncu);
+        out += line;
+    }
+    snprintf(line, sizeof(line), " dispersion_patch=%dx%d conc_patch=%dx%d\n", kShapeW[bd], 64 / kShapeW[bd],
+             kShapeW[bc], 64 / kShapeW[bc]);
+    out += line;
+    if (report && n) {
+        size_t m = out.size() < n - 1 ? out.size() : n - 1;
+        std::memcpy(report, out.data(), m);
+        report[m] = 0;
+    }
+    return build_graph(e);
+}
+
+int xbg_ens_kernel_times(xbg_ens* e, int nsteps, double* us) {
+    if (!e || nsteps < 1 || !us) return XBG_E_ARG;
+    for (int k = 0; k < EK_N; k++) us[k] = 0.0;
+    e->prof = true;
+    for (int r = 0; r < nsteps; r++) {
+        int rc = ens_launch_step(e);
+        if (rc) { e->prof = false; return rc; }
+        EHIP(e, hipStreamSynchronize(e->stream));
+        for (int k = 0; k < EK_N; k++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e->ev[2 * k], e->ev[2 * k + 1]) == hipSuccess) us[k] += 1000.0 * ms / nsteps;
+        }
+    }
+    e->prof = false;
+    return XBG_OK;
+}
+
+const char* xbg_ens_kernel_name(int k) { return (k >= 0 && k < EK_N) ? kEnsName[k] : nullptr; }
+
+const char* xbg_ens_last_error(const xbg_ens* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+int xbg_ens_destroy(xbg_ens* e) {
+    if (!e) return XBG_OK;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    TileMap* maps[] = {&e->m_cfl, &e->m_2d, &e->m_corr};
+    for (auto* m : maps) if (m->d) hipFree(m->d);
+    for (int s = 0; s < N_SHAPE; s++) {
+        if (e->m_disp[s].d) hipFree(e->m_disp[s].d);
+        if (e->m_conc[s].d) hipFree(e->m_conc[s].d);
+    }
+    for (auto& w : e->waves) if (w.map.d) hipFree(w.map.d);
+    for (auto& w : e->rows) if (w.map.d) hipFree(w.map.d);
+    if (e->part) hipFree(e->part);
+    if (e->dtab) hipFree(e->dtab);
+    for (auto& q : e->ev) if (q) hipEventDestroy(q);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
     return XBG_OK;
 }
 

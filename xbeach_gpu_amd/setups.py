@@ -163,3 +163,51 @@ class GroupForcing:
         S = np.stack([self.St(t0 + m * self.dtbc) for m in range(n)])
         Q = np.stack([self.qbnd(t0 + m * self.dtbc) for m in range(n)])
         return S.astype(np.float64), Q.astype(np.float64)
+
+
+class JonswapForcing:
+    """Seeded JONSWAP wave groups and bound long waves of the reference's
+    wavebndtype 4 (makjonswap + GenWGnLBW, makjonswap.cpp:73-1419, built in
+    xbg_bnd.cpp): one generated block of ``rtlength`` seconds at ``dtbc``
+    spacing whose record 0 sits at time ``t0``.  Same interface as
+    GroupForcing: St(t) and qbnd(t) interpolate the records linearly in
+    double and round to float as wavebnd / flowbnd do (Wavestep.cu:722-745,
+    tools.cpp:22-25); records(t0, n) hands out the stored block."""
+
+    def __init__(self, p: XbgParams, Hm0: float, Tp: float, dir_deg: float = 0.0, s: float = 10.0,
+                 seed: int = 0, gamma: float = 3.3, offdepth: float = 8.0, rtlength: float = 600.0,
+                 dtbc: float = 1.0, t0: float = 0.0):
+        from .bnd_jonswap import jonswap, wave_groups
+
+        q = p.copy()
+        q.rtlength = float(rtlength)
+        q.dtbc = float(dtbc)
+        # mean direction in the grid convention is dir_deg (nautical 270 - dir)
+        spec = jonswap(Hm0, Tp, 270.0 - dir_deg, s, gamma, p.grdalpha)
+        blk = wave_groups(q, offdepth, spec, seed)
+        self.p = p
+        self.Trep = float(blk.Trep)
+        self.dtbc = float(dtbc)
+        self.t0 = float(t0)
+        self.S = blk.Stfile
+        self.Q = blk.qfile
+
+    def _interp(self, A: np.ndarray, t: float) -> np.ndarray:
+        x = (t - self.t0) / self.dtbc
+        m = int(math.floor(x))
+        if m < 0 or m + 1 >= len(A):
+            raise ValueError(f"time {t} outside the generated block")
+        tsl = t - (self.t0 + m * self.dtbc)
+        return (A[m] + tsl / self.dtbc * (A[m + 1] - A[m])).astype(np.float32)
+
+    def St(self, t: float) -> np.ndarray:
+        return self._interp(self.S, t)
+
+    def qbnd(self, t: float) -> np.ndarray:
+        return self._interp(self.Q, t)
+
+    def records(self, t0: float, n: int):
+        m = int(round((t0 - self.t0) / self.dtbc))
+        if abs(self.t0 + m * self.dtbc - t0) > 1e-9 or m < 0 or m + n > len(self.S):
+            raise ValueError("requested records are not on the generated block")
+        return self.S[m:m + n].copy(), self.Q[m:m + n].copy()
