@@ -246,6 +246,25 @@ int xbg_read_points(xbg_ctx* ctx, float* zs, float* H, int cap_records, int* nre
  */
 int xbg_mass_balance(xbg_ctx* ctx, double* water_volume, double* bed_volume);
 
+/*
+ * The same mass balance without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * hh and zb are DEVICE pointers to n floats each (e.g. a domain's fields);
+ * the 256 workgroup partial pairs go to a file-scope device array of the
+ * library (4 KiB + 16 B per device, held until the process ends), so the
+ * call allocates nothing; results equal xbg_mass_balance's bit for bit on
+ * the same fields.  stream may be NULL (the null stream); the call returns
+ * after the stream has reached the read-back.  n above
+ * XBG_NOALLOC_MAX_CELLS (the library's grid limit) is refused with
+ * XBG_E_ARG.  Calls that overlap on one GPU share the array and race:
+ * serialise them.  Errors: xbg_noalloc_last_error().
+ */
+#define XBG_NOALLOC_MAX_CELLS ((size_t)2147483647)
+int xbg_mass_balance_noalloc(const float* hh, const float* zb, size_t n, double dx, void* stream,
+                             double* water_volume, double* bed_volume);
+const char* xbg_noalloc_last_error(void);
+
 /* ------------------------------------------------------------------------
  * Offshore wave boundary generation (host code; makjonswap.cpp)
  * ---------------------------------------------------------------------- */

@@ -96,6 +96,8 @@ PROTOTYPES = {
     "xbg_reset_means": (C.c_int, [_CTX]),
     "xbg_divide_means": (C.c_int, [_CTX, C.c_double]),
     "xbg_mass_balance": (C.c_int, [_CTX, _D, _D]),
+    "xbg_mass_balance_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_double, C.c_void_p, _D, _D]),
+    "xbg_noalloc_last_error": (C.c_char_p, []),
     "xbg_sync": (C.c_int, [_CTX]),
     "xbg_stream": (C.c_void_p, [_CTX]),
     "xbg_profile": (C.c_int, [_CTX, C.c_int]),

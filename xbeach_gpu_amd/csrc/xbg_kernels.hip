@@ -2740,6 +2740,99 @@ int xbg_profile_report(xbg_ctx* c, char* buf, size_t n) {
 }  // extern "C"
 
 // ===========================================================================
+// Mass balance without allocation: the partial sums live in a file-scope
+// device array instead of the context's buffers
+// ===========================================================================
+namespace {
+
+// 256 workgroup pairs (the workgroup count of xbg_mass_balance) and the
+// final pair: 4 KiB + 16 B on every device the library runs on, held until
+// the process ends.  k_sum2 writes every pair and k_kahan_pairs the final
+// one, so nothing carries over from an earlier call.
+constexpr int NOALLOC_PARTS = 256;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[2 * NOALLOC_PARTS + 2];
+
+namespace {
+
+// The array's device address per device, looked up once per device with
+// that device current.  Host code uses these addresses, never the symbol.
+constexpr int NOALLOC_MAX_DEV = 64;
+double* g_scratch[NOALLOC_MAX_DEV] = {};
+std::mutex g_scratch_mu;
+thread_local std::string g_noalloc_error;
+
+int noalloc_fail(int code, const std::string& msg) {
+    g_noalloc_error = msg;
+    return code;
+}
+
+// The host loop of xbg_mass_balance on the device: one lane combines the
+// pairs in workgroup order with the same compensated sums.
+__global__ __launch_bounds__(64) void k_kahan_pairs(const double* part, int n, double* out) {
+    if (threadIdx.x != 0) return;
+    double a = 0.0, b = 0.0, ca = 0.0, cb = 0.0;
+    for (int j = 0; j < n; j++) {
+        double y = part[2 * j] - ca, t = a + y;
+        ca = (t - a) - y;
+        a = t;
+        y = part[2 * j + 1] - cb;
+        t = b + y;
+        cb = (t - b) - y;
+        b = t;
+    }
+    out[0] = a;
+    out[1] = b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xbg_mass_balance_noalloc(const float* hh, const float* zb, size_t n, double dx, void* stream,
+                             double* water_volume, double* bed_volume) {
+    if (!hh || !zb) return noalloc_fail(XBG_E_ARG, "xbg_mass_balance_noalloc: NULL field");
+    if (n > XBG_NOALLOC_MAX_CELLS)
+        return noalloc_fail(XBG_E_ARG, "xbg_mass_balance_noalloc: n = " + std::to_string(n) +
+                                           " cells exceeds the maximum of " + std::to_string(XBG_NOALLOC_MAX_CELLS));
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(XBG_E_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) return noalloc_fail(XBG_E_ARG, "xbg_mass_balance_noalloc: device index");
+    double* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch[dev]) {
+            void* a = nullptr;
+            e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+            if (e != hipSuccess)
+                return noalloc_fail(XBG_E_HIP, std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+            g_scratch[dev] = (double*)a;
+        }
+        ptr = g_scratch[dev];
+    }
+    hipStream_t st = (hipStream_t)stream;
+    k_sum2<<<dim3(NOALLOC_PARTS), dim3(256), 0, st>>>(hh, zb, n, ptr);
+    k_kahan_pairs<<<dim3(1), dim3(64), 0, st>>>(ptr, NOALLOC_PARTS, ptr + 2 * NOALLOC_PARTS);
+    e = hipGetLastError();
+    if (e != hipSuccess) return noalloc_fail(XBG_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+    double r[2] = {0.0, 0.0};
+    e = hipMemcpyAsync(r, ptr + 2 * NOALLOC_PARTS, sizeof(r), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return noalloc_fail(XBG_E_HIP, std::string("read-back: ") + hipGetErrorString(e));
+    const double area = dx * dx;
+    if (water_volume) *water_volume = r[0] * area;
+    if (bed_volume) *bed_volume = r[1] * area;
+    return XBG_OK;
+}
+
+const char* xbg_noalloc_last_error(void) { return g_noalloc_error.c_str(); }
+
+}  // extern "C"
+
+// ===========================================================================
 // Batched ensemble of independent domains
 //
 // The domains of a storm-scenario ensemble (uneven grids, own forcing, own
