@@ -247,6 +247,27 @@ int xbg_read_points(xbg_ctx* ctx, float* zs, float* H, int cap_records, int* nre
 int xbg_mass_balance(xbg_ctx* ctx, double* water_volume, double* bed_volume);
 
 /*
+ * Surrogate coupling (no counterpart in the reference).  xbg_field_device:
+ * the device address of a field (as xbg_get_field names it; NULL if
+ * unknown), for building surrogate inputs on the device.
+ * xbg_set_wave_products: the fields the flow and sediment steps read from
+ * the wave step (k, c, cg, H, E, Dw, DR, R, Fx, Fy, urms, ust, thetamean)
+ * formed from surrogate predictions y (device FP32 [ny][nx][5]: H, Dw, DR,
+ * Fx, Fy, normalised; norm10 = 5 means then 5 standard deviations, host)
+ * and the offshore mean direction dir (grid radians).  xbg_step_nowave: a
+ * step with those products held (offshore boundary, flow, sediment and
+ * bed; no wave action solve).  xbg_advance_window: main-loop iterations on
+ * the resident records until t_end (steps aligned on it), full steps
+ * (wave_mode 1) or xbg_step_nowave steps (0), adding to the running means
+ * after each step when accumulate is set; *nsteps = steps taken.
+ */
+void* xbg_field_device(xbg_ctx* ctx, const char* name);
+int xbg_set_wave_products(xbg_ctx* ctx, const float* y, const float* norm10, double dir);
+int xbg_step_nowave(xbg_ctx* ctx, double dt, double totaltime);
+int xbg_advance_window(xbg_ctx* ctx, int wave_mode, double t0, double dtbc, double Trep, double t_end,
+                       double endtime, int accumulate, double* totaltime, double* dt, int* nsteps);
+
+/*
  * The same mass balance without allocating: no counterpart in the
  * reference.  For callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
@@ -377,8 +398,72 @@ int xbg_ens_tune(xbg_ens* e, int rounds, char* report, size_t n);
 #define XBG_ENS_NKERNELS 13
 int xbg_ens_kernel_times(xbg_ens* e, int nsteps, double* us);
 const char* xbg_ens_kernel_name(int k);
+/* Wave-group window means for surrogate training data: _enable allocates
+ * per-domain sums (8 fields) and gives the float count of one sample set
+ * (12 per cell, domains in order); _add (after a step) adds zs, uu, vv, H,
+ * Dw, DR, Fx, Fy of the state the last step left; _sample writes the means
+ * over count additions to out (device) as [cell][zb, zs, uu, vv, Hm0, Tp,
+ * dir, H, Dw, DR, Fx, Fy] with meta = per-domain Hm0, Tp, dir, and
+ * restarts the sums. */
+int xbg_ens_window_enable(xbg_ens* e, size_t* sample_floats);
+int xbg_ens_window_add(xbg_ens* e);
+int xbg_ens_window_sample(xbg_ens* e, int count, const float* meta, float* out);
 const char* xbg_ens_last_error(const xbg_ens* e);
 int xbg_ens_destroy(xbg_ens* e);
+
+/* ------------------------------------------------------------------------
+ * Learned wave surrogate (no counterpart in the reference): a 4-level
+ * convolutional encoder-decoder (32-256 channels, 1.8 M parameters) that
+ * maps zb, zs, uu, vv and the offshore Hm0, Tp, mean direction to the
+ * wave-group-averaged H, wave dissipation, roller dissipation, Fx, Fy that
+ * the flow and sediment steps read from the wave solve (wavestep,
+ * Wavestep.cu:755-1167).  BF16 weights and activations, FP32 accumulation
+ * on the matrix cores (v_mfma_f32_16x16x32_bf16), FP32 master weights and
+ * Adam.  Tensors are NHWC DEVICE buffers: inputs BF16 [n][H][W][8] (the 7
+ * channels and a zero), targets BF16 [n][H][W][5], outputs FP32
+ * [n][H][W][5], all normalised per channel by the caller.
+ * ---------------------------------------------------------------------- */
+typedef struct xbg_sur xbg_sur;
+/* Buffers for batches of up to nmax images of H x W (H a multiple of 16,
+ * W = 256 or a multiple of 512); He-normal weights from seed. */
+int xbg_sur_create(int nmax, int H, int W, unsigned seed, xbg_sur** out);
+int xbg_sur_destroy(xbg_sur* s);
+const char* xbg_sur_last_error(const xbg_sur* s);   /* s may be NULL */
+/* Parameter vector (FP32, per convolution: weights [taps][cout][cin] then
+ * bias [cout]; the head last): size, read, write (write resets Adam). */
+size_t xbg_sur_nparams(const xbg_sur* s);
+int xbg_sur_get_params(xbg_sur* s, float* host);
+int xbg_sur_set_params(xbg_sur* s, const float* host);
+/* Gradient of the last xbg_sur_backward / xbg_sur_train_step (host copy). */
+int xbg_sur_get_grads(xbg_sur* s, float* host);
+/* Inference: y (device, may be NULL) receives the FP32 outputs. */
+int xbg_sur_forward(xbg_sur* s, const void* x, int n, float* y);
+/* Mean squared error over the normalised targets (forward only). */
+int xbg_sur_loss(xbg_sur* s, const void* x, const void* t, int n, double* loss);
+/* Forward, loss and every parameter gradient (no update). */
+int xbg_sur_backward(xbg_sur* s, const void* x, const void* t, int n, double* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) at learning rate lr; loss is
+ * the batch loss before the update.  Returns without waiting for the
+ * update (xbg_sur_sync). */
+int xbg_sur_train_step(xbg_sur* s, const void* x, const void* t, int n, double lr, double* loss);
+int xbg_sur_sync(xbg_sur* s);
+/* One convolution on caller device buffers ('same' padding, taps 9 = 3x3
+ * or 1 = 1x1; Cin, Cout multiples of 32; W a multiple of 64 or 32):
+ * forward Y = relu?(conv(X, W) + bias) in BF16; data gradient dX of a
+ * pre-activation gradient dZ; weight gradient dW [taps][Cout][Cin] and
+ * bias gradient db [Cout] (FP32, db may be NULL).  Weights [taps][Cout][Cin]
+ * BF16.  These allocate scratch and synchronise the device. */
+int xbg_sur_conv_fwd(const void* X, const void* Wt, const float* bias, void* Y, int n, int H, int W, int Cin,
+                     int Cout, int taps, int relu);
+int xbg_sur_conv_dgrad(const void* dZ, const void* Wt, void* dX, int n, int H, int W, int Cin, int Cout, int taps);
+int xbg_sur_conv_wgrad(const void* X, const void* dZ, float* dW, float* db, int n, int H, int W, int Cin, int Cout,
+                       int taps);
+/* Surrogate inputs of one domain on the device: x8 (BF16 [n][8]) from the
+ * device fields zb, zs, uu, vv (n cells) and the offshore Hm0, Tp, mean
+ * direction, each normalised with norm14 (7 means then 7 standard
+ * deviations, host).  Synchronises the device. */
+int xbg_sur_pack_inputs(const float* zb, const float* zs, const float* uu, const float* vv, size_t n, double Hm0,
+                        double Tp, double dir, const float* norm14, void* x8);
 
 /* ------------------------------------------------------------------------
  * netCDF output (host code, no netCDF library; writenetcdf.cpp:32-239)

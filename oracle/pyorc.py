@@ -13,15 +13,19 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
-_lib = None
+# the same oracle with the host libm where the reference CPU path calls it
+LIBM_PATH = os.path.join(HERE, "liborc_libm.so")
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
-        L = C.CDLL(LIB_PATH)
+def lib(variant: str = "shared"):
+    """variant "shared": the deterministic functions the HIP kernels share;
+    "libm": the host libm as the reference's CPU path calls it."""
+    if variant not in _libs:
+        path = LIB_PATH if variant == "shared" else LIBM_PATH
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+        L = C.CDLL(path)
         L.orc_create.restype = C.c_void_p
         L.orc_create.argtypes = [C.c_void_p, C.c_int]
         L.orc_destroy.argtypes = [C.c_void_p]
@@ -33,8 +37,8 @@ def lib():
         L.orc_stage.argtypes = [C.c_void_p, C.c_char_p, C.c_double, C.c_double]
         L.orc_step.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.orc_cfl.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 DIR = ("ee", "rr", "ctheta", "xadvec", "yadvec", "thetaadvec")
@@ -44,8 +48,8 @@ INT = ("wetu", "wetv")
 class Oracle:
     """One domain of the CPU restatement, FP32 (prec=4) or FP64 (prec=8)."""
 
-    def __init__(self, params, prec: int = 4):
-        self.L = lib()
+    def __init__(self, params, prec: int = 4, variant: str = "shared"):
+        self.L = lib(variant)
         self.p = params.copy()
         self.prec = prec
         self.nx, self.ny, self.nt = params.nx, params.ny, params.ntheta

@@ -83,6 +83,7 @@ inline int pp2(int i, int n) { return i >= n - 2 ? n - 1 : i + 2; }
 // step can be checked bit for bit.  sqrt, division and the arithmetic are
 // IEEE correctly rounded on both sides.
 template <class T> struct Fm;
+#ifndef XBG_ORACLE_LIBM
 template <> struct Fm<float> {
     static float sqrt(float x) { return sqrtf(x); }
     static float exp(float x) { return xm::expf(x); }
@@ -107,6 +108,38 @@ template <> struct Fm<double> {
     static double sin(double x) { return xm::sin(x); }
     static double fabs(double x) { return std::fabs(x); }
 };
+#else
+// The host libm exactly where the reference's CPU path calls it
+// (wave_CPU.cpp: the float overloads tanhf, powf, sinhf, expf, ... of
+// <cmath>, and double tanh / exp where the argument is double): the build
+// that measures how far the shared deterministic functions move results
+// from the reference's own CPU arithmetic.
+template <> struct Fm<float> {
+    static float sqrt(float x) { return sqrtf(x); }
+    static float exp(float x) { return ::expf(x); }
+    static float log(float x) { return ::logf(x); }
+    static float log10(float x) { return ::log10f(x); }
+    static float pow(float x, float y) { return ::powf(x, y); }
+    static float sinh(float x) { return ::sinhf(x); }
+    static float tanh(float x) { return ::tanhf(x); }
+    static float cos(float x) { return ::cosf(x); }
+    static float sin(float x) { return ::sinf(x); }
+    static float fabs(float x) { return fabsf(x); }
+};
+template <> struct Fm<double> {
+    static double sqrt(double x) { return std::sqrt(x); }
+    static double exp(double x) { return std::exp(x); }
+    static double log(double x) { return std::log(x); }
+    static double log10(double x) { return std::log10(x); }
+    static double pow(double x, double y) { return std::pow(x, y); }
+    static double sinh(double x) { return std::sinh(x); }
+    static double tanh(double x) { return std::tanh(x); }
+    static double cos(double x) { return std::cos(x); }
+    static double sin(double x) { return std::sin(x); }
+    static double fabs(double x) { return std::fabs(x); }
+};
+#endif
+typedef Fm<double> Dm;  // functions of double arguments in either precision
 
 inline double D(double x) { return x; }  // marks a promotion to double
 
@@ -255,7 +288,7 @@ template <class T> struct Dom {
             T L0 = twopi * g / (sigm[i] * sigm[i]);
             T L1 = L0, L2, err;
             for (int n = 1; n < 200; n++) {
-                L2 = (T)(D(L0) * xm::tanh(2 * PI_ * D(hh[i]) / D(L1)));
+                L2 = (T)(D(L0) * Dm::tanh(2 * PI_ * D(hh[i]) / D(L1)));
                 err = F::fabs(L2 - L1);
                 L1 = (L1 * aphi + L2 * bphi);
                 if (err <= T(0.0001f)) break;
@@ -481,7 +514,7 @@ template <class T> struct Dom {
             T h = hh[i];
             T et = E[i];
             T Ht = F::sqrt(et * fac);
-            T Qb = (T)(1.0 - xm::exp(tmax(-1.0 * D(F::pow(Ht / gammaa / h, nexp)), -100.0)));
+            T Qb = (T)(1.0 - Dm::exp(tmax(-1.0 * D(F::pow(Ht / gammaa / h, nexp)), -100.0)));
             Qb = tmin(Qb, T(1.0f));
             Dd[i] = (T)(D(Qb) * 2.0 * D(alpha) / D(Trep) * D(et) * D(Ht) / D(h));
             T uorb = (T)(PI_ * D(Ht) / D(Trep * F::sinh(tmin(k[i] * h, T(10.0f)))));

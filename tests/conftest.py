@@ -10,6 +10,15 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+    # torch's HIP runtime (it provides device buffers to some GPU tests)
+    # must initialise before libxbg's: once libxbg has opened the device,
+    # torch reports no GPU
+    try:
+        import torch
+
+        torch.cuda.is_available()
+    except Exception:
+        pass
 
 
 def gpu_available() -> bool:

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 MAIN = ("zs", "uu", "vv", "ee", "H", "C", "zb")
 MORE = ("rr", "hh", "stdep", "k", "c", "cg", "E", "DR", "R", "urms", "ust", "Sxx", "Syy", "Sxy", "hu", "hv",
         "qx", "qy", "ueu", "vev", "uv", "vu", "dzb", "zsold", "umeanbnd", "rolthick", "Fx", "Fy", "thetamean",
-        "nuh", "dzsdt", "Sus", "Svs", "ceqsg")
+        "nuh", "dzsdt", "Sus", "Svs", "ceqsg", "D")
 
 # uneven grids: neither dimension a multiple of the tiles; ny - 1 a multiple
 # of 8 (41, 33) puts the boundary rows of the wave step in their own launch
@@ -123,13 +123,15 @@ def test_ensemble_tuner_keeps_results_and_graph_replays(need_gpu):
             lst.append(d)
         runs.append(RunSetup(t_start, 0.05, 1e9, t_start, 1.0, frc.Trep))
     ea, eb = Ensemble(a_doms, runs), Ensemble(b_doms, runs)
-    report = ea.tune(rounds=1)       # 12 timed steps
+    report = ea.tune(rounds=1)       # a warm-up step and 12 timed steps
     assert "chosen:" in report and "kb_wave" in report
+    _, _, s0, _ = ea.status()
+    assert (s0 == 13).all()
     ea.step(8)
-    eb.step(20)
+    eb.step(21)
     ta, _, sa, _ = ea.status()
     tb, _, sb, _ = eb.status()
-    assert (sa == 20).all() and (sb == 20).all() and np.array_equal(ta, tb)
+    assert (sa == 21).all() and (sb == 21).all() and np.array_equal(ta, tb)
     for da, db in zip(a_doms, b_doms):
         for f in MAIN:
             assert np.array_equal(da.get(f), db.get(f)), f

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 MAIN = ("zs", "uu", "vv", "ee", "H", "C", "zb")
 MORE = ("rr", "hh", "stdep", "k", "c", "cg", "E", "DR", "R", "urms", "ust", "Sxx", "Syy", "Sxy", "hu", "hv",
         "qx", "qy", "ueu", "vev", "uv", "vu", "dzb", "zsold", "umeanbnd", "rolthick", "Fx", "Fy", "thetamean",
-        "nuh", "dzsdt", "Sus", "Svs", "ceqsg")
+        "nuh", "dzsdt", "Sus", "Svs", "ceqsg", "D")
 
 
 def rel(a, b):

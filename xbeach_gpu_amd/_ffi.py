@@ -141,6 +141,31 @@ PROTOTYPES = {
     "xbg_ens_kernel_name": (C.c_char_p, [C.c_int]),
     "xbg_ens_last_error": (C.c_char_p, [C.c_void_p]),
     "xbg_ens_destroy": (C.c_int, [C.c_void_p]),
+    "xbg_ens_window_enable": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "xbg_ens_window_add": (C.c_int, [C.c_void_p]),
+    "xbg_ens_window_sample": (C.c_int, [C.c_void_p, C.c_int, _F, C.c_void_p]),
+    "xbg_sur_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(C.c_void_p)]),
+    "xbg_sur_destroy": (C.c_int, [C.c_void_p]),
+    "xbg_sur_last_error": (C.c_char_p, [C.c_void_p]),
+    "xbg_sur_nparams": (C.c_size_t, [C.c_void_p]),
+    "xbg_sur_get_params": (C.c_int, [C.c_void_p, _F]),
+    "xbg_sur_set_params": (C.c_int, [C.c_void_p, _F]),
+    "xbg_sur_get_grads": (C.c_int, [C.c_void_p, _F]),
+    "xbg_sur_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "xbg_sur_loss": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _D]),
+    "xbg_sur_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _D]),
+    "xbg_sur_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, _D]),
+    "xbg_sur_sync": (C.c_int, [C.c_void_p]),
+    "xbg_sur_conv_fwd": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int] * 7),
+    "xbg_sur_conv_dgrad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int] * 6),
+    "xbg_sur_conv_wgrad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p] + [C.c_int] * 6),
+    "xbg_sur_pack_inputs": (C.c_int, [C.c_void_p] * 4 + [C.c_size_t, C.c_double, C.c_double, C.c_double, _F,
+                                                         C.c_void_p]),
+    "xbg_field_device": (C.c_void_p, [_CTX, C.c_char_p]),
+    "xbg_set_wave_products": (C.c_int, [_CTX, C.c_void_p, _F, C.c_double]),
+    "xbg_step_nowave": (C.c_int, [_CTX, C.c_double, C.c_double]),
+    "xbg_advance_window": (C.c_int, [_CTX, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, _D, _D, C.POINTER(C.c_int)]),
 }
 
 

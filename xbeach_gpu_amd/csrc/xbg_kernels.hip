@@ -373,6 +373,7 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     float kk = f.k[i];
     float fw = fwm_of(p, f.stdep[i]);
     float Dv = p.breakmodel == 1 ? roelvink_cell(p, s.Trep, E, h, kk, fw) : baldock_cell(p, s.Trep, Ht, h, kk, fw);
+    f.Dw[i] = Dv;  // wave dissipation D (an output field of the reference, and a surrogate target)
     float cc = f.c[i];
     float uui = 0.0f, vvi = 0.0f;
     if (CUR) {
@@ -2146,7 +2147,7 @@ void register_fields(xbg_ctx* c) {
     R("cg", &f.cg, N); R("H", &f.H, N); R("E", &f.E, N); R("DR", &f.DR, N); R("R", &f.R, N);
     R("urms", &f.urms, N); R("ust", &f.ust, N); R("thetamean", &f.thetamean, N); R("Sxx", &f.Sxx, N);
     R("Sxy", &f.Sxy, N); R("Syy", &f.Syy, N); R("Fx", &f.Fx, N); R("Fy", &f.Fy, N); R("nuh", &f.nuh, N);
-    R("Sus", &f.Sus, N); R("Svs", &f.Svs, N); R("ceqsg", &f.ceqsg, N);
+    R("Sus", &f.Sus, N); R("Svs", &f.Svs, N); R("ceqsg", &f.ceqsg, N); R("D", &f.Dw, N);
     R("umeanbnd", &f.umean, (size_t)c->P.ny);
     R("Hmean", &c->Hmean, N); R("uumean", &c->uumean, N); R("vvmean", &c->vvmean, N);
     R("hhmean", &c->hhmean, N); R("zsmean", &c->zsmean, N); R("zsmax", &c->zsmax, N);
@@ -2167,7 +2168,7 @@ float* resolve(xbg_ctx* c, const std::string& name, size_t* cnt) {
 const char kFieldNames[] =
     "ee\0rr\0C\0zs\0uu\0vv\0hh\0zb\0stdep\0hu\0hv\0ueu\0vev\0vmageu\0vmagev\0vu\0uv\0qx\0qy\0uuold\0"
     "vvold\0zsold\0dzsdt\0rolthick\0dzb\0k\0c\0cg\0H\0E\0DR\0R\0urms\0ust\0thetamean\0Sxx\0Sxy\0Syy\0"
-    "Fx\0Fy\0nuh\0Sus\0Svs\0ceqsg\0umeanbnd\0Hmean\0uumean\0vvmean\0hhmean\0zsmean\0zsmax\0Cmean\0";
+    "Fx\0Fy\0nuh\0Sus\0Svs\0ceqsg\0D\0umeanbnd\0Hmean\0uumean\0vvmean\0hhmean\0zsmean\0zsmax\0Cmean\0";
 
 // defer_force: the flow step follows and forms the wave forces in its first
 // launch
@@ -2740,6 +2741,129 @@ int xbg_profile_report(xbg_ctx* c, char* buf, size_t n) {
 }  // extern "C"
 
 // ===========================================================================
+// Wave products from the learned surrogate (no counterpart in the
+// reference): between full-solver checkpoints of a long morphology run the
+// wave action solve is replaced by the surrogate's window-mean H, wave
+// dissipation Dw, roller dissipation DR and forces Fx, Fy; this kernel forms
+// from them every field the flow and sediment steps read from the wave step.
+// ===========================================================================
+namespace {
+
+struct SurNorm {
+    float mean[5], std[5];  // H, Dw, DR, Fx, Fy
+};
+
+// Per cell: k, c, cg of dispersionCPU on the current depth; the
+// predictions denormalised (H, Dw, DR clipped at zero); E = rho g H^2 / 8
+// (energintCPU); R = DR c / (2 g beta), the roller energy whose dissipation
+// 2 g beta R / c (dissipationCPU, wave_CPU.cpp:1030-1089) is DR; urms and
+// the Stokes drift of breakerdelayCPU (wave_CPU.cpp:1334-1374);
+// thetamean the offshore mean direction.  Dry cells get no waves.
+__global__ __launch_bounds__(256) void k_wave_from_sur(KP p, SP s, Fld f, const float* __restrict__ y, SurNorm nm,
+                                                       float dir) {
+    const BI bi{(int)blockIdx.x, (int)blockIdx.y};
+    const int x = gx(bi), yy = gy(bi);
+    if (x >= p.nx || yy >= p.ny) return;
+    const size_t i = (size_t)x + (size_t)yy * p.nx;
+    const float h = f.hh[i];
+    float kk, cc, cgg;
+    dispersion_cell(p, s.sigm, h, kk, cc, cgg);
+    f.k[i] = kk;
+    f.c[i] = cc;
+    f.cg[i] = cgg;
+    const float* q = y + i * 5;
+    const float wet = h > p.eps ? 1.0f : 0.0f;
+    const float H = fmx(q[0] * nm.std[0] + nm.mean[0], 0.0f) * wet;
+    const float Dw = fmx(q[1] * nm.std[1] + nm.mean[1], 0.0f) * wet;
+    const float DR = fmx(q[2] * nm.std[2] + nm.mean[2], 0.0f) * wet;
+    const float Fx = (q[3] * nm.std[3] + nm.mean[3]) * wet;
+    const float Fy = (q[4] * nm.std[4] + nm.mean[4]) * wet;
+    const float E = p.rho * p.g * H * H / 8.0f;
+    const float R = p.beta > 0.0f ? DR * cc / (2.0f * p.g * p.beta) : 0.0f;
+    const float hmin = 0.2f;
+    const float hhi = fmx(h, hmin);
+    const float cib = fmx(cc, sqrtf(hmin * p.g));
+    const float uorb = (float)(PI_ * (double)H / (double)s.Trep / (double)xm::sinhf(fmn(fmx(kk, 0.01f) * hhi, 10.0f)));
+    f.H[i] = H;
+    f.E[i] = E;
+    f.Dw[i] = Dw;
+    f.DR[i] = DR;
+    f.R[i] = R;
+    f.Fx[i] = Fx;
+    f.Fy[i] = Fy;
+    f.urms[i] = uorb / (sqrtf(2.0f));
+    f.ust[i] = (E / cib / p.rho / hhi + 2.0f * R / cib / p.rho / hhi) * wet;
+    f.thetamean[i] = dir;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* xbg_field_device(xbg_ctx* c, const char* name) {
+    if (!c || !name) return nullptr;
+    size_t n = 0;
+    return (void*)resolve(c, name, &n);
+}
+
+int xbg_set_wave_products(xbg_ctx* c, const float* y, const float* norm10, double dir) {
+    if (!c || !y || !norm10) return fail(c, XBG_E_ARG, "xbg_set_wave_products: NULL argument");
+    SurNorm nm;
+    for (int k = 0; k < 5; k++) {
+        nm.mean[k] = norm10[k];
+        nm.std[k] = norm10[5 + k];
+    }
+    SP s = make_sp(c, 0.0, 0.0);
+    LAUNCH(c, "k_wave_from_sur", k_wave_from_sur, grid2(c->kp), blk2, c->kp, s, c->f, y, nm, (float)dir);
+    return XBG_OK;
+}
+
+// A step with the wave products held (xbg_set_wave_products): the offshore
+// boundary, the flow and the sediment and bed steps, without the wave action
+// solve.
+int xbg_step_nowave(xbg_ctx* c, double dt, double totaltime) {
+    if (!c) return XBG_E_ARG;
+    if (c->P.flow != 1) return fail(c, XBG_E_ARG, "xbg_step_nowave: needs flow = 1");
+    SP s = make_sp(c, dt, totaltime);
+    int rc;
+    LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), c->kp, s, c->f, 1);
+    const int sed = c->P.sedtrans == 1 && totaltime >= c->P.sedstart;
+    if ((rc = launch_flow(c, s, sed, 0)) != 0) return rc;
+    if (sed && (rc = launch_sed(c, s, sed)) != 0) return rc;
+    return XBG_OK;
+}
+
+// Main-loop iterations on the resident records until t_end (the steps are
+// aligned on it, as on an output time): full steps (wave_mode 1, the
+// xbg_advance_resident loop) or steps with the held wave products
+// (wave_mode 0: the flow time step alone limits dt).  With accumulate set
+// every step adds to the running means (xbg_accumulate_means).
+int xbg_advance_window(xbg_ctx* c, int wave_mode, double t0, double dtbc, double Trep, double t_end, double endtime,
+                       int accumulate, double* totaltime, double* dt, int* nsteps) {
+    if (!c || !totaltime || !dt || !(dtbc > 0.0)) return XBG_E_ARG;
+    int n = 0, rc;
+    while (*totaltime < t_end && *totaltime < endtime) {
+        double dtf, dtw, dtn;
+        if ((rc = xbg_cfl_dt(c, 0, &dtf, &dtw))) return rc;
+        if ((rc = xbg_next_dt(dtf, dtw, wave_mode ? c->P.swave == 1 : 0, *dt, *totaltime, t_end, endtime, &dtn)))
+            return fail(c, rc, "Model crashed: non-finite or zero flow time step");
+        const double t = *totaltime + dtn;
+        const int rec = (int)std::floor((t - t0) / dtbc);
+        const double tsl = t - (rec * dtbc + t0);
+        if ((rc = xbg_use_bnd_record(c, rec, tsl, dtbc, Trep, c->zsbndi, c->zsbndn, c->windu, c->windv))) return rc;
+        if ((rc = wave_mode ? xbg_step(c, dtn, t) : xbg_step_nowave(c, dtn, t))) return rc;
+        if (accumulate && (rc = xbg_accumulate_means(c))) return rc;
+        *totaltime = t;
+        *dt = dtn;
+        n++;
+    }
+    if (nsteps) *nsteps = n;
+    return XBG_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
 // Mass balance without allocation: the partial sums live in a file-scope
 // device array instead of the context's buffers
 // ===========================================================================
@@ -2867,6 +2991,9 @@ struct DomDesc {
     double t0, dtbc, endtime, nextout;
     float Trep, zsbndi, zsbndn, windu, windv;
     int cfl_first, cfl_count, nwave;
+    // wave-group window sums (xbg_ens_window_*): 8 planes of nx*ny
+    float* win;
+    size_t wout;  // offset of this domain's samples in the output array
     // run state, written by k_ens_dt
     double t, dt;
     int steps, par, status, pad_;
@@ -3150,6 +3277,56 @@ __global__ __launch_bounds__(256) void k_ens_dt(DomDesc* tab, const float* part)
     for (int j = tid; j < D.nq; j += 256) {
         qo[j] = (float)D.qfile[(size_t)rec * D.nq + j];
         qn[j] = (float)D.qfile[(size_t)(rec + 1) * D.nq + j];
+    }
+}
+
+// Window sums of the fields the surrogate learns from (zs, uu, vv) and
+// predicts (H, Dw, DR, Fx, Fy), after a step: the state after the last step
+// is the start state of the other parity.
+constexpr int WIN_F = 8;
+__global__ __launch_bounds__(256) void kb_window_add(DTab tab, TMap map, int ntiles) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        const BI bi{e.bx, e.by};
+        const int x = gx(bi), y = gy(bi);
+        if (D.win && x < D.p.nx && y < D.p.ny) {
+            const Fld& f = D.f[D.par ^ 1][PH_START];
+            const size_t i = (size_t)x + (size_t)y * D.p.nx, N = D.p.N;
+            float* w = D.win;
+            w[i] += f.zs[i];
+            w[N + i] += f.uu[i];
+            w[2 * N + i] += f.vv[i];
+            w[3 * N + i] += f.H[i];
+            w[4 * N + i] += f.Dw[i];
+            w[5 * N + i] += f.DR[i];
+            w[6 * N + i] += f.Fx[i];
+            w[7 * N + i] += f.Fy[i];
+        }
+    }
+}
+
+// Window means as surrogate samples: out[wout + i*12 + c] for the channels
+// zb, zs, uu, vv, Hm0, Tp, dir (from meta[3*d..]), H, Dw, DR, Fx, Fy; the
+// sums restart at zero.
+__global__ __launch_bounds__(256) void kb_window_sample(DTab tab, TMap map, int ntiles, float inv,
+                                                        const float* __restrict__ meta, float* __restrict__ out) {
+    ENS_FOR_TILES(map, ntiles) {
+        const TileEnt e = map[t_];
+        const DomDesc& D = tab[e.dom];
+        const BI bi{e.bx, e.by};
+        const int x = gx(bi), y = gy(bi);
+        if (D.win && x < D.p.nx && y < D.p.ny) {
+            const Fld& f = D.f[D.par ^ 1][PH_START];
+            const size_t i = (size_t)x + (size_t)y * D.p.nx, N = D.p.N;
+            float* w = D.win;
+            float* o = out + D.wout + i * 12;
+            o[0] = f.zb[i];
+            for (int k = 0; k < 3; k++) o[1 + k] = w[k * N + i] * inv;
+            for (int k = 0; k < 3; k++) o[4 + k] = meta[3 * e.dom + k];
+            for (int k = 3; k < WIN_F; k++) o[4 + k] = w[k * N + i] * inv;
+            for (int k = 0; k < WIN_F; k++) w[k * N + i] = 0.0f;
+        }
     }
 }
 
@@ -3649,6 +3826,49 @@ int xbg_ens_kernel_times(xbg_ens* e, int nsteps, double* us) {
     return XBG_OK;
 }
 
+int xbg_ens_window_enable(xbg_ens* e, size_t* sample_floats) {
+    if (!e) return XBG_E_ARG;
+    EHIP(e, hipStreamSynchronize(e->stream));
+    size_t off = 0;
+    for (int d = 0; d < e->nd; d++) {
+        DomDesc& D = e->h[d];
+        if (!D.win) {
+            EHIP(e, hipMalloc((void**)&D.win, WIN_F * (size_t)D.p.N * sizeof(float)));
+            EHIP(e, hipMemset(D.win, 0, WIN_F * (size_t)D.p.N * sizeof(float)));
+        }
+        D.wout = off;
+        off += 12 * (size_t)D.p.N;
+    }
+    // the run state on the device is newer than the host copy: patch only
+    // the window fields of each descriptor
+    for (int d = 0; d < e->nd; d++) {
+        EHIP(e, hipMemcpy(&e->dtab[d].win, &e->h[d].win, sizeof(float*), hipMemcpyHostToDevice));
+        EHIP(e, hipMemcpy(&e->dtab[d].wout, &e->h[d].wout, sizeof(size_t), hipMemcpyHostToDevice));
+    }
+    if (sample_floats) *sample_floats = off;
+    return XBG_OK;
+}
+
+int xbg_ens_window_add(xbg_ens* e) {
+    if (!e) return XBG_E_ARG;
+    kb_window_add<<<dim3(e->m_2d.n), dim3(BX, BY), 0, e->stream>>>(e->dtab, e->m_2d.d, e->m_2d.n);
+    EHIP(e, hipGetLastError());
+    return XBG_OK;
+}
+
+int xbg_ens_window_sample(xbg_ens* e, int count, const float* meta, float* out) {
+    if (!e || count < 1 || !meta || !out) return XBG_E_ARG;
+    float* dm = nullptr;
+    EHIP(e, hipMallocAsync((void**)&dm, 3 * sizeof(float) * e->nd, e->stream));
+    EHIP(e, hipMemcpyAsync(dm, meta, 3 * sizeof(float) * e->nd, hipMemcpyHostToDevice, e->stream));
+    kb_window_sample<<<dim3(e->m_2d.n), dim3(BX, BY), 0, e->stream>>>(e->dtab, e->m_2d.d, e->m_2d.n, 1.0f / count,
+                                                                     dm, out);
+    EHIP(e, hipGetLastError());
+    EHIP(e, hipFreeAsync(dm, e->stream));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    return XBG_OK;
+}
+
 const char* xbg_ens_kernel_name(int k) { return (k >= 0 && k < EK_N) ? kEnsName[k] : nullptr; }
 
 const char* xbg_ens_last_error(const xbg_ens* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
@@ -3666,6 +3886,8 @@ int xbg_ens_destroy(xbg_ens* e) {
     for (auto& w : e->waves) if (w.map.d) hipFree(w.map.d);
     for (auto& w : e->rows) if (w.map.d) hipFree(w.map.d);
     if (e->part) hipFree(e->part);
+    for (auto& D : e->h)
+        if (D.win) hipFree(D.win);
     if (e->dtab) hipFree(e->dtab);
     for (auto& q : e->ev) if (q) hipEventDestroy(q);
     if (e->stream) hipStreamDestroy(e->stream);

@@ -155,6 +155,33 @@ class Domain:
                                               float(endtime), C.byref(t), C.byref(d)))
         return t.value, d.value
 
+    # -- surrogate coupling ----------------------------------------------------
+    def device_ptr(self, name: str) -> int:
+        """Device address of a field (for building surrogate inputs)."""
+        p = self.L.xbg_field_device(self._ctx, name.encode())
+        if not p:
+            raise KeyError(name)
+        return p
+
+    def set_wave_products(self, y_ptr: int, norm10, dir_rad: float) -> None:
+        """Wave fields of the flow and sediment steps from surrogate
+        predictions (device FP32 [ny][nx][5], normalised by norm10)."""
+        n = np.ascontiguousarray(norm10, np.float32)
+        self._chk(self.L.xbg_set_wave_products(self._ctx, C.c_void_p(y_ptr), _fp(n), float(dir_rad)))
+
+    def step_nowave(self, dt: float, totaltime: float) -> None:
+        self._chk(self.L.xbg_step_nowave(self._ctx, float(dt), float(totaltime)))
+
+    def advance_window(self, wave_mode: int, t0: float, dtbc: float, Trep: float, t_end: float, endtime: float,
+                       accumulate: bool, totaltime: float, prev_dt: float) -> tuple[float, float, int]:
+        """Main-loop iterations until t_end: full steps (wave_mode 1) or steps
+        with the held wave products (0); returns (t, dt, steps)."""
+        t, d, n = C.c_double(totaltime), C.c_double(prev_dt), C.c_int()
+        self._chk(self.L.xbg_advance_window(self._ctx, int(wave_mode), float(t0), float(dtbc), float(Trep),
+                                            float(t_end), float(endtime), 1 if accumulate else 0, C.byref(t),
+                                            C.byref(d), C.byref(n)))
+        return t.value, d.value, n.value
+
     def accumulate_means(self) -> None:
         self._chk(self.L.xbg_accumulate_means(self._ctx))
 

@@ -82,6 +82,25 @@ class Ensemble:
         self._chk(self.L.xbg_ens_kernel_times(self._e, int(nsteps), us.ctypes.data_as(C.POINTER(C.c_double))))
         return {self.L.xbg_ens_kernel_name(k).decode(): float(us[k]) for k in range(13)}
 
+    # -- wave-group window means (surrogate training samples) ----------------
+    def window_enable(self) -> int:
+        """Allocate the window sums; returns the float count of one sample
+        set (12 channels per cell, domains in order)."""
+        n = C.c_size_t()
+        self._chk(self.L.xbg_ens_window_enable(self._e, C.byref(n)))
+        return n.value
+
+    def window_add(self) -> None:
+        self._chk(self.L.xbg_ens_window_add(self._e))
+
+    def window_sample(self, count: int, meta: np.ndarray, out_ptr: int) -> None:
+        """Means over count additions into the device array at out_ptr
+        ([cell][zb, zs, uu, vv, Hm0, Tp, dir, H, Dw, DR, Fx, Fy]); meta holds
+        Hm0, Tp, dir per domain."""
+        m = np.ascontiguousarray(meta, np.float32).reshape(self.n, 3)
+        self._chk(self.L.xbg_ens_window_sample(self._e, int(count), m.ctypes.data_as(C.POINTER(C.c_float)),
+                                               C.c_void_p(out_ptr)))
+
     def close(self) -> None:
         if self._e:
             self.L.xbg_ens_destroy(self._e)

@@ -1,0 +1,252 @@
+"""Training data and training of the wave surrogate.
+
+Data: the plain solver itself, run as a batched ensemble of seeded domains,
+spun up, then averaged over wave-group windows of ``win_steps`` steps on the
+device (xbg_ens_window_*).  Each window of each domain is one sample:
+inputs zb and the window means of zs, uu, vv with the offshore Hm0, Tp and
+mean direction broadcast (7 channels and a zero); targets the window means
+of H, wave dissipation, roller dissipation, Fx, Fy.  Two families:
+
+  ensemble  256 x 256 domains at dx = 4 m, bathymetries and sea states drawn
+            as in the storm-scenario ensemble (the training configuration)
+  site      the morphology site (1024 x 512 at dx = 3 m, barred rip-channel
+            beach) under sea states drawn from the storm's range with seeds
+            other than the storm run's, cut into 256 x 256 crops
+
+Samples stay in HBM as BF16 (NHWC), normalised per channel with the mean and
+standard deviation of the first batch.
+
+Training: Adam (lr 1e-3, batch 64) on the hand-written BF16 MFMA forward and
+backward kernels (xbg_sur_*), 80/20 split, held-out loss reported.  torch
+provides device buffers and the batch gathers only; every product of the
+network and every update runs in the library's kernels.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import time
+
+import numpy as np
+
+IN_CH = 8   # zb, zs, uu, vv, Hm0, Tp, dir, 0
+OUT_CH = 5  # H, Dw, Dr, Fx, Fy
+RAW_CH = 12
+
+
+def site_specs(n: int, seed: int = 99):
+    """The morphology site under n sea states from the storm's range."""
+    from .workloads import DomainSpec
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for d in range(n):
+        Hm0 = float(rng.uniform(1.0, 4.0))
+        out.append(DomainSpec(1024, 512, 3.0, 1 / 40.0, 1.0, Hm0, 8.0 + 6.0 * (Hm0 - 1.0) / 3.0,
+                              float(rng.uniform(-20, 20)), 10.0, 24, 250e-6, 10_000 + seed * 100 + d))
+    return out
+
+
+def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, dt_max: float = 0.16):
+    """One ensemble of the given domains (all the same size): spin-up, then
+    nwin windows; returns (raw FP32 [nwin*ndom, ny, nx, 12] on the GPU,
+    GPU seconds, host set-up seconds, cell-steps)."""
+    import torch
+
+    from . import Domain, workloads
+    from .ensemble import Ensemble, RunSetup
+
+    t0 = 50.0
+    doms, runs, meta = [], [], []
+    h0 = time.perf_counter()
+    rtl = float(int(30 + (spin_steps + nwin * win_steps) * dt_max))
+    for sp in specs:
+        p, st, frc = workloads.build(sp, t0=t0, rtlength=rtl, zb_accum=0)
+        d = Domain(p)
+        for k, v in st.items():
+            d.set(k, v)
+        d.init_state(frc.Trep)
+        d.set_bnd_series(frc.S, frc.Q)
+        doms.append(d)
+        runs.append(RunSetup(t0 + 1.0, 0.05, 1e9, t0, frc.dtbc, frc.Trep))
+        meta.append((sp.Hm0, sp.Tp, math.radians(sp.dir_deg)))
+    host_s = time.perf_counter() - h0
+    ens = Ensemble(doms, runs)
+    nfl = ens.window_enable()
+    raw = torch.empty(nwin, nfl, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ens.step(spin_steps)
+    for w in range(nwin):
+        for _ in range(win_steps):
+            ens.step(1)
+            ens.window_add()
+        ens.window_sample(win_steps, np.array(meta), raw[w].data_ptr())
+    ens.sync()
+    gpu_s = time.perf_counter() - w0
+    t, dt, steps, status = ens.status()
+    if not (status == 0).all():
+        raise RuntimeError(f"a training domain stopped: {status}")
+    ny, nx = specs[0].ny, specs[0].nx
+    cell_steps = float(nx * ny * steps.sum())
+    ens.close()
+    for d in doms:
+        d.close()
+    return raw.view(nwin * len(specs), ny, nx, RAW_CH), gpu_s, host_s, cell_steps
+
+
+def crops(raw, size: int = 256):
+    """Non-overlapping size x size crops of [S, ny, nx, C] samples."""
+    S, ny, nx, C = raw.shape
+    if ny == size and nx == size:
+        return raw
+    r = raw[:, : ny // size * size, : nx // size * size]
+    r = r.reshape(S, ny // size, size, nx // size, size, C).permute(0, 1, 3, 2, 4, 5)
+    return r.reshape(-1, size, size, C)
+
+
+class Dataset:
+    """Normalised BF16 samples in HBM; channel statistics from the first
+    batch added."""
+
+    def __init__(self):
+        self.X, self.T = [], []
+        self.mean = self.std = None
+
+    def add(self, raw):
+        import torch
+
+        flat = raw.reshape(-1, RAW_CH)
+        if self.mean is None:
+            self.mean = flat.mean(0)
+            self.std = flat.std(0).clamp_min(1e-6)
+        m, s = self.mean, self.std
+        X = torch.zeros(raw.shape[:3] + (IN_CH,), dtype=torch.bfloat16, device=raw.device)
+        X[..., :7] = ((raw[..., :7] - m[:7]) / s[:7]).to(torch.bfloat16)
+        T = ((raw[..., 7:] - m[7:]) / s[7:]).to(torch.bfloat16).contiguous()
+        self.X.append(X)
+        self.T.append(T)
+
+    def tensors(self):
+        import torch
+
+        X = torch.cat(self.X) if len(self.X) > 1 else self.X[0]
+        T = torch.cat(self.T) if len(self.T) > 1 else self.T[0]
+        self.X, self.T = [X], [T]
+        return X, T
+
+    def norm14(self):
+        m, s = self.mean.cpu().numpy(), self.std.cpu().numpy()
+        return np.concatenate([m[:7], s[:7]]).astype(np.float32)
+
+    def norm10(self):
+        m, s = self.mean.cpu().numpy(), self.std.cpu().numpy()
+        return np.concatenate([m[7:], s[7:]]).astype(np.float32)
+
+
+def train(X, T, epochs: int, batch: int = 64, lr: float = 1e-3, seed: int = 0, log=print):
+    """Adam on 80 % of the samples; returns (surrogate, per-epoch train
+    loss, train seconds, held-out loss, steps)."""
+    import torch
+
+    from .surrogate import Surrogate
+
+    S, H, W = X.shape[:3]
+    g = torch.Generator().manual_seed(seed)
+    perm = torch.randperm(S, generator=g)
+    ntr = int(0.8 * S)
+    tr, te = perm[:ntr].cuda(), perm[ntr:].cuda()
+    sur = Surrogate(batch, H, W, seed=seed)
+    hist = []
+    w0 = time.perf_counter()
+    nstep = 0
+    for ep in range(epochs):
+        order = tr[torch.randperm(ntr, generator=g).cuda()]
+        tot = 0.0
+        nb = 0
+        for k in range(0, ntr - batch + 1, batch):
+            idx = order[k:k + batch]
+            xb = X.index_select(0, idx).contiguous()
+            tb = T.index_select(0, idx).contiguous()
+            tot += sur.train_step(xb.data_ptr(), tb.data_ptr(), batch, lr)
+            nb += 1
+            nstep += 1
+        sur.sync()
+        hist.append(tot / max(nb, 1))
+        log(f"epoch {ep}: train loss {hist[-1]:.5f}  ({time.perf_counter() - w0:.1f} s)")
+    train_s = time.perf_counter() - w0
+    held = evaluate(sur, X, T, te, batch)
+    return sur, hist, train_s, held, nstep
+
+
+def evaluate(sur, X, T, idx, batch):
+    tot, n = 0.0, 0
+    for k in range(0, len(idx), batch):
+        ii = idx[k:k + batch]
+        xb = X.index_select(0, ii).contiguous()
+        tb = T.index_select(0, ii).contiguous()
+        tot += sur.loss(xb.data_ptr(), tb.data_ptr(), len(ii)) * len(ii)
+        n += len(ii)
+    return tot / max(n, 1)
+
+
+def main():
+    import torch
+
+    from . import workloads
+
+    # torch's HIP runtime (the sample buffers) initialises before the
+    # library opens the device
+    torch.cuda.init()
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--family", default="ensemble", choices=["ensemble", "site", "both"])
+    ap.add_argument("--batches", type=int, default=2)
+    ap.add_argument("--domains", type=int, default=256)
+    ap.add_argument("--windows", type=int, default=40)
+    ap.add_argument("--win-steps", type=int, default=80)
+    ap.add_argument("--spin-steps", type=int, default=600)
+    ap.add_argument("--site-domains", type=int, default=16)
+    ap.add_argument("--site-windows", type=int, default=40)
+    ap.add_argument("--site-win-steps", type=int, default=130)
+    ap.add_argument("--site-spin-steps", type=int, default=4000)
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--save", default=None, help="write parameters and normalisation (.npz)")
+    a = ap.parse_args()
+    ds = Dataset()
+    info = {"gen_gpu_s": 0.0, "gen_host_s": 0.0, "gen_cell_steps": 0.0}
+
+    def take(raw, g, h, c):
+        info["gen_gpu_s"] += g
+        info["gen_host_s"] += h
+        info["gen_cell_steps"] += c
+        ds.add(crops(raw))
+        del raw
+
+    if a.family in ("ensemble", "both"):
+        for b in range(a.batches):
+            specs = workloads.ensemble_specs(a.domains, seed=7000 + b, fixed_size=(256, 256, 4.0))
+            take(*run_batch(specs, a.windows, a.win_steps, a.spin_steps))
+            print(f"ensemble batch {b} done", flush=True)
+    if a.family in ("site", "both"):
+        take(*run_batch(site_specs(a.site_domains), a.site_windows, a.site_win_steps, a.site_spin_steps,
+                        dt_max=0.1))
+        print("site batch done", flush=True)
+    X, T = ds.tensors()
+    print(f"{X.shape[0]} samples", flush=True)
+    sur, hist, train_s, held, nstep = train(X, T, a.epochs, a.batch)
+    # forward 18.9 GFLOP per 256 x 256 image (the 17 convolutions); training
+    # runs forward, data and weight gradients: about three times that
+    flops = 3 * 18.9e9 * X.shape[1] * X.shape[2] / 65536.0 * a.batch * nstep
+    out = {"samples": int(X.shape[0]), "family": a.family, "gen_gpu_s": info["gen_gpu_s"],
+           "gen_host_s": info["gen_host_s"], "gen_cell_steps_per_s": info["gen_cell_steps"] / info["gen_gpu_s"],
+           "train_s": train_s, "train_steps": nstep, "train_tflops_est": flops / train_s / 1e12,
+           "train_loss": hist, "heldout_loss": held, "norm14": ds.norm14().tolist(), "norm10": ds.norm10().tolist()}
+    if a.save:
+        np.savez(a.save, params=sur.get_params(), norm14=ds.norm14(), norm10=ds.norm10(), heldout=held)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
