@@ -2754,7 +2754,8 @@ struct SurNorm {
 };
 
 // Per cell: k, c, cg of dispersionCPU on the current depth; the
-// predictions denormalised (H, Dw, DR clipped at zero); E = rho g H^2 / 8
+// predictions denormalised (H, Dw, DR clipped at zero) and depth-limited
+// as energint limits the wave energy; E = rho g H^2 / 8
 // (energintCPU); R = DR c / (2 g beta), the roller energy whose dissipation
 // 2 g beta R / c (dissipationCPU, wave_CPU.cpp:1030-1089) is DR; urms and
 // the Stokes drift of breakerdelayCPU (wave_CPU.cpp:1334-1374);
@@ -2773,11 +2774,16 @@ __global__ __launch_bounds__(256) void k_wave_from_sur(KP p, SP s, Fld f, const 
     f.cg[i] = cgg;
     const float* q = y + i * 5;
     const float wet = h > p.eps ? 1.0f : 0.0f;
-    const float H = fmx(q[0] * nm.std[0] + nm.mean[0], 0.0f) * wet;
-    const float Dw = fmx(q[1] * nm.std[1] + nm.mean[1], 0.0f) * wet;
-    const float DR = fmx(q[2] * nm.std[2] + nm.mean[2], 0.0f) * wet;
-    const float Fx = (q[3] * nm.std[3] + nm.mean[3]) * wet;
-    const float Fy = (q[4] * nm.std[4] + nm.mean[4]) * wet;
+    // the depth limitation of energintCPU (wave_CPU.cpp:897-930): H at most
+    // gammax h, the energy-like products scaled by the same (Hmax/H)^2
+    const float Hp = fmx(q[0] * nm.std[0] + nm.mean[0], 0.0f);
+    const float Hmax = p.gammax * h;
+    const float sc = Hp > Hmax ? (Hmax / Hp) * (Hmax / Hp) : 1.0f;
+    const float H = fmn(Hp, Hmax) * wet;
+    const float Dw = fmx(q[1] * nm.std[1] + nm.mean[1], 0.0f) * sc * wet;
+    const float DR = fmx(q[2] * nm.std[2] + nm.mean[2], 0.0f) * sc * wet;
+    const float Fx = (q[3] * nm.std[3] + nm.mean[3]) * sc * wet;
+    const float Fy = (q[4] * nm.std[4] + nm.mean[4]) * sc * wet;
     const float E = p.rho * p.g * H * H / 8.0f;
     const float R = p.beta > 0.0f ? DR * cc / (2.0f * p.g * p.beta) : 0.0f;
     const float hmin = 0.2f;

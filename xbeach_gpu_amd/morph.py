@@ -206,6 +206,7 @@ def main():
     ap.add_argument("--hours", type=float, default=24.0, help="morphological hours of the storm to run")
     ap.add_argument("--ckpt-every", type=int, default=50)
     ap.add_argument("--tol", type=float, default=0.05)
+    ap.add_argument("--save", default=None, help="write the initial and final bed levels (.npz)")
     a = ap.parse_args()
     hydro_s = a.hours * 3600.0 / MORFAC
     p = site_params(hydro_s)
@@ -216,6 +217,9 @@ def main():
     out = {"hydro_s": hydro_s, "plain": pl, "surrogate": {k: v for k, v in s.items() if k != "checkpoints"},
            "checkpoints": s["checkpoints"], "speedup": pl["wall_s"] / s["wall_s"]}
     out.update(compare(p, zb0, zp, zs_))
+    if a.save:
+        np.savez_compressed(a.save, zb0=zb0.astype(np.float32), zb_plain=zp.astype(np.float32),
+                            zb_surrogate=zs_.astype(np.float32))
     print(json.dumps(out))
 
 

@@ -48,10 +48,11 @@ def site_specs(n: int, seed: int = 99):
     return out
 
 
-def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, dt_max: float = 0.16):
+def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, on_window, dt_max: float = 0.16):
     """One ensemble of the given domains (all the same size): spin-up, then
-    nwin windows; returns (raw FP32 [nwin*ndom, ny, nx, 12] on the GPU,
-    GPU seconds, host set-up seconds, cell-steps)."""
+    nwin windows, each window's samples (raw FP32 [ndom, ny, nx, 12] on the
+    GPU, reused between windows) handed to on_window; returns (GPU seconds,
+    host set-up seconds, cell-steps)."""
     import torch
 
     from . import Domain, workloads
@@ -74,7 +75,8 @@ def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, dt_max: float =
     host_s = time.perf_counter() - h0
     ens = Ensemble(doms, runs)
     nfl = ens.window_enable()
-    raw = torch.empty(nwin, nfl, dtype=torch.float32, device="cuda")
+    raw = torch.empty(nfl, dtype=torch.float32, device="cuda")
+    ny, nx = specs[0].ny, specs[0].nx
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     ens.step(spin_steps)
@@ -82,18 +84,18 @@ def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, dt_max: float =
         for _ in range(win_steps):
             ens.step(1)
             ens.window_add()
-        ens.window_sample(win_steps, np.array(meta), raw[w].data_ptr())
+        ens.window_sample(win_steps, np.array(meta), raw.data_ptr())
+        on_window(raw.view(len(specs), ny, nx, RAW_CH))
     ens.sync()
     gpu_s = time.perf_counter() - w0
     t, dt, steps, status = ens.status()
     if not (status == 0).all():
         raise RuntimeError(f"a training domain stopped: {status}")
-    ny, nx = specs[0].ny, specs[0].nx
     cell_steps = float(nx * ny * steps.sum())
     ens.close()
     for d in doms:
         d.close()
-    return raw.view(nwin * len(specs), ny, nx, RAW_CH), gpu_s, host_s, cell_steps
+    return gpu_s, host_s, cell_steps
 
 
 def crops(raw, size: int = 256):
@@ -207,35 +209,39 @@ def main():
     ap.add_argument("--win-steps", type=int, default=80)
     ap.add_argument("--spin-steps", type=int, default=600)
     ap.add_argument("--site-domains", type=int, default=16)
+    ap.add_argument("--site-batches", type=int, default=1)
     ap.add_argument("--site-windows", type=int, default=40)
     ap.add_argument("--site-win-steps", type=int, default=130)
     ap.add_argument("--site-spin-steps", type=int, default=4000)
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--save", default=None, help="write parameters and normalisation (.npz)")
     a = ap.parse_args()
     ds = Dataset()
     info = {"gen_gpu_s": 0.0, "gen_host_s": 0.0, "gen_cell_steps": 0.0}
 
-    def take(raw, g, h, c):
+    def take(g, h, c):
         info["gen_gpu_s"] += g
         info["gen_host_s"] += h
         info["gen_cell_steps"] += c
+
+    def add(raw):
         ds.add(crops(raw))
-        del raw
 
     if a.family in ("ensemble", "both"):
         for b in range(a.batches):
             specs = workloads.ensemble_specs(a.domains, seed=7000 + b, fixed_size=(256, 256, 4.0))
-            take(*run_batch(specs, a.windows, a.win_steps, a.spin_steps))
+            take(*run_batch(specs, a.windows, a.win_steps, a.spin_steps, add))
             print(f"ensemble batch {b} done", flush=True)
     if a.family in ("site", "both"):
-        take(*run_batch(site_specs(a.site_domains), a.site_windows, a.site_win_steps, a.site_spin_steps,
-                        dt_max=0.1))
-        print("site batch done", flush=True)
+        for b in range(a.site_batches):
+            take(*run_batch(site_specs(a.site_domains, seed=99 + b), a.site_windows, a.site_win_steps,
+                            a.site_spin_steps, add, dt_max=0.1))
+            print(f"site batch {b} done", flush=True)
     X, T = ds.tensors()
     print(f"{X.shape[0]} samples", flush=True)
-    sur, hist, train_s, held, nstep = train(X, T, a.epochs, a.batch)
+    sur, hist, train_s, held, nstep = train(X, T, a.epochs, a.batch, a.lr)
     # forward 18.9 GFLOP per 256 x 256 image (the 17 convolutions); training
     # runs forward, data and weight gradients: about three times that
     flops = 3 * 18.9e9 * X.shape[1] * X.shape[2] / 65536.0 * a.batch * nstep
