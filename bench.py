@@ -101,13 +101,54 @@ def run_gpu(p, steps, warmup, t0=300.0):
     return wall, sim, prof, fin
 
 
-def cpu_baseline(p, budget_s=20.0):
-    """CPU oracle (FP32, single thread) on the same workload: as many whole
-    steps of the large domain as fit in about budget_s seconds (at least 1)."""
+def run_gpu_graph(p, steps, warmup, t0=300.0):
+    """The same workload through the batched path with one domain: the time
+    step, the boundary record and the step scalars chosen on the device
+    (k_ens_dt) and each step one hipGraph replay, so no host round trip
+    separates two steps.  Bit-identical to the xbg_advance_resident loop."""
+    from xbeach_gpu_amd import Domain
+    from xbeach_gpu_amd.ensemble import Ensemble, RunSetup
+
+    st, frc = build_domain(p)
+    d = Domain(p)
+    for k, v in st.items():
+        d.set(k, v)
+    d.init_state(frc.Trep)
+    d.set_bnd_series(frc.S, frc.Q)
+    ens = Ensemble([d], [RunSetup(t0, 0.05, 1e9, t0, frc.dtbc, frc.Trep)])
+    ens.step(warmup)
+    ens.sync()
+    ta, _, s0, _ = ens.status()
+    w0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    wall = time.perf_counter() - w0
+    tb, _, s1, status = ens.status()
+    if status[0] != 0 or s1[0] - s0[0] != steps:
+        raise RuntimeError(f"graph path stopped: status {status[0]}, {s1[0] - s0[0]} steps")
+    kt = ens.kernel_times(min(steps, 10))
+    fin = {k: bool(np.isfinite(d.get(k)).all()) for k in ("zs", "uu", "H", "zb")}
+    ens.close()
+    d.close()
+    # same keys as the host path's per-kernel report
+    prof = {k: (1, 0.0, v) for k, v in kt.items()}
+    return wall, float(tb[0] - ta[0]), prof, fin
+
+
+def omp_threads():
+    """OpenMP threads of the oracle's multi-core build: OMP_NUM_THREADS when
+    set (the host's share of cores), else every CPU."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def _time_oracle(p, variant, budget_s, max_steps):
     from oracle.pyorc import Oracle
 
     st, frc = build_domain(p)
-    o = Oracle(p, 4)
+    o = Oracle(p, 4, variant)
     for k, v in st.items():
         o.set(k, v)
     o.init_state(frc.Trep)
@@ -118,12 +159,25 @@ def cpu_baseline(p, budget_s=20.0):
         o.set_forcing(frc.St(t), frc.qbnd(t), frc.qbnd(t + 1), frc.Trep, 0.5, 1.0)
         o.step(dt, t)
         n += 1
-        if time.perf_counter() - w0 > budget_s or n >= 50:
+        if time.perf_counter() - w0 > budget_s or n >= max_steps:
             break
     wall = time.perf_counter() - w0
     o.close()
-    return {"value": p.nx * p.ny * n / wall, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} whole steps of the {p.nx}x{p.ny}x{p.ntheta} domain, oracle FP32, 1 thread",
+    return n, wall
+
+
+def cpu_baseline(p, budget_s=20.0):
+    """CPU oracle (FP32) on the same workload: as many whole steps of the
+    large domain as fit in about budget_s seconds (at least 1) with the
+    per-cell loops on OpenMP threads (results equal the sequential oracle's
+    bit for bit), and a shorter single-thread sample beside it."""
+    thr = omp_threads()
+    n, wall = _time_oracle(p, "omp", budget_s, 200)
+    n1, wall1 = _time_oracle(p, "shared", budget_s / 4, 50)
+    return {"value": p.nx * p.ny * n / wall, "unit": "cell-steps/s", "cores": thr, "kind": "port",
+            "sample": f"{n} whole steps of the {p.nx}x{p.ny}x{p.ntheta} domain, oracle FP32, OpenMP on {thr} threads",
+            "single_thread": {"value": p.nx * p.ny * n1 / wall1, "cores": 1,
+                              "sample": f"{n1} whole steps, oracle FP32, 1 thread"},
             "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
@@ -222,6 +276,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="large", choices=["large", "small", "ensemble"])
     ap.add_argument("--domains", type=int, default=256, help="ensemble size (--workload ensemble)")
+    ap.add_argument("--path", default="graph", choices=["graph", "host"],
+                    help="graph: device-side time step and graph-replayed steps (the batched path with one "
+                         "domain); host: one xbg_advance_resident call per step (dt chosen on the host)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -255,10 +312,13 @@ def main():
         print(json.dumps(out))
         return
     p = setups.large_params() if a.workload == "large" else setups.small_params()
-    wall, sim, prof, fin = run_gpu(p, a.steps, a.warmup)
+    if a.path == "graph":
+        wall, sim, prof, fin = run_gpu_graph(p, a.steps, a.warmup)
+    else:
+        wall, sim, prof, fin = run_gpu(p, a.steps, a.warmup)
     N = p.nx * p.ny
     cps = N * a.steps / wall
-    kw = prof.get("k_wave", (0, 0.0, 0.0))
+    kw = prof.get("k_wave", prof.get("kb_wave", (0, 0.0, 0.0)))
     kw_us = kw[2]
     b_kw = alg_bytes(p, "k_wave") * N
     kw_achieved = b_kw / (kw_us * 1e-6) / 1e9 if kw_us > 0 else 0.0
@@ -280,7 +340,8 @@ def main():
         "data": "synthetic",
         "sim_h_per_wall_h": sim / wall,
         "config": {"workload": f"{a.workload} single domain {p.nx}x{p.ny}, dx={p.dx} m, ntheta={p.ntheta}, "
-                               f"wave+flow+sediment+bed, zb_accum={p.zb_accum}, seeded JONSWAP wave groups",
+                               f"wave+flow+sediment+bed, zb_accum={p.zb_accum}, seeded JONSWAP wave groups, "
+                               f"{'device-side dt, graph-replayed steps' if a.path == 'graph' else 'host dt per step'}",
                    "nx": p.nx, "ny": p.ny, "ntheta": p.ntheta},
         "roofline": {"bound": "hbm", "achieved": step_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": step_achieved / HBM_PEAK_GBS, "traffic": measured_traffic(p, "step"),

@@ -461,9 +461,12 @@ int xbg_sur_conv_wgrad(const void* X, const void* dZ, float* dW, float* db, int 
 /* Surrogate inputs of one domain on the device: x8 (BF16 [n][8]) from the
  * device fields zb, zs, uu, vv (n cells) and the offshore Hm0, Tp, mean
  * direction, each normalised with norm14 (7 means then 7 standard
- * deviations, host).  Synchronises the device. */
+ * deviations, host).  Runs on stream (hipStream_t; pass xbg_stream(ctx) of
+ * the domain whose fields these are, so that the pack follows the steps and
+ * xbg_divide_means queued there; NULL: the null stream, which does not wait
+ * for a domain's non-blocking stream) and waits for it. */
 int xbg_sur_pack_inputs(const float* zb, const float* zs, const float* uu, const float* vv, size_t n, double Hm0,
-                        double Tp, double dir, const float* norm14, void* x8);
+                        double Tp, double dir, const float* norm14, void* x8, void* stream);
 
 /* ------------------------------------------------------------------------
  * netCDF output (host code, no netCDF library; writenetcdf.cpp:32-239)

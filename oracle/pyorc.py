@@ -15,14 +15,19 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 # the same oracle with the host libm where the reference CPU path calls it
 LIBM_PATH = os.path.join(HERE, "liborc_libm.so")
+# the shared-math oracle with its per-cell loops on OpenMP threads
+OMP_PATH = os.path.join(HERE, "liborc_omp.so")
+_PATHS = {"shared": LIB_PATH, "libm": LIBM_PATH, "omp": OMP_PATH}
 _libs = {}
 
 
 def lib(variant: str = "shared"):
     """variant "shared": the deterministic functions the HIP kernels share;
-    "libm": the host libm as the reference's CPU path calls it."""
+    "libm": the host libm as the reference's CPU path calls it; "omp": the
+    "shared" functions with the per-cell loops on OpenMP threads (same
+    results, the multi-core CPU baseline)."""
     if variant not in _libs:
-        path = LIB_PATH if variant == "shared" else LIBM_PATH
+        path = _PATHS[variant]
         if not os.path.exists(path):
             raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
         L = C.CDLL(path)

@@ -48,6 +48,13 @@
  * Sediment_CPU.cpp compiled under oracle/_ref.  The exact FP32 rounding
  * sequence of the wave functions follows wave_CPU.cpp by reading; it cannot
  * be executed here (wave_CPU.cpp needs netcdf.h/fftw3.h, absent).
+ *
+ * Built with -DXBG_ORACLE_OMP -fopenmp (liborc_omp.so), the per-cell loops
+ * marked PFOR are shared out over OpenMP threads: each of them writes only
+ * its own cell from values no iteration of the same loop writes, so the
+ * results equal the sequential build's bit for bit (tests check it).  The
+ * in-place boundary copies, the ubnd1D column and the dt minima stay
+ * sequential.  That build is the multi-core CPU baseline.
  */
 #include <cmath>
 #include <cstdio>
@@ -59,6 +66,12 @@
 
 #include "../include/xbg.h"
 #include "../xbeach_gpu_amd/csrc/xbg_math.h"
+
+#ifdef XBG_ORACLE_OMP
+#define PFOR _Pragma("omp parallel for schedule(static)")
+#else
+#define PFOR
+#endif
 
 namespace {
 
@@ -279,11 +292,13 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:58-82 sanityCPU
     void sanity() {
+        PFOR
         for (size_t j = 0; j < N * nt; j++) ee[j] = tmax(ee[j], T(0.000f));
     }
 
     // wave_CPU.cpp:142-191 dispersionCPU
     void dispersion() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T L0 = twopi * g / (sigm[i] * sigm[i]);
             T L1 = L0, L2, err;
@@ -312,6 +327,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:194-207 calcwciCPU
     void calcwci() {
+        PFOR
         for (size_t i = 0; i < N; i++) wci[i] = wci_p * tmin(hh[i] / hwci, T(1.0f));
     }
 
@@ -326,6 +342,7 @@ template <class T> struct Dom {
     }
     // wave_CPU.cpp:209-224 slopesCPU
     void slopes() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -340,6 +357,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:259-287 propagthetaCPU
     void propagtheta() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T sig = sigm[i] / F::sinh(T(2.0f) * kh[i]);
             for (int th = 0; th < nt; th++) {
@@ -355,6 +373,7 @@ template <class T> struct Dom {
     // wave_CPU.cpp:289-309 actionCPU
     void action() {
         for (int th = 0; th < nt; th++)
+            PFOR
             for (size_t i = 0; i < N; i++) ee[i + th * N] = ee[i + th * N] / sigm[i];
     }
 
@@ -375,6 +394,7 @@ template <class T> struct Dom {
     // wave_CPU.cpp:367-468 xadvecupwind2CPU (speed = cg for ee, c for rr)
     void xadvecupwind2(const std::vector<T>& e, const std::vector<T>& spd) {
         T dxplus_i = T(1.0f) / dx;
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -405,6 +425,7 @@ template <class T> struct Dom {
     // wave_CPU.cpp:471-564 yadvecupwind2CPU
     void yadvecupwind2(const std::vector<T>& e, const std::vector<T>& spd) {
         T dxplus_i = T(1.0f) / dx;
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -434,6 +455,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:706-808 thetaadvecuw2hoCPU
     void thetaadvecuw2ho(const std::vector<T>& e) {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             if (nt <= 1) {
                 thetaadvec[i] = T(0.0f);
@@ -464,6 +486,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:810-828 eulerupwindCPU
     void eulerupwind(std::vector<T>& e, T dt) {
+        PFOR
         for (size_t j = 0; j < N * nt; j++)
             e[j] = e[j] - dt * (xadvec[j] + yadvec[j] + thetaadvec[j]);
     }
@@ -485,6 +508,7 @@ template <class T> struct Dom {
     // wave_CPU.cpp:873-895 energyCPU
     void energy() {
         for (int th = 0; th < nt; th++)
+            PFOR
             for (size_t i = 0; i < N; i++) {
                 T t = ee[i + th * N] * sigm[i];
                 ee[i + th * N] = tmax(t, T(0.0f));
@@ -493,6 +517,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:897-930 energintCPU
     void energint() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T Et = T(0.0f);
             T h = hh[i];
@@ -509,6 +534,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:932-985 roelvinkCPU (Qb and D in double as written)
     void roelvink() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T fac = (T)(8.0 / D(rho) / D(g));
             T h = hh[i];
@@ -525,6 +551,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:987-1027 baldockCPU (breakmodel != 1)
     void baldock() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T h = hh[i], ki = k[i], Ht = H[i];
             T khh = ki * h;
@@ -539,6 +566,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:1030-1089 dissipationCPU
     void dissipation(T dt) {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T RR = T(0.0f), drr = T(0.0f);
             T cc = c[i];
@@ -572,6 +600,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:1092-1116 meandirCPU
     void meandir() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T sumethet = T(0), sume = T(0);
             for (int th = 0; th < nt; th++) {
@@ -587,6 +616,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:1118-1157 radstressCPU
     void radstress() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T nn = cg[i] / c[i];
             T sume = 0, txx = 0, txy = 0, tyy = 0, rxx = 0, rxy = 0, ryy = 0;
@@ -609,6 +639,7 @@ template <class T> struct Dom {
 
     // wave_CPU.cpp:1159-1223 wavforceCPU
     void wavforce() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -640,6 +671,7 @@ template <class T> struct Dom {
     // wave_CPU.cpp:1334-1374 breakerdelayCPU (called with R, Wavestep.cu:1295)
     void breakerdelay() {
         T hmin = T(0.2);
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T Hi = H[i], ki = k[i];
             T hhi = tmax(hh[i], hmin);
@@ -734,6 +766,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:410-466
     void wlevslopes() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -744,6 +777,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:546-651
     void depthmomcont() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -758,6 +792,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:752-823
     void ududx_adv() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy), l = id(mm1(ix, nx), iy), r = id(pp1(ix, nx), iy);
@@ -780,6 +815,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:887-949
     void vdudy_adv() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xp = pp1(ix, nx), ym = mm1(iy, ny), yp = pp1(iy, ny);
@@ -803,6 +839,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1282-1317 (nuh holds smag when usesmago==1)
     void smago() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx), ym = mm1(iy, ny), yp = pp1(iy, ny);
@@ -824,6 +861,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1319-1397
     void viscou() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx);
@@ -832,6 +870,7 @@ template <class T> struct Dom {
                 T dudx2 = nuhg[i] * hh[i] * (uu[i] - uu[l]) / dx;
                 viscu[i] = (T(1.0f) / hum[i]) * ((dudx1 - dudx2) / (dx));
             }
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xp = pp1(ix, nx), ym = mm1(iy, ny), yp = pp1(iy, ny);
@@ -849,6 +888,7 @@ template <class T> struct Dom {
     void eulerustep(T dt) {
         T hmin = T(0.2);
         int signwind = (D(windu) > 0.0) - (D(windu) < 0.0);
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -883,6 +923,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1024-1100 vdvdy_adv + vdvdy_fixbnd
     void vdvdy_adv() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -914,6 +955,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1159-1210
     void udvdx_adv() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx), yp = pp1(iy, ny);
@@ -937,6 +979,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1399-1483 viscov1, viscovbnd, viscov2
     void viscov() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int ym = mm1(iy, ny), yp = pp1(iy, ny);
@@ -949,6 +992,7 @@ template <class T> struct Dom {
             viscv[id(ix, ny - 2)] = viscv[id(ix, ny - 3)];
             viscv[id(ix, 0)] = viscv[id(ix, 1)];
         }
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx), yp = pp1(iy, ny);
@@ -966,6 +1010,7 @@ template <class T> struct Dom {
     void eulervstep(T dt) {
         T hmin = T(0.2);
         int signwind = (D(windv) > 0.0) - (D(windv) < 0.0);
+        PFOR
         for (int iy = 0; iy < ny - 2; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -985,6 +1030,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1960-2044 wrkuu2Ocorr
     void wrkuu2Ocorr() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xm2 = mm2(ix, nx), xp = pp1(ix, nx), xp2 = pp2(ix, nx);
@@ -1016,6 +1062,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:2210-2245 uu2Ocorr
     void uu2Ocorr(T dt) {
+        PFOR
         for (int iy = 1; iy < ny; iy++)
             for (int ix = 1; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx), ym = mm1(iy, ny);
@@ -1031,6 +1078,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:2046-2128 wrkvv2Ocorr
     void wrkvv2Ocorr() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx), xp2 = pp2(ix, nx);
@@ -1062,6 +1110,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:2247-2282 vv2Ocorr
     void vv2Ocorr(T dt) {
+        PFOR
         for (int iy = 1; iy < ny; iy++)
             for (int ix = 1; ix < nx; ix++) {
                 int xm = mm1(ix, nx), ym = mm1(iy, ny), yp = pp1(iy, ny);
@@ -1078,6 +1127,7 @@ template <class T> struct Dom {
     // Flow_kernel.cu:653-750 depthhu / depthhv (uumin = 0, Wave_gpu.cu:63)
     void depthhuv() {
         const T ummn = T(0.0f);
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy), r = id(pp1(ix, nx), iy), t = id(ix, pp1(iy, ny));
@@ -1097,6 +1147,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1917-1958 CalcQFlow
     void calcqflow() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             qx[i] = uu[i] * hu[i];
             qy[i] = vv[i] * hv[i];
@@ -1105,6 +1156,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1626-1681 continuity
     void continuity(T dt) {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 1; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -1116,6 +1168,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:2130-2208 wrkzs2Ocorr
     void wrkzs2Ocorr() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx), xp2 = pp2(ix, nx);
@@ -1147,6 +1200,7 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:2284-2319 zs2Ocorr
     void zs2Ocorr(T dt) {
+        PFOR
         for (int iy = 1; iy < ny; iy++)
             for (int ix = 1; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -1158,11 +1212,13 @@ template <class T> struct Dom {
 
     // Flow_kernel.cu:1683-1710 hsbnd
     void hsbnd() {
+        PFOR
         for (size_t i = 0; i < N; i++) hh[i] = tmax(zb[i] + zs[i], eps);
     }
 
     // Flow_kernel.cu:467-541 calcuvvu
     void calcuvvu() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xm = mm1(ix, nx), xp = pp1(ix, nx), ym = mm1(iy, ny), yp = pp1(iy, ny);
@@ -1232,6 +1288,7 @@ template <class T> struct Dom {
     // Sediment_kernel.cu:27-165 longturb: kturb is computed and set to 0
     // (line 161); rolthick is the only live output.
     void longturb(T dt) {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T cr = beta * c[i];
             T rolth = rolthick[i] + dt * (F::fabs(dzsdt[i]) - cr);
@@ -1245,6 +1302,7 @@ template <class T> struct Dom {
         T zo = T(0.006f), sedcal = T(1.0f), bulk = T(1.0f);
         T drho = (rhosed - rho) / rho;
         T dester = F::pow(drho * g, T(1.0f) / T(3.0f)) / T(0.0001f) * D50;
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -1287,6 +1345,7 @@ template <class T> struct Dom {
         T m1 = T(0.0f), m2 = T(0.7939f), m3 = T(-0.6065f), m4 = T(0.3539f), m5 = T(0.6373f), m6 = T(0.5995f);
         T alph = T(-1.0f) * F::log10(F::exp(T(1.0f))) / m4;
         T bet = F::exp(m3 / m4);
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T kk = (T)(2 * PI_ / D(c[i] * Trep));
             T Ur = T(3.0f) / T(8.0f) * F::sqrt(T(2.0f)) * H[i] * kk / F::pow(kk * hh[i], T(3.0f));
@@ -1301,6 +1360,7 @@ template <class T> struct Dom {
 
     // Sediment_kernel.cu:512-531 Erosus
     void Erosus(T dt) {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T pbbed = T(1.0f);
             T exp_ero = morfac * dt / (T(1.0f) - por) * hh[i] * (ceqsg[i] * pbbed / Tsg[i] + ceqbg[i] * pbbed / dt);
@@ -1311,6 +1371,7 @@ template <class T> struct Dom {
     // Sediment_kernel.cu:534-717 Susp
     void Susp() {
         T facsl = T(1.6f), pbbed = T(1.0f);
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 int xp = pp1(ix, nx), yp = pp1(iy, ny);
@@ -1351,6 +1412,7 @@ template <class T> struct Dom {
     // Sediment_kernel.cu:721-779 Conc
     void Conc(T dt) {
         std::vector<T> Cn(Cc);
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -1382,6 +1444,7 @@ template <class T> struct Dom {
     // Sediment_kernel.cu:824-879 hardlayer (feeds only the commented-out
     // bed-load term of bedupdate; kept for the Sout/indSub/indSvb outputs)
     void hardlayer() {
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -1441,6 +1504,7 @@ template <class T> struct Dom {
     // Sediment_kernel.cu:881-964 bedupdate (bed-load divergence commented
     // out at line 954: only ero - depo moves the bed)
     void bedupdate(T dt) {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T dzg = morfac * dt / (T(1.0f) - por) * (ero[i] - depo[i]);
             zb_add(i, dzg);
@@ -1453,6 +1517,7 @@ template <class T> struct Dom {
     // Sediment_kernel.cu:1020-1137 avalanching, gather form
     void avalanching(T dt) {
         T maxchg = maxslpchg;
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy), r = id(pp1(ix, nx), iy), t = id(ix, pp1(iy, ny));
@@ -1485,6 +1550,7 @@ template <class T> struct Dom {
                 dzbx[i] = bx;
                 dzby[i] = by;
             }
+        PFOR
         for (int iy = 0; iy < ny; iy++)
             for (int ix = 0; ix < nx; ix++) {
                 size_t i = id(ix, iy);
@@ -1497,6 +1563,7 @@ template <class T> struct Dom {
 
     // Sediment_kernel.cu:1139-1159 updatezb
     void updatezb() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             T d = ddzb[i];
             zb_add(i, d);
@@ -1509,6 +1576,7 @@ template <class T> struct Dom {
 
     // Sediment_kernel.cu:1162-1189 updatezom == Sediment_CPU.cpp:13-40
     void updatezom() {
+        PFOR
         for (size_t i = 0; i < N; i++) {
             if (stdep[i] < T(0.05f)) { cfm[i] = cfreef; fwm[i] = fwreef; }
             else { cfm[i] = cfsand; fwm[i] = fwsand; }

@@ -2442,6 +2442,8 @@ int xbg_set_flow_forcing(xbg_ctx* c, const float* qbndold, const float* qbndnew,
 int xbg_set_bnd_series(xbg_ctx* c, const double* Stfile, const double* qfile, int nrec) {
     if (!c || !Stfile || !qfile || nrec < 2) return fail(c, XBG_E_ARG, "xbg_set_bnd_series: bad argument");
     size_t ns = (size_t)nrec * c->P.ny * c->P.ntheta, nq = (size_t)nrec * 4 * c->P.ny;
+    // steps still queued on the domain's stream read the old records
+    HIPC(c, hipStreamSynchronize(c->stream));
     if (c->Stfile) hipFree(c->Stfile);
     if (c->qfile) hipFree(c->qfile);
     c->Stfile = nullptr;

@@ -949,14 +949,18 @@ __global__ __launch_bounds__(256) void k_pack_inputs(const float* __restrict__ z
 }  // namespace
 
 extern "C" int xbg_sur_pack_inputs(const float* zb, const float* zs, const float* uu, const float* vv, size_t n,
-                                   double Hm0, double Tp, double dir, const float* norm14, void* x8) {
+                                   double Hm0, double Tp, double dir, const float* norm14, void* x8, void* stream) {
     if (!zb || !zs || !uu || !vv || !norm14 || !x8) return sfail(nullptr, XBG_E_ARG, "xbg_sur_pack_inputs: NULL");
     InNorm nm;
     for (int k = 0; k < 7; k++) {
         nm.mean[k] = norm14[k];
         nm.std[k] = norm14[7 + k];
     }
-    k_pack_inputs<<<nblk(n), 256, 0, nullptr>>>(zb, zs, uu, vv, n, (float)Hm0, (float)Tp, (float)dir, nm, (bf16*)x8);
-    if (hipDeviceSynchronize() != hipSuccess) return sfail(nullptr, XBG_E_HIP, "xbg_sur_pack_inputs: run");
+    // on the stream that produced the fields (a domain's stream is
+    // non-blocking: the null stream would not wait for its window means)
+    hipStream_t st = (hipStream_t)stream;
+    k_pack_inputs<<<nblk(n), 256, 0, st>>>(zb, zs, uu, vv, n, (float)Hm0, (float)Tp, (float)dir, nm, (bf16*)x8);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        return sfail(nullptr, XBG_E_HIP, "xbg_sur_pack_inputs: run");
     return XBG_OK;
 }

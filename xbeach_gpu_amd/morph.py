@@ -125,7 +125,7 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_windows: int = 40,
 
     def predict(Hm0, Tp, dgrid):
         rc = L.xbg_sur_pack_inputs(ptr["zb"], ptr["zsmean"], ptr["uumean"], ptr["vvmean"], nx * ny, Hm0, Tp, dgrid,
-                                   norm14.ctypes.data_as(C.POINTER(C.c_float)), x8.data_ptr())
+                                   norm14.ctypes.data_as(C.POINTER(C.c_float)), x8.data_ptr(), d.stream)
         if rc != 0:
             raise RuntimeError("xbg_sur_pack_inputs failed")
         sur.forward(x8.data_ptr(), 1, y.data_ptr())

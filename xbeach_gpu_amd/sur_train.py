@@ -86,6 +86,8 @@ def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, on_window, dt_m
             ens.window_add()
         ens.window_sample(win_steps, np.array(meta), raw.data_ptr())
         on_window(raw.view(len(specs), ny, nx, RAW_CH))
+        # torch's reads of raw finish before the next window overwrites it
+        torch.cuda.synchronize()
     ens.sync()
     gpu_s = time.perf_counter() - w0
     t, dt, steps, status = ens.status()
