@@ -9,15 +9,22 @@ of sigma 6 h centred at 12 h; Tp = 8 + 6 (Hm0 - 1)/3 s; mean direction from
 wave-group blocks (wavebndtype 4 with one record per block, regenerated per
 block as the reference regenerates per record, Wavestep.cu:512-753).
 
-Surrogate-assisted run: windows of one peak period; after a full-solver
-spin-up, each window's wave products come from the surrogate applied to the
-previous window's means (zb, zs, uu, vv) and the offshore sea state, and the
-flow, sediment and bed steps run without the wave action solve
-(xbg_step_nowave).  Every ``ckpt_every`` windows the full solver runs a
-re-spin window and a checkpoint window whose mean H judges the surrogate's
-prediction for it; an error above ``tol`` (relative L2 over wet cells) puts
-the run on the full solver, window by window, until the surrogate's
-prediction is within tol again.
+Surrogate-assisted run: windows of one peak period.  The run starts on the
+full solver (spin-up of ``spin_s`` hydrodynamic seconds, the time the wave
+field needs to cross the domain), then judges the surrogate: for ``check``
+full-solver windows the surrogate's prediction of each window (from the
+previous window's means of zb, zs, uu, vv and the offshore sea state) is
+kept beside the window's mean H, and the relative L2 error (wet cells) of
+the ``check``-window mean prediction against the ``check``-window mean H
+decides.  Wave groups make single windows differ by 15-25 % from their
+mean, so a single window cannot judge a mean-field surrogate; the mean over
+``check`` windows can.  Within ``tol`` the surrogate takes over: each window's
+wave products come from it and the flow, sediment and bed steps run without
+the wave action solve (xbg_step_nowave).  After ``ckpt_every`` surrogate
+windows the full solver runs ``respin`` windows (its wave field catches up
+with the bed, currents and sea state) and a checkpoint of ``check`` windows
+judged as above; above ``tol`` the run stays on the full solver, checkpoint
+after checkpoint, until the surrogate is within ``tol`` again.
 """
 from __future__ import annotations
 
@@ -99,10 +106,11 @@ def run_plain(p, blocks, hydro_s: float):
     return {"wall_s": wall, "steps": steps, "t": t}, zb0, zb
 
 
-def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_windows: int = 40, ckpt_every: int = 50,
-                  tol: float = 0.05, log=print):
+def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, ckpt_every: int = 50,
+                  respin: int = 4, check: int = 8, tol: float = 0.05, log=print):
     import torch
 
+    from .sur_train import phys_norm
     from .surrogate import Surrogate
 
     d = new_domain(p, blocks)
@@ -112,23 +120,26 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_windows: int = 40,
     sur.set_params(ckpt["params"])
     norm14 = np.asarray(ckpt["norm14"], np.float32)
     norm10 = np.asarray(ckpt["norm10"], np.float32)
+    tpow = np.asarray(ckpt.get("target_pow", np.zeros(5)), np.float64)
     x8 = torch.empty(ny * nx * 8, dtype=torch.bfloat16, device="cuda")
     y = torch.empty(ny * nx * 5, dtype=torch.float32, device="cuda")
     L = d.L
     ptr = {k: d.device_ptr(k) for k in ("zb", "zsmean", "uumean", "vvmean")}
     t, dt = 0.0, 0.05
-    w, cur_block = 0, -1
-    have_means, fallback = False, False
+    cur_block = -1
     stats = {"full_windows": 0, "sur_windows": 0, "steps_full": 0, "steps_sur": 0, "checkpoints": [],
-             "fallback_windows": 0}
+             "fallback_checks": 0}
     w0 = time.perf_counter()
 
     def predict(Hm0, Tp, dgrid):
+        """The surrogate's outputs for the coming window (in y) and the
+        physical normalisation for this sea state."""
         rc = L.xbg_sur_pack_inputs(ptr["zb"], ptr["zsmean"], ptr["uumean"], ptr["vvmean"], nx * ny, Hm0, Tp, dgrid,
                                    norm14.ctypes.data_as(C.POINTER(C.c_float)), x8.data_ptr(), d.stream)
         if rc != 0:
             raise RuntimeError("xbg_sur_pack_inputs failed")
         sur.forward(x8.data_ptr(), 1, y.data_ptr())
+        return phys_norm(norm10, tpow, Hm0)
 
     def window(mode, t_end):
         nonlocal t, dt
@@ -141,6 +152,8 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_windows: int = 40,
         stats["steps_" + key] += n
         return n
 
+    phase, count = "spin", 0
+    pred_sum = full_sum = None
     while t < hydro_s:
         b = blocks.index(t)
         if b != cur_block:
@@ -149,40 +162,50 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_windows: int = 40,
         Hm0, Tp, ddeg = blocks.state[b]
         dgrid = math.radians(ddeg)
         t_end = min(t + Tp, (b + 1) * blocks.block_s, hydro_s)
-        warm = have_means and w >= spin_windows
-        checkpoint = warm and (w % ckpt_every == 0)
-        if warm and not checkpoint and not fallback:
-            predict(Hm0, Tp, dgrid)
-            d.set_wave_products(y.data_ptr(), norm10, dgrid)
+        if phase == "spin":
+            window(1, t_end)
+            if t >= spin_s:
+                phase, count = "check", 0
+        elif phase == "sur":
+            nm = predict(Hm0, Tp, dgrid)
+            d.set_wave_products(y.data_ptr(), nm, dgrid)
             window(0, t_end)
-        elif warm:
-            if checkpoint:
-                window(1, t_end)   # the wave field catches up with the bed and currents
-                if t >= hydro_s:
-                    break
-                t_end = min(t + Tp, (b + 1) * blocks.block_s, hydro_s)
-            predict(Hm0, Tp, dgrid)
+            count += 1
+            if count >= ckpt_every:
+                phase, count = "respin", 0
+        elif phase == "respin":
             window(1, t_end)
-            Hp = y.view(ny, nx, 5)[..., 0].double() * float(norm10[5]) + float(norm10[0])
-            Hp = Hp.clamp_min(0.0).cpu().numpy()
-            Hf = d.get("Hmean").astype(np.float64)
-            wet = d.get("hh") > p.eps
-            err = float(np.linalg.norm((Hp - Hf)[wet]) / max(np.linalg.norm(Hf[wet]), 1e-30))
-            stats["checkpoints"].append((w, round(t, 2), err))
-            if fallback:
-                stats["fallback_windows"] += 1
-            fallback = err > tol
-            log(f"window {w} t={t:.0f} s checkpoint H error {err:.3f} -> {'full solver' if fallback else 'surrogate'}")
-        else:
+            count += 1
+            if count >= respin:
+                phase, count = "check", 0
+        else:  # check: full windows with the surrogate's predictions beside them
+            nm = predict(Hm0, Tp, dgrid)
             window(1, t_end)
-        have_means = True
-        w += 1
+            hh = torch.from_numpy(d.get("hh")).cuda()
+            wet = hh > p.eps
+            Hp = (y.view(ny, nx, 5)[..., 0].double() * float(nm[5]) + float(nm[0])).clamp_min(0.0)
+            Hp = torch.minimum(Hp, p.gammax * hh.double()) * wet
+            Hf = torch.from_numpy(d.get("Hmean")).cuda().double()
+            pred_sum = Hp if count == 0 else pred_sum + Hp
+            full_sum = Hf if count == 0 else full_sum + Hf
+            count += 1
+            if count >= check:
+                err = float(torch.linalg.norm((pred_sum - full_sum)[wet]) / torch.linalg.norm(full_sum[wet]).clamp_min(1e-30))
+                ok = err <= tol
+                stats["checkpoints"].append((round(t, 2), round(err, 4), bool(ok)))
+                log(f"t={t:.0f} s ({t * MORFAC / 3600.0:.2f} h morph.) checkpoint H error {err:.3f} over {check} "
+                    f"windows -> {'surrogate' if ok else 'full solver'}")
+                if ok:
+                    phase, count = "sur", 0
+                else:
+                    stats["fallback_checks"] += 1
+                    count = 0
     d.sync()
     wall = time.perf_counter() - w0
     zb = d.get("zb").astype(np.float64)
     sur.close()
     d.close()
-    stats.update({"wall_s": wall, "windows": w, "t": t})
+    stats.update({"wall_s": wall, "t": t})
     return stats, zb0, zb
 
 
@@ -205,6 +228,9 @@ def main():
     ap.add_argument("--ckpt", required=True, help=".npz from sur_train --save")
     ap.add_argument("--hours", type=float, default=24.0, help="morphological hours of the storm to run")
     ap.add_argument("--ckpt-every", type=int, default=50)
+    ap.add_argument("--respin", type=int, default=4)
+    ap.add_argument("--check", type=int, default=8)
+    ap.add_argument("--spin", type=float, default=600.0, help="full-solver spin-up (hydrodynamic s)")
     ap.add_argument("--tol", type=float, default=0.05)
     ap.add_argument("--save", default=None, help="write the initial and final bed levels (.npz)")
     a = ap.parse_args()
@@ -212,7 +238,8 @@ def main():
     p = site_params(hydro_s)
     blocks = StormBlocks(p, hydro_s)
     ck = dict(np.load(a.ckpt))
-    s, zb0, zs_ = run_surrogate(p, blocks, hydro_s, ck, ckpt_every=a.ckpt_every, tol=a.tol)
+    s, zb0, zs_ = run_surrogate(p, blocks, hydro_s, ck, spin_s=a.spin, ckpt_every=a.ckpt_every, respin=a.respin,
+                                check=a.check, tol=a.tol)
     pl, zb0b, zp = run_plain(p, blocks, hydro_s)
     out = {"hydro_s": hydro_s, "plain": pl, "surrogate": {k: v for k, v in s.items() if k != "checkpoints"},
            "checkpoints": s["checkpoints"], "speedup": pl["wall_s"] / s["wall_s"]}

@@ -13,8 +13,14 @@ of H, wave dissipation, roller dissipation, Fx, Fy.  Two families:
             beach) under sea states drawn from the storm's range with seeds
             other than the storm run's, cut into 256 x 256 crops
 
-Samples stay in HBM as BF16 (NHWC), normalised per channel with the mean and
-standard deviation of the first batch.
+Targets are made dimensionless with the offshore wave height before the
+normalisation: H / Hm0 and the energy-like products (Dw, DR, Fx, Fy) over
+Hm0^2 (``TARGET_POW``), so the network does not have to learn the scale of
+the sea state; inference multiplies back (``phys_norm``).  Samples stay in
+HBM as BF16 (NHWC), normalised per channel with the mean and standard
+deviation of the first batch.  The site family is kept at full size
+(1024 x 512: one sample per domain and window, the size the morphology run
+infers on); the ensemble family is 256 x 256.
 
 Training: Adam (lr 1e-3, batch 64) on the hand-written BF16 MFMA forward and
 backward kernels (xbg_sur_*), 80/20 split, held-out loss reported.  torch
@@ -33,6 +39,15 @@ import numpy as np
 IN_CH = 8   # zb, zs, uu, vv, Hm0, Tp, dir, 0
 OUT_CH = 5  # H, Dw, Dr, Fx, Fy
 RAW_CH = 12
+TARGET_POW = (1.0, 2.0, 2.0, 2.0, 2.0)  # targets divided by Hm0 ** pow before normalising
+
+
+def phys_norm(norm10, target_pow, Hm0: float) -> np.ndarray:
+    """Means and standard deviations (10 values) that turn the network's
+    normalised outputs into physical H, Dw, DR, Fx, Fy for offshore Hm0."""
+    n = np.asarray(norm10, np.float64)
+    sc = np.float64(Hm0) ** np.asarray(target_pow, np.float64)
+    return np.concatenate([n[:5] * sc, n[5:] * sc]).astype(np.float32)
 
 
 def site_specs(n: int, seed: int = 99):
@@ -121,14 +136,17 @@ class Dataset:
     def add(self, raw):
         import torch
 
-        flat = raw.reshape(-1, RAW_CH)
+        pw = torch.tensor(TARGET_POW, dtype=raw.dtype, device=raw.device)
+        tgt = raw[..., 7:] / raw[..., 4:5].clamp_min(0.1) ** pw
         if self.mean is None:
-            self.mean = flat.mean(0)
-            self.std = flat.std(0).clamp_min(1e-6)
+            fi = raw[..., :7].reshape(-1, 7)
+            ft = tgt.reshape(-1, OUT_CH)
+            self.mean = torch.cat([fi.mean(0), ft.mean(0)])
+            self.std = torch.cat([fi.std(0), ft.std(0)]).clamp_min(1e-6)
         m, s = self.mean, self.std
         X = torch.zeros(raw.shape[:3] + (IN_CH,), dtype=torch.bfloat16, device=raw.device)
         X[..., :7] = ((raw[..., :7] - m[:7]) / s[:7]).to(torch.bfloat16)
-        T = ((raw[..., 7:] - m[7:]) / s[7:]).to(torch.bfloat16).contiguous()
+        T = ((tgt - m[7:]) / s[7:]).to(torch.bfloat16).contiguous()
         self.X.append(X)
         self.T.append(T)
 
@@ -204,19 +222,19 @@ def main():
     # library opens the device
     torch.cuda.init()
     ap = argparse.ArgumentParser()
-    ap.add_argument("--family", default="ensemble", choices=["ensemble", "site", "both"])
+    ap.add_argument("--family", default="ensemble", choices=["ensemble", "site"])
     ap.add_argument("--batches", type=int, default=2)
     ap.add_argument("--domains", type=int, default=256)
     ap.add_argument("--windows", type=int, default=40)
     ap.add_argument("--win-steps", type=int, default=80)
     ap.add_argument("--spin-steps", type=int, default=600)
     ap.add_argument("--site-domains", type=int, default=16)
-    ap.add_argument("--site-batches", type=int, default=1)
+    ap.add_argument("--site-batches", type=int, default=4)
     ap.add_argument("--site-windows", type=int, default=40)
     ap.add_argument("--site-win-steps", type=int, default=130)
-    ap.add_argument("--site-spin-steps", type=int, default=4000)
+    ap.add_argument("--site-spin-steps", type=int, default=8000)
     ap.add_argument("--epochs", type=int, default=10)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=None, help="default 64 (ensemble), 8 (site, full size)")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--save", default=None, help="write parameters and normalisation (.npz)")
     a = ap.parse_args()
@@ -228,31 +246,31 @@ def main():
         info["gen_host_s"] += h
         info["gen_cell_steps"] += c
 
-    def add(raw):
-        ds.add(crops(raw))
-
-    if a.family in ("ensemble", "both"):
+    if a.family == "ensemble":
+        batch = a.batch or 64
         for b in range(a.batches):
             specs = workloads.ensemble_specs(a.domains, seed=7000 + b, fixed_size=(256, 256, 4.0))
-            take(*run_batch(specs, a.windows, a.win_steps, a.spin_steps, add))
+            take(*run_batch(specs, a.windows, a.win_steps, a.spin_steps, lambda r: ds.add(crops(r))))
             print(f"ensemble batch {b} done", flush=True)
-    if a.family in ("site", "both"):
+    else:
+        batch = a.batch or 8
         for b in range(a.site_batches):
             take(*run_batch(site_specs(a.site_domains, seed=99 + b), a.site_windows, a.site_win_steps,
-                            a.site_spin_steps, add, dt_max=0.1))
+                            a.site_spin_steps, ds.add, dt_max=0.1))
             print(f"site batch {b} done", flush=True)
     X, T = ds.tensors()
-    print(f"{X.shape[0]} samples", flush=True)
-    sur, hist, train_s, held, nstep = train(X, T, a.epochs, a.batch, a.lr)
+    print(f"{X.shape[0]} samples of {X.shape[1]} x {X.shape[2]}", flush=True)
+    sur, hist, train_s, held, nstep = train(X, T, a.epochs, batch, a.lr)
     # forward 18.9 GFLOP per 256 x 256 image (the 17 convolutions); training
     # runs forward, data and weight gradients: about three times that
-    flops = 3 * 18.9e9 * X.shape[1] * X.shape[2] / 65536.0 * a.batch * nstep
+    flops = 3 * 18.9e9 * X.shape[1] * X.shape[2] / 65536.0 * batch * nstep
     out = {"samples": int(X.shape[0]), "family": a.family, "gen_gpu_s": info["gen_gpu_s"],
            "gen_host_s": info["gen_host_s"], "gen_cell_steps_per_s": info["gen_cell_steps"] / info["gen_gpu_s"],
            "train_s": train_s, "train_steps": nstep, "train_tflops_est": flops / train_s / 1e12,
            "train_loss": hist, "heldout_loss": held, "norm14": ds.norm14().tolist(), "norm10": ds.norm10().tolist()}
     if a.save:
-        np.savez(a.save, params=sur.get_params(), norm14=ds.norm14(), norm10=ds.norm10(), heldout=held)
+        np.savez(a.save, params=sur.get_params(), norm14=ds.norm14(), norm10=ds.norm10(), heldout=held,
+                 target_pow=np.asarray(TARGET_POW, np.float32))
     print(json.dumps(out))
 
 
