@@ -182,7 +182,7 @@ def test_morphology_checkpoints_and_fallback():
     st1, _, zb_b = morph.run_surrogate(p, blocks, 150.0, ck, spin_s=30.0, ckpt_every=3, respin=1, check=2,
                                        tol=1e30, log=logs.append)
     assert st1["sur_windows"] >= 3 and st1["steps_sur"] > 0 and st1["fallback_checks"] == 0
-    assert all(c[2] for c in st1["checkpoints"])
+    assert all(c[3] for c in st1["checkpoints"])
     assert np.isfinite(zb_b).all() and st1["t"] == pytest.approx(150.0)
     # the surrogate changed the run (its wave products are not the solver's)
     assert not np.array_equal(zb_a, zb_b)

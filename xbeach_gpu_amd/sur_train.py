@@ -167,9 +167,10 @@ class Dataset:
         return np.concatenate([m[7:], s[7:]]).astype(np.float32)
 
 
-def train(X, T, epochs: int, batch: int = 64, lr: float = 1e-3, seed: int = 0, log=print):
-    """Adam on 80 % of the samples; returns (surrogate, per-epoch train
-    loss, train seconds, held-out loss, steps)."""
+def train(X, T, epochs: int, batch: int = 64, lr: float = 1e-3, seed: int = 0, log=print, lr_final=None):
+    """Adam on 80 % of the samples, at learning rate lr (with lr_final set,
+    cosine decay from lr to lr_final over the run); returns (surrogate,
+    per-epoch train loss, train seconds, held-out loss, steps)."""
     import torch
 
     from .surrogate import Surrogate
@@ -183,6 +184,8 @@ def train(X, T, epochs: int, batch: int = 64, lr: float = 1e-3, seed: int = 0, l
     hist = []
     w0 = time.perf_counter()
     nstep = 0
+    total = epochs * ((ntr - batch) // batch + 1)
+    lr_end = lr if lr_final is None else lr_final
     for ep in range(epochs):
         order = tr[torch.randperm(ntr, generator=g).cuda()]
         tot = 0.0
@@ -191,7 +194,8 @@ def train(X, T, epochs: int, batch: int = 64, lr: float = 1e-3, seed: int = 0, l
             idx = order[k:k + batch]
             xb = X.index_select(0, idx).contiguous()
             tb = T.index_select(0, idx).contiguous()
-            tot += sur.train_step(xb.data_ptr(), tb.data_ptr(), batch, lr)
+            lr_k = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * nstep / max(total - 1, 1)))
+            tot += sur.train_step(xb.data_ptr(), tb.data_ptr(), batch, lr_k)
             nb += 1
             nstep += 1
         sur.sync()
@@ -236,6 +240,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--batch", type=int, default=None, help="default 64 (ensemble), 8 (site, full size)")
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-final", type=float, default=None, help="cosine decay to this learning rate")
     ap.add_argument("--save", default=None, help="write parameters and normalisation (.npz)")
     a = ap.parse_args()
     ds = Dataset()
@@ -260,7 +265,7 @@ def main():
             print(f"site batch {b} done", flush=True)
     X, T = ds.tensors()
     print(f"{X.shape[0]} samples of {X.shape[1]} x {X.shape[2]}", flush=True)
-    sur, hist, train_s, held, nstep = train(X, T, a.epochs, batch, a.lr)
+    sur, hist, train_s, held, nstep = train(X, T, a.epochs, batch, a.lr, lr_final=a.lr_final)
     # forward 18.9 GFLOP per 256 x 256 image (the 17 convolutions); training
     # runs forward, data and weight gradients: about three times that
     flops = 3 * 18.9e9 * X.shape[1] * X.shape[2] / 65536.0 * batch * nstep
