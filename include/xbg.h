@@ -169,6 +169,17 @@ int xbg_set_flow_forcing(xbg_ctx* ctx, const float* qbndold, const float* qbndne
                          double timesincelast, double timenext,
                          double zsbndi, double zsbndn, double windu, double windv);
 
+/* Snapshot of the whole device state of a domain — every field, running
+ * sum and partial-result buffer the context allocated, with the host-side
+ * buffer parities and boundary scalars — by stream-ordered device-to-device
+ * copies into buffers allocated on the first call (one more copy of the
+ * domain's device memory); xbg_state_restore puts it back.  The resident
+ * boundary records (xbg_set_bnd_series) are not part of it.  No counterpart
+ * in the reference: the surrogate-assisted morphology run returns to its
+ * last checkpoint when a window on the surrogate's products goes unstable. */
+int xbg_state_save(xbg_ctx* ctx);
+int xbg_state_restore(xbg_ctx* ctx);
+
 /*
  * Resident boundary records (no counterpart in the reference, which copies
  * St and qbnd host->device every step, Wavestep.cu:749-779): upload a whole

@@ -52,7 +52,10 @@ def products(seed, ny, nx):
 
 def expected_products(p, y, norm, h, kk, cc, Trep):
     """k_wave_from_sur restated in float32, operation by operation; urms
-    and ust need sinhf and are returned for a tolerance check."""
+    and ust need sinhf and are returned for a tolerance check.  Rows 0 and
+    ny-1 take the predictions of rows 1 and ny-2."""
+    y = y.copy()
+    y[0], y[-1] = y[1], y[-2]
     q = [y[..., j] for j in range(5)]
     m, s = norm[:5].astype(F32), norm[5:].astype(F32)
     wet = (h > F32(p.eps)).astype(F32)
@@ -186,3 +189,36 @@ def test_morphology_checkpoints_and_fallback():
     assert np.isfinite(zb_b).all() and st1["t"] == pytest.approx(150.0)
     # the surrogate changed the run (its wave products are not the solver's)
     assert not np.array_equal(zb_a, zb_b)
+
+
+def test_state_save_and_restore():
+    """xbg_state_restore returns the domain to the xbg_state_save snapshot:
+    the same steps after it reproduce the same fields bit for bit (ping-pong
+    parities, FP64 bed, running means, boundary scalars included)."""
+    from xbeach_gpu_amd import XbgError
+
+    p, d, o, frc, t = spun_pair(nx=96, ny=40, morfac=10.0, zb_accum=2)
+    o.close()
+    with pytest.raises(XbgError):
+        d.state_restore()
+    d.reset_means()
+    d.set_flow_forcing(frc.qbnd(t - 0.5), frc.qbnd(t + 0.5), 0.3, 1.0, zsbndi=0.1, zsbndn=0.2)
+    d.state_save()
+
+    def run(t, n=3, dt=0.12):
+        for k in range(n):
+            t += dt
+            if k:
+                d.set_wave_forcing(frc.St(t), frc.Trep)
+                d.set_flow_forcing(frc.qbnd(t - 0.5), frc.qbnd(t + 0.5), 0.5, 1.0)
+            d.step(dt, t)
+            d.accumulate_means()
+        return {k: d.get(k) for k in ("zs", "uu", "vv", "ee", "rr", "H", "C", "zb", "hh", "Hmean", "zsmean")}
+
+    a = run(t)
+    d.state_restore()
+    b = run(t)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    assert not np.array_equal(a["zs"], b["zs"] * 0)
+    d.close()

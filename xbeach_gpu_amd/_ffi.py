@@ -162,6 +162,8 @@ PROTOTYPES = {
     "xbg_sur_pack_inputs": (C.c_int, [C.c_void_p] * 4 + [C.c_size_t, C.c_double, C.c_double, C.c_double, _F,
                                                          C.c_void_p, C.c_void_p]),
     "xbg_field_device": (C.c_void_p, [_CTX, C.c_char_p]),
+    "xbg_state_save": (C.c_int, [_CTX]),
+    "xbg_state_restore": (C.c_int, [_CTX]),
     "xbg_set_wave_products": (C.c_int, [_CTX, C.c_void_p, _F, C.c_double]),
     "xbg_step_nowave": (C.c_int, [_CTX, C.c_double, C.c_double]),
     "xbg_advance_window": (C.c_int, [_CTX, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,

@@ -1948,7 +1948,15 @@ struct xbg_ctx {
     hipEvent_t cfl_done = nullptr;  // the CFL partials have reached the host
     size_t N = 0, ND = 0;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;  // sizes of allocs, same order
+    // xbg_state_save: a device copy of every allocation and the host-side
+    // buffer parities and boundary scalars
+    std::vector<void*> snap;
+    int snap_cur = 0, snap_ccur = 0, snap_zcur = 0, snap_acc = 0;
+    float snap_sc[7] = {};
+    bool snap_valid = false;
     Fld f;
+    Fld snap_f;
     float* ee[2] = {nullptr, nullptr};
     float* rr[2] = {nullptr, nullptr};
     float* Cb[2] = {nullptr, nullptr};
@@ -2053,6 +2061,7 @@ int dalloc(xbg_ctx* c, T** p, size_t n) {
     if (e != hipSuccess) return fail(c, XBG_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     hipMemset(q, 0, n * sizeof(T));
     c->allocs.push_back(q);
+    c->alloc_bytes.push_back(n * sizeof(T));
     *p = (T*)q;
     return 0;
 }
@@ -2354,6 +2363,7 @@ int xbg_destroy(xbg_ctx* c) {
     if (!c) return XBG_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void* q : c->allocs) hipFree(q);
+    for (void* q : c->snap) hipFree(q);
     for (auto& e : c->evpool) {
         hipEventDestroy(e.a);
         hipEventDestroy(e.b);
@@ -2436,6 +2446,59 @@ int xbg_set_flow_forcing(xbg_ctx* c, const float* qbndold, const float* qbndnew,
     c->zsbndn = (float)zsbndn;
     c->windu = (float)windu;
     c->windv = (float)windv;
+    return XBG_OK;
+}
+
+// Snapshot of the whole device state of a domain (every field, running
+// sum and partial-result buffer the context allocated) and of the host-side
+// buffer parities and boundary scalars: stream-ordered device-to-device
+// copies into buffers allocated on the first call.  No counterpart in the
+// reference: the surrogate-assisted morphology run returns to its last
+// checkpoint when a window on the surrogate's products goes unstable.
+int xbg_state_save(xbg_ctx* c) {
+    if (!c) return XBG_E_ARG;
+    if (c->snap.empty()) {
+        for (size_t k = 0; k < c->allocs.size(); k++) {
+            void* q = nullptr;
+            hipError_t e = hipMalloc(&q, c->alloc_bytes[k] > 0 ? c->alloc_bytes[k] : 16);
+            if (e != hipSuccess) {
+                for (void* r : c->snap) hipFree(r);
+                c->snap.clear();
+                return fail(c, XBG_E_HIP, std::string("xbg_state_save: hipMalloc: ") + hipGetErrorString(e));
+            }
+            c->snap.push_back(q);
+        }
+    }
+    for (size_t k = 0; k < c->allocs.size(); k++)
+        HIPC(c, hipMemcpyAsync(c->snap[k], c->allocs[k], c->alloc_bytes[k], hipMemcpyDeviceToDevice, c->stream));
+    c->snap_cur = c->cur;
+    c->snap_ccur = c->ccur;
+    c->snap_zcur = c->zcur;
+    c->snap_acc = c->acc_cur;
+    const float sc[7] = {c->Trep, c->tsl, c->tnext, c->zsbndi, c->zsbndn, c->windu, c->windv};
+    std::memcpy(c->snap_sc, sc, sizeof(sc));
+    c->snap_f = c->f;
+    c->snap_valid = true;
+    return XBG_OK;
+}
+
+int xbg_state_restore(xbg_ctx* c) {
+    if (!c) return XBG_E_ARG;
+    if (!c->snap_valid) return fail(c, XBG_E_ARG, "xbg_state_restore: no saved state");
+    for (size_t k = 0; k < c->allocs.size(); k++)
+        HIPC(c, hipMemcpyAsync(c->allocs[k], c->snap[k], c->alloc_bytes[k], hipMemcpyDeviceToDevice, c->stream));
+    c->cur = c->snap_cur;
+    c->ccur = c->snap_ccur;
+    c->zcur = c->snap_zcur;
+    c->acc_cur = c->snap_acc;
+    c->Trep = c->snap_sc[0];
+    c->tsl = c->snap_sc[1];
+    c->tnext = c->snap_sc[2];
+    c->zsbndi = c->snap_sc[3];
+    c->zsbndn = c->snap_sc[4];
+    c->windu = c->snap_sc[5];
+    c->windv = c->snap_sc[6];
+    c->f = c->snap_f;
     return XBG_OK;
 }
 
@@ -2761,7 +2824,10 @@ struct SurNorm {
 // (energintCPU); R = DR c / (2 g beta), the roller energy whose dissipation
 // 2 g beta R / c (dissipationCPU, wave_CPU.cpp:1030-1089) is DR; urms and
 // the Stokes drift of breakerdelayCPU (wave_CPU.cpp:1334-1374);
-// thetamean the offshore mean direction.  Dry cells get no waves.
+// thetamean the offshore mean direction.  Dry cells get no waves.  Rows 0
+// and ny-1 take the predictions of rows 1 and ny-2, as the wave step copies
+// its lateral rows (rollerlatbnd / twodimbnd, wave_CPU.cpp:830-871,
+// 1224-1331).
 __global__ __launch_bounds__(256) void k_wave_from_sur(KP p, SP s, Fld f, const float* __restrict__ y, SurNorm nm,
                                                        float dir) {
     const BI bi{(int)blockIdx.x, (int)blockIdx.y};
@@ -2774,7 +2840,8 @@ __global__ __launch_bounds__(256) void k_wave_from_sur(KP p, SP s, Fld f, const 
     f.k[i] = kk;
     f.c[i] = cc;
     f.cg[i] = cgg;
-    const float* q = y + i * 5;
+    const int ys = p.ny < 3 ? yy : (yy == 0 ? 1 : (yy == p.ny - 1 ? p.ny - 2 : yy));
+    const float* q = y + ((size_t)x + (size_t)ys * p.nx) * 5;
     const float wet = h > p.eps ? 1.0f : 0.0f;
     // the depth limitation of energintCPU (wave_CPU.cpp:897-930): H at most
     // gammax h, the energy-like products scaled by the same (Hmax/H)^2

@@ -219,6 +219,14 @@ class Domain:
         self._chk(self.L.xbg_mass_balance(self._ctx, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def state_save(self) -> None:
+        """Snapshot of the whole device state (xbg_state_save)."""
+        self._chk(self.L.xbg_state_save(self._ctx))
+
+    def state_restore(self) -> None:
+        """Back to the last snapshot (xbg_state_restore)."""
+        self._chk(self.L.xbg_state_restore(self._ctx))
+
     def sync(self) -> None:
         self._chk(self.L.xbg_sync(self._ctx))
 

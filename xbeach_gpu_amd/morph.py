@@ -12,9 +12,12 @@ block as the reference regenerates per record, Wavestep.cu:512-753).
 Surrogate-assisted run: windows of one peak period.  The run starts on the
 full solver (spin-up of ``spin_s`` hydrodynamic seconds, the time the wave
 field needs to cross the domain), then judges the surrogate: for ``check``
-full-solver windows the surrogate's prediction of each window (from the
-previous window's means of zb, zs, uu, vv and the offshore sea state) is
-kept beside the window's mean H.  Wave groups make the mean H of single
+full-solver windows the surrogate's prediction of each window is kept
+beside the window's mean H.  The network's inputs are the current zb and
+offshore sea state and the zs, uu, vv means of the last full-solver window:
+windows run on the surrogate's products never feed back into its inputs
+(they would close a loop in which the surrogate's errors drive currents
+that drive its next prediction further out of its training range).  Wave groups make the mean H of single
 windows differ by 15-25 % from their mean over many windows, so a single
 window cannot judge a mean-field surrogate.  The checkpoint statistic is the
 relative L2 error (wet cells) of the ``check``-window mean prediction P
@@ -33,6 +36,12 @@ windows the full solver runs ``respin`` windows (its wave field catches up
 with the bed, currents and sea state) and a checkpoint of ``check`` windows
 judged as above; above ``tol`` the run stays on the full solver, checkpoint
 after checkpoint, until the surrogate is within ``tol`` again.
+
+Stability guard: when the surrogate takes over, the whole device state is
+saved (xbg_state_save).  A surrogate window that crashes (non-finite time
+step) or ends with a current faster than ``vfac`` times the fastest of the
+checkpoint windows sends the run back to that state (xbg_state_restore) and
+onto the full solver for ``ckpt_every`` windows, then to a new checkpoint.
 """
 from __future__ import annotations
 
@@ -128,9 +137,10 @@ def checkpoint_error(P, F, F2, K: int, wet):
 
 
 def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, ckpt_every: int = 50,
-                  respin: int = 4, check: int = 12, tol: float = 0.05, log=print):
+                  respin: int = 4, check: int = 12, tol: float = 0.05, vfac: float = 2.0, log=print):
     import torch
 
+    from .domain import ModelCrashed
     from .sur_train import phys_norm
     from .surrogate import Surrogate
 
@@ -145,17 +155,20 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, 
     x8 = torch.empty(ny * nx * 8, dtype=torch.bfloat16, device="cuda")
     y = torch.empty(ny * nx * 5, dtype=torch.float32, device="cuda")
     L = d.L
-    ptr = {k: d.device_ptr(k) for k in ("zb", "zsmean", "uumean", "vvmean")}
+    zb_ptr = d.device_ptr("zb")
+    # zs, uu, vv window means of the last full-solver window
+    ref = {k: torch.zeros(ny * nx, dtype=torch.float32, device="cuda") for k in ("zsmean", "uumean", "vvmean")}
     t, dt = 0.0, 0.05
     cur_block = -1
     stats = {"full_windows": 0, "sur_windows": 0, "steps_full": 0, "steps_sur": 0, "checkpoints": [],
-             "fallback_checks": 0}
+             "fallback_checks": 0, "rollbacks": [], "sur_windows_kept": 0}
     w0 = time.perf_counter()
 
     def predict(Hm0, Tp, dgrid):
         """The surrogate's outputs for the coming window (in y) and the
         physical normalisation for this sea state."""
-        rc = L.xbg_sur_pack_inputs(ptr["zb"], ptr["zsmean"], ptr["uumean"], ptr["vvmean"], nx * ny, Hm0, Tp, dgrid,
+        rc = L.xbg_sur_pack_inputs(zb_ptr, ref["zsmean"].data_ptr(), ref["uumean"].data_ptr(),
+                                   ref["vvmean"].data_ptr(), nx * ny, Hm0, Tp, dgrid,
                                    norm14.ctypes.data_as(C.POINTER(C.c_float)), x8.data_ptr(), d.stream)
         if rc != 0:
             raise RuntimeError("xbg_sur_pack_inputs failed")
@@ -168,13 +181,21 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, 
         f = blocks.blocks[cur_block]
         t, dt, n = d.advance_window(mode, f.t0, f.dtbc, f.Trep, t_end, hydro_s, True, t, dt)
         d.divide_means(max(n, 1))
+        if mode:
+            for k, v in ref.items():
+                v.copy_(torch.from_numpy(d.get(k).reshape(-1)))
         key = "full" if mode else "sur"
         stats[key + "_windows"] += 1
         stats["steps_" + key] += n
         return n
 
+    def vmax():
+        return float(max(np.abs(d.get("uu")).max(), np.abs(d.get("vv")).max()))
+
     phase, count = "spin", 0
     pred_sum = full_sum = full_sq = None
+    saved = None      # (t, dt, block) of the state saved when the surrogate took over
+    v_check = 0.0     # fastest current at the end of the checkpoint windows
     while t < hydro_s:
         b = blocks.index(t)
         if b != cur_block:
@@ -190,14 +211,28 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, 
         elif phase == "sur":
             nm = predict(Hm0, Tp, dgrid)
             d.set_wave_products(y.data_ptr(), nm, dgrid)
-            window(0, t_end)
+            try:
+                window(0, t_end)
+                v = vmax()
+                bad = not (v <= vfac * v_check)
+            except ModelCrashed:
+                v, bad = float("nan"), True
             count += 1
-            if count >= ckpt_every:
+            if bad:
+                d.state_restore()
+                t, dt, cur_block = saved
+                d.set_bnd_series(blocks.blocks[cur_block].S, blocks.blocks[cur_block].Q)
+                stats["rollbacks"].append((round(t, 2), count, v))
+                log(f"t={t:.0f} s: surrogate window {count} unstable (max current {v:.2f} m/s, checkpoint "
+                    f"{v_check:.2f}) -> back to the checkpoint, full solver")
+                phase, count = "fallback", 0
+            elif count >= ckpt_every:
+                stats["sur_windows_kept"] += count
                 phase, count = "respin", 0
-        elif phase == "respin":
+        elif phase in ("respin", "fallback"):
             window(1, t_end)
             count += 1
-            if count >= respin:
+            if count >= (respin if phase == "respin" else ckpt_every):
                 phase, count = "check", 0
         else:  # check: full windows with the surrogate's predictions beside them
             nm = predict(Hm0, Tp, dgrid)
@@ -210,6 +245,7 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, 
             pred_sum = Hp if count == 0 else pred_sum + Hp
             full_sum = Hf if count == 0 else full_sum + Hf
             full_sq = Hf * Hf if count == 0 else full_sq + Hf * Hf
+            v_check = vmax() if count == 0 else max(v_check, vmax())
             count += 1
             if count >= check:
                 err, raw = checkpoint_error(pred_sum / check, full_sum / check, full_sq / check, check, wet)
@@ -218,10 +254,14 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, 
                 log(f"t={t:.0f} s ({t * MORFAC / 3600.0:.2f} h morph.) checkpoint H error {err:.3f} "
                     f"(raw {raw:.3f}) over {check} windows -> {'surrogate' if ok else 'full solver'}")
                 if ok:
+                    d.state_save()
+                    saved = (t, dt, cur_block)
                     phase, count = "sur", 0
                 else:
                     stats["fallback_checks"] += 1
                     count = 0
+    if phase == "sur":
+        stats["sur_windows_kept"] += count
     d.sync()
     wall = time.perf_counter() - w0
     zb = d.get("zb").astype(np.float64)
@@ -254,6 +294,7 @@ def main():
     ap.add_argument("--check", type=int, default=12)
     ap.add_argument("--spin", type=float, default=600.0, help="full-solver spin-up (hydrodynamic s)")
     ap.add_argument("--tol", type=float, default=0.05)
+    ap.add_argument("--vfac", type=float, default=2.0, help="stability bound on the current, times the checkpoint's")
     ap.add_argument("--save", default=None, help="write the initial and final bed levels (.npz)")
     a = ap.parse_args()
     hydro_s = a.hours * 3600.0 / MORFAC
@@ -261,7 +302,7 @@ def main():
     blocks = StormBlocks(p, hydro_s)
     ck = dict(np.load(a.ckpt))
     s, zb0, zs_ = run_surrogate(p, blocks, hydro_s, ck, spin_s=a.spin, ckpt_every=a.ckpt_every, respin=a.respin,
-                                check=a.check, tol=a.tol)
+                                check=a.check, tol=a.tol, vfac=a.vfac)
     pl, zb0b, zp = run_plain(p, blocks, hydro_s)
     out = {"hydro_s": hydro_s, "plain": pl, "surrogate": {k: v for k, v in s.items() if k != "checkpoints"},
            "checkpoints": s["checkpoints"], "speedup": pl["wall_s"] / s["wall_s"]}

@@ -16,9 +16,11 @@ of H, wave dissipation, roller dissipation, Fx, Fy.  Two families:
 Targets are made dimensionless with the offshore wave height before the
 normalisation: H / Hm0 and the energy-like products (Dw, DR, Fx, Fy) over
 Hm0^2 (``TARGET_POW``), so the network does not have to learn the scale of
-the sea state; inference multiplies back (``phys_norm``).  Samples stay in
-HBM as BF16 (NHWC), normalised per channel with the mean and standard
-deviation of the first batch.  The site family is kept at full size
+the sea state; inference multiplies back (``phys_norm``).  In the site
+family the zs, uu, vv inputs of a sample are the means of an earlier window
+(``ref_every``), matching the morphology run.  Samples stay in HBM as BF16
+(NHWC), normalised per channel with the mean and standard deviation of the
+first batch.  The site family is kept at full size
 (1024 x 512: one sample per domain and window, the size the morphology run
 infers on); the ensemble family is 256 x 256.
 
@@ -63,11 +65,17 @@ def site_specs(n: int, seed: int = 99):
     return out
 
 
-def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, on_window, dt_max: float = 0.16):
+def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, on_window, dt_max: float = 0.16,
+              ref_every: int = 0):
     """One ensemble of the given domains (all the same size): spin-up, then
     nwin windows, each window's samples (raw FP32 [ndom, ny, nx, 12] on the
     GPU, reused between windows) handed to on_window; returns (GPU seconds,
-    host set-up seconds, cell-steps)."""
+    host set-up seconds, cell-steps).  With ref_every = L the windows form
+    segments of L: the first window of a segment is the reference, and the
+    other L - 1 windows are samples whose zs, uu, vv inputs are the
+    reference window's means (zb, the sea state and the targets their own),
+    as the surrogate-assisted run feeds the network the last full-solver
+    window's hydrodynamics."""
     import torch
 
     from . import Domain, workloads
@@ -95,12 +103,20 @@ def run_batch(specs, nwin: int, win_steps: int, spin_steps: int, on_window, dt_m
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     ens.step(spin_steps)
+    ref = None
     for w in range(nwin):
         for _ in range(win_steps):
             ens.step(1)
             ens.window_add()
         ens.window_sample(win_steps, np.array(meta), raw.data_ptr())
-        on_window(raw.view(len(specs), ny, nx, RAW_CH))
+        r = raw.view(len(specs), ny, nx, RAW_CH)
+        if ref_every > 1:
+            if w % ref_every == 0:
+                ref = r[..., 1:4].clone()
+                continue
+            r = r.clone()
+            r[..., 1:4] = ref
+        on_window(r)
         # torch's reads of raw finish before the next window overwrites it
         torch.cuda.synchronize()
     ens.sync()
@@ -237,6 +253,8 @@ def main():
     ap.add_argument("--site-windows", type=int, default=40)
     ap.add_argument("--site-win-steps", type=int, default=130)
     ap.add_argument("--site-spin-steps", type=int, default=8000)
+    ap.add_argument("--ref-every", type=int, default=10,
+                    help="site family: zs, uu, vv inputs held from the first window of each segment of this many")
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--batch", type=int, default=None, help="default 64 (ensemble), 8 (site, full size)")
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -261,7 +279,7 @@ def main():
         batch = a.batch or 8
         for b in range(a.site_batches):
             take(*run_batch(site_specs(a.site_domains, seed=99 + b), a.site_windows, a.site_win_steps,
-                            a.site_spin_steps, ds.add, dt_max=0.1))
+                            a.site_spin_steps, ds.add, dt_max=0.1, ref_every=a.ref_every))
             print(f"site batch {b} done", flush=True)
     X, T = ds.tensors()
     print(f"{X.shape[0]} samples of {X.shape[1]} x {X.shape[2]}", flush=True)
