@@ -13,11 +13,12 @@ Surrogate-assisted run: windows of one peak period.  The run starts on the
 full solver (spin-up of ``spin_s`` hydrodynamic seconds, the time the wave
 field needs to cross the domain), then judges the surrogate: for ``check``
 full-solver windows the surrogate's prediction of each window is kept
-beside the window's mean H.  The network's inputs are the current zb and
-offshore sea state and the zs, uu, vv means of the last full-solver window:
-windows run on the surrogate's products never feed back into its inputs
-(they would close a loop in which the surrogate's errors drive currents
-that drive its next prediction further out of its training range).  Wave groups make the mean H of single
+beside the window's mean H.  The network's inputs are the current offshore
+sea state and the zb, zs, uu, vv of the last full-solver window (the bed at
+its end, the others its means): windows run on the surrogate's products
+never feed back into its inputs.  They would close a loop in which the
+surrogate's errors drive currents and bed changes that drive its next
+prediction further out of its training range.  Wave groups make the mean H of single
 windows differ by 15-25 % from their mean over many windows, so a single
 window cannot judge a mean-field surrogate.  The checkpoint statistic is the
 relative L2 error (wet cells) of the ``check``-window mean prediction P
@@ -155,9 +156,8 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, 
     x8 = torch.empty(ny * nx * 8, dtype=torch.bfloat16, device="cuda")
     y = torch.empty(ny * nx * 5, dtype=torch.float32, device="cuda")
     L = d.L
-    zb_ptr = d.device_ptr("zb")
-    # zs, uu, vv window means of the last full-solver window
-    ref = {k: torch.zeros(ny * nx, dtype=torch.float32, device="cuda") for k in ("zsmean", "uumean", "vvmean")}
+    # zb and the zs, uu, vv window means of the last full-solver window
+    ref = {k: torch.zeros(ny * nx, dtype=torch.float32, device="cuda") for k in ("zb", "zsmean", "uumean", "vvmean")}
     t, dt = 0.0, 0.05
     cur_block = -1
     stats = {"full_windows": 0, "sur_windows": 0, "steps_full": 0, "steps_sur": 0, "checkpoints": [],
@@ -167,7 +167,7 @@ def run_surrogate(p, blocks, hydro_s: float, ckpt: dict, spin_s: float = 600.0, 
     def predict(Hm0, Tp, dgrid):
         """The surrogate's outputs for the coming window (in y) and the
         physical normalisation for this sea state."""
-        rc = L.xbg_sur_pack_inputs(zb_ptr, ref["zsmean"].data_ptr(), ref["uumean"].data_ptr(),
+        rc = L.xbg_sur_pack_inputs(ref["zb"].data_ptr(), ref["zsmean"].data_ptr(), ref["uumean"].data_ptr(),
                                    ref["vvmean"].data_ptr(), nx * ny, Hm0, Tp, dgrid,
                                    norm14.ctypes.data_as(C.POINTER(C.c_float)), x8.data_ptr(), d.stream)
         if rc != 0:
