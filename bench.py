@@ -276,9 +276,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="large", choices=["large", "small", "ensemble"])
     ap.add_argument("--domains", type=int, default=256, help="ensemble size (--workload ensemble)")
-    ap.add_argument("--path", default="graph", choices=["graph", "host"],
-                    help="graph: device-side time step and graph-replayed steps (the batched path with one "
-                         "domain); host: one xbg_advance_resident call per step (dt chosen on the host)")
+    ap.add_argument("--path", default="host", choices=["graph", "host"],
+                    help="host: one xbg_advance_resident call per step (dt chosen on the host); graph: "
+                         "device-side time step and graph-replayed steps (the batched path with one domain: "
+                         "no host round trip, but its kernels read the domain through a descriptor table and "
+                         "run slower on one large domain)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
