@@ -37,6 +37,14 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 def build_domain(p, seed=1234):
     from xbeach_gpu_amd import Domain, setups
 
+    if p.nx == 200 and p.ny == 50:
+        # the small configuration: planar 1:50 beach from -8 m, seeded JONSWAP
+        # wave groups Hm0 1.5 m, Tp 10 s, 0 deg, s = 10
+        elev = setups.small_bathy(p.nx, p.ny, p.dx)
+        st = setups.spun_up_state(p, elev, seed=seed, Hrms=1.5 / math.sqrt(2.0))
+        frc = setups.JonswapForcing(p, 1.5, 10.0, dir_deg=0.0, s=10.0, seed=0, offdepth=8.0, rtlength=600.0,
+                                    t0=300.0)
+        return st, frc
     elev = setups.barred_bathy(p.nx, p.ny, p.dx)
     st = setups.spun_up_state(p, elev, seed=seed, Hrms=3.0 / math.sqrt(2.0))
     # seeded JONSWAP wave groups (wavebndtype 4, seed 0): Hm0 3 m, Tp 12 s,

@@ -90,6 +90,15 @@ struct Fld {
     double* red_sum;
 };
 
+// A load / store through a global-address-space pointer.  Field pointers
+// that reach a kernel through device memory (the batched launches'
+// descriptor table) are generic: their accesses would be flat loads and
+// stores, which need a 64-bit address per lane and also count against the
+// LDS counter that the wave kernel's per-bin barrier waits on (so every bin
+// would wait for the prefetch of the next plane).
+XDEV float gload(const float* q) { return *(const __attribute__((address_space(1))) float*)q; }
+XDEV void gstore(float* q, float v) { *(__attribute__((address_space(1))) float*)q = v; }
+
 // Block (tile) coordinates of a kernel body: blockIdx in a single-domain
 // launch, the tile-map entry in a batched ensemble launch.
 struct BI {
@@ -442,8 +451,8 @@ XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const fl
         rxx = rxx + (dtab<L>(f.dirs, 6, th) * rv);   // (cs*cs) * rv
         ryy = ryy + (dtab<L>(f.dirs, 7, th) * rv);   // (sn*sn) * rv
         rxy = rxy + (dtab<L>(f.dirs, 5, th) * rv);   // sn*cs*rv
-        f.ee_out[i + (size_t)th * N] = ev;
-        f.rr_out[i + (size_t)th * N] = rv;
+        gstore(f.ee_out + i + (size_t)th * N, ev);
+        gstore(f.rr_out + i + (size_t)th * N, rv);
     }
     f.Sxx[i] = (nn * txx - 0.5f * sume) * p.dtheta + rxx * p.dtheta;
     f.Syy[i] = (nn * tyy - 0.5f * sume) * p.dtheta + ryy * p.dtheta;
@@ -524,6 +533,7 @@ XDEV Slots make_slots(const KP& p, int x0, int y0, int tid) {
     return sl;
 }
 
+
 // The raw values of one plane (+halo) for the thread's staging slots: the
 // global loads are issued here and consumed by store_plane one bin later,
 // so their latency overlaps the arithmetic of the current bin.
@@ -545,7 +555,7 @@ XDEV void load_plane(float* raw, const KP& p, const Fld& f, const Slots& sl, int
             src = code >= 0 ? plane + (uint32_t)code : f.St + ((uint32_t)(-code - 1) + (uint32_t)thv * p.ny);
         else        // rr: the offshore column carries its own value
             src = plane + (code >= 0 ? (uint32_t)code : (uint32_t)(-code - 1) * p.nx);
-        raw[it] = *src;
+        raw[it] = gload(src);
     }
 }
 // RQ: the divisions by sigm and dtheta go through their double reciprocals
