@@ -98,6 +98,11 @@ struct Fld {
 // would wait for the prefetch of the next plane).
 XDEV float gload(const float* q) { return *(const __attribute__((address_space(1))) float*)q; }
 XDEV void gstore(float* q, float v) { *(__attribute__((address_space(1))) float*)q = v; }
+// The same for any field: G(f.uu)[i] reads and writes through a global pointer.
+template <class T>
+XDEV __attribute__((address_space(1))) T* G(T* q) {
+    return (__attribute__((address_space(1))) T*)q;
+}
 
 // Block (tile) coordinates of a kernel body: blockIdx in a single-domain
 // launch, the tile-map entry in a batched ensemble launch.
@@ -145,7 +150,7 @@ XDEV void offshore_wg(const KP& p, const SP& s, const Fld& f, int waves, int tid
     float acc = 0.0f;
     for (int base = 0; base < ny; base += 1024) {
         int cnt = min(1024, ny - base);
-        for (int t = tid; t < cnt; t += nthr) col[t] = f.uu[(size_t)(base + t) * nx];
+        for (int t = tid; t < cnt; t += nthr) col[t] = G(f.uu)[(size_t)(base + t) * nx];
         __syncthreads();
         if (tid == 0) {
             // sequential row-order sum; LDS reads unrolled ahead of the adds
@@ -163,19 +168,19 @@ XDEV void offshore_wg(const KP& p, const SP& s, const Fld& f, int waves, int tid
         size_t i = (size_t)iy * nx;
         size_t ip = i + p1(0, nx);
         float zsbnd = s.zsbndi + (s.zsbndn - s.zsbndi) * ((float)iy / (float)ny);
-        float zsplus = f.zs[ip];
+        float zsplus = G(f.zs)[ip];
         float ui = 0.0f;
         if (waves) {
-            float qx = (f.qbo[iy] + s.tsl / s.tnext * (f.qbn[iy] - f.qbo[iy])) * taper;
-            float ht = zsbnd + f.zb[i];
+            float qx = (G(f.qbo)[iy] + s.tsl / s.tnext * (G(f.qbn)[iy] - G(f.qbo)[iy])) * taper;
+            float ht = zsbnd + G(f.zb)[i];
             ui = qx / ht;
         }
-        float ur = -1.0f * sqrtf(p.g / f.hh[i]) * (zsplus - zsbnd);
-        float um = factime * uuavg + f.umean[iy] * (1 - factime);
-        f.umean[iy] = um;
-        f.uu[i] = (2.0f) * ui + ur + um;
-        f.zs[i] = zsplus;
-        f.vv[i] = f.vv[ip];
+        float ur = -1.0f * sqrtf(p.g / G(f.hh)[i]) * (zsplus - zsbnd);
+        float um = factime * uuavg + G(f.umean)[iy] * (1 - factime);
+        G(f.umean)[iy] = um;
+        G(f.uu)[i] = (2.0f) * ui + ur + um;
+        G(f.zs)[i] = zsplus;
+        G(f.vv)[i] = G(f.vv)[ip];
     }
 }
 
@@ -212,10 +217,10 @@ XDEV void dispersion_body(const KP& p, const SP& s, const Fld& f, int offshore, 
     if (x >= p.nx || y >= p.ny) return;
     size_t i = (size_t)x + (size_t)y * p.nx;
     float k, c, cg;
-    dispersion_cell(p, s.sigm, f.hh[i], k, c, cg);
-    f.k[i] = k;
-    f.c[i] = c;
-    f.cg[i] = cg;
+    dispersion_cell(p, s.sigm, G(f.hh)[i], k, c, cg);
+    G(f.k)[i] = k;
+    G(f.c)[i] = c;
+    G(f.cg)[i] = cg;
 }
 __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offshore) {
     dispersion_body(p, s, f, offshore, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -223,12 +228,12 @@ __global__ __launch_bounds__(256) void k_dispersion(KP p, SP s, Fld f, int offsh
 
 // Wave action after offshorebndWav + sanity + action (wave_CPU.cpp:24-309).
 XDEV float act_ee(const KP& p, const SP& s, const Fld& f, int x, int y, int th) {
-    float v = (x == 0) ? f.St[y + th * p.ny] * s.taper
-                       : f.ee_in[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
+    float v = (x == 0) ? G(f.St)[y + th * p.ny] * s.taper
+                       : G(f.ee_in)[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
     return div_uniform(fmx(v, 0.000f), s.sigm, s.rsigm, s.sigm_recip);
 }
 XDEV float act_rr(const KP& p, const Fld& f, int x, int y, int th) {
-    return f.rr_in[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
+    return G(f.rr_in)[(size_t)x + (size_t)y * p.nx + (size_t)th * p.N];
 }
 
 // Refraction speed of propagthetaCPU (wave_CPU.cpp:259-287), with the
@@ -246,18 +251,18 @@ XDEV Refr refr_cell(const KP& p, const SP& s, const Fld& f, int x, int y) {
     float ddx = (float)(unsigned)(xp - xm) * p.dx;
     float ddy = (float)(unsigned)(yp - ym) * p.dx;
     Refr r;
-    r.dhdx = (f.hh[ir] - f.hh[il]) / ddx;
-    r.dhdy = (f.hh[it] - f.hh[ib]) / ddy;
+    r.dhdx = (G(f.hh)[ir] - G(f.hh)[il]) / ddx;
+    r.dhdy = (G(f.hh)[it] - G(f.hh)[ib]) / ddy;
     if (CUR) {
-        r.dudx = (f.uu[ir] - f.uu[il]) / ddx;
-        r.dvdx = (f.vv[ir] - f.vv[il]) / ddx;
-        r.dudy = (f.uu[it] - f.uu[ib]) / ddy;
-        r.dvdy = (f.vv[it] - f.vv[ib]) / ddy;
+        r.dudx = (G(f.uu)[ir] - G(f.uu)[il]) / ddx;
+        r.dvdx = (G(f.vv)[ir] - G(f.vv)[il]) / ddx;
+        r.dudy = (G(f.uu)[it] - G(f.uu)[ib]) / ddy;
+        r.dvdy = (G(f.vv)[it] - G(f.vv)[ib]) / ddy;
     } else {
         r.dudx = r.dvdx = r.dudy = r.dvdy = 0.0f;
     }
-    float h = f.hh[i];
-    float kh = fmn(f.k[i] * h, 10.0f);
+    float h = G(f.hh)[i];
+    float kh = fmn(G(f.k)[i] * h, 10.0f);
     r.sig = s.sigm / xm::sinhf(2.0f * kh);
     r.w = p.wci * fmn(h / p.hwci, 1.0f);
     r.lo = -0.25f * s.sigm;
@@ -302,13 +307,13 @@ XDEV AdvC advc_cell(const KP& p, const Fld& f, const float* S, int x, int y) {
         a.ui = a.uxm = a.uxp = a.vi = a.vym = a.vyp = 0.0f;
         return a;
     }
-    float w = p.wci * fmn(f.hh[x + row] / p.hwci, 1.0f);
-    a.ui = f.uu[x + row] * w;
-    a.uxm = f.uu[xm + row] * w;
-    a.uxp = f.uu[xp + row] * w;
-    a.vi = w * f.vv[x + row];
-    a.vym = w * f.vv[x + (size_t)ym * nx];
-    a.vyp = w * f.vv[x + (size_t)yp * nx];
+    float w = p.wci * fmn(G(f.hh)[x + row] / p.hwci, 1.0f);
+    a.ui = G(f.uu)[x + row] * w;
+    a.uxm = G(f.uu)[xm + row] * w;
+    a.uxp = G(f.uu)[xp + row] * w;
+    a.vi = w * G(f.vv)[x + row];
+    a.vym = w * G(f.vv)[x + (size_t)ym * nx];
+    a.vyp = w * G(f.vv)[x + (size_t)yp * nx];
     return a;
 }
 
@@ -366,7 +371,7 @@ template <int NT, bool CUR, bool L = false>
 XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e, float* r, float& Rout,
                       float& DRout) {
     const int nt = NT > 0 ? NT : p.nt;
-    float h = f.hh[i];
+    float h = G(f.hh)[i];
     float Et = 0.0f;
 #pragma unroll
     for (int th = 0; th < (NT > 0 ? NT : MAXT); th++) {
@@ -379,16 +384,16 @@ XDEV void dissip_cell(const KP& p, const SP& s, const Fld& f, size_t i, float* e
     float idiv = fmx(1.0f, xm::powf(Ht / (p.gammax * h), 2.0f));
     Ht = fmn(Ht, p.gammax * h);
     float E = (p.rho * p.g * Ht * Ht) / 8.0f;
-    float kk = f.k[i];
-    float fw = fwm_of(p, f.stdep[i]);
+    float kk = G(f.k)[i];
+    float fw = fwm_of(p, G(f.stdep)[i]);
     float Dv = p.breakmodel == 1 ? roelvink_cell(p, s.Trep, E, h, kk, fw) : baldock_cell(p, s.Trep, Ht, h, kk, fw);
-    f.Dw[i] = Dv;  // wave dissipation D (an output field of the reference, and a surrogate target)
-    float cc = f.c[i];
+    G(f.Dw)[i] = Dv;  // wave dissipation D (an output field of the reference, and a surrogate target)
+    float cc = G(f.c)[i];
     float uui = 0.0f, vvi = 0.0f;
     if (CUR) {
         float w = p.wci * fmn(h / p.hwci, 1.0f);
-        uui = f.uu[i] * w;
-        vvi = f.vv[i] * w;
+        uui = G(f.uu)[i] * w;
+        vvi = G(f.vv)[i] * w;
     }
     float RR = 0.0f, drr = 0.0f;
     bool wetc = h > p.eps;
@@ -436,7 +441,7 @@ XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const fl
                       float DRv) {
     const int nt = NT > 0 ? NT : p.nt;
     const size_t N = p.N;
-    float cgi = f.cg[i], ci = f.c[i];
+    float cgi = G(f.cg)[i], ci = G(f.c)[i];
     float nn = cgi / ci;
     float sume = 0.0f, sumethet = 0.0f, txx = 0.0f, tyy = 0.0f, txy = 0.0f, rxx = 0.0f, ryy = 0.0f, rxy = 0.0f;
 #pragma unroll
@@ -454,25 +459,25 @@ XDEV void finish_cell(const KP& p, const SP& s, const Fld& f, size_t i, const fl
         gstore(f.ee_out + i + (size_t)th * N, ev);
         gstore(f.rr_out + i + (size_t)th * N, rv);
     }
-    f.Sxx[i] = (nn * txx - 0.5f * sume) * p.dtheta + rxx * p.dtheta;
-    f.Syy[i] = (nn * tyy - 0.5f * sume) * p.dtheta + ryy * p.dtheta;
-    f.Sxy[i] = nn * txy * p.dtheta + rxy * p.dtheta;
+    G(f.Sxx)[i] = (nn * txx - 0.5f * sume) * p.dtheta + rxx * p.dtheta;
+    G(f.Syy)[i] = (nn * tyy - 0.5f * sume) * p.dtheta + ryy * p.dtheta;
+    G(f.Sxy)[i] = nn * txy * p.dtheta + rxy * p.dtheta;
     float sm = fmx(sume, 0.00001f);
-    f.thetamean[i] = (sumethet / (float)nt) / (sm / (float)nt);
+    G(f.thetamean)[i] = (sumethet / (float)nt) / (sm / (float)nt);
     float Em = sm * p.dtheta;
     float Hm = sqrtf(sm * p.dtheta / (p.rho * p.g / 8));
-    f.E[i] = Em;
-    f.H[i] = Hm;
-    f.R[i] = Rv;
-    f.DR[i] = DRv;
+    G(f.E)[i] = Em;
+    G(f.H)[i] = Hm;
+    G(f.R)[i] = Rv;
+    G(f.DR)[i] = DRv;
     float hmin = 0.2f;
-    float hhi = fmx(f.hh[i], hmin);
+    float hhi = fmx(G(f.hh)[i], hmin);
     float cib = fmx(ci, sqrtf(hmin * p.g));
-    float uorb = (float)(PI_ * (double)Hm / (double)s.Trep / (double)xm::sinhf(fmn(fmx(f.k[i], 0.01f) * hhi, 10.0f)));
-    f.urms[i] = uorb / (sqrtf(2.0f));
+    float uorb = (float)(PI_ * (double)Hm / (double)s.Trep / (double)xm::sinhf(fmn(fmx(G(f.k)[i], 0.01f) * hhi, 10.0f)));
+    G(f.urms)[i] = uorb / (sqrtf(2.0f));
     float ustw = Em / cib / p.rho / hhi;
     float ustr = 2.0f * Rv / cib / p.rho / hhi;
-    f.ust_raw[i] = ustw + ustr;
+    G(f.ust_raw)[i] = ustw + ustr;
 }
 
 // Wave tile: 64 cells along x (one wavefront per row) by WTY rows; each
@@ -761,7 +766,7 @@ XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     const bool valid = (x0 + cx < nx) && (y0 + cy < ny);
     const uint32_t i = (uint32_t)x + (uint32_t)y * nx;
     float ea[A];
-    for (int l = tid; l < 8 * MAXT; l += WTHREADS) s_dirs[l] = f.dirs[l];
+    for (int l = tid; l < 8 * MAXT; l += WTHREADS) s_dirs[l] = G(f.dirs)[l];
     const Refr rf = refr_cell<CUR>(p, s, f, x, y);
     const Slots sl = make_slots(p, x0, y0, tid);
     __syncthreads();
@@ -780,7 +785,7 @@ XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
 #pragma unroll
         for (int th = 0; th < A; th++) {
             if (th >= nt) break;
-            f.bnd_e1[((uint32_t)side * nt + th) * nx + x] = ea[th];
+            G(f.bnd_e1)[((uint32_t)side * nt + th) * nx + x] = ea[th];
         }
     }
     // the post-Euler ee bins wait in LDS while rr is advected (they would
@@ -798,14 +803,14 @@ XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
 #pragma unroll
             for (int th = 0; th < A; th++) {
                 if (th >= nt) break;
-                f.bnd_r1[((uint32_t)side * nt + th) * nx + x] = ra[th];
+                G(f.bnd_r1)[((uint32_t)side * nt + th) * nx + x] = ra[th];
             }
         }
     } else {
 #pragma unroll
         for (int th = 0; th < A; th++) {
             if (th >= nt) break;
-            ra[th] = f.rr_in[i + (uint32_t)th * p.N];
+            ra[th] = G(f.rr_in)[i + (uint32_t)th * p.N];
         }
     }
     // Rows 0 and ny-1 (rollerlatbnd, wave_CPU.cpp:830-871): the post-Euler
@@ -829,7 +834,7 @@ XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
             }
             lds_barrier();
         }
-        if (bsrc != 0) wetb = f.hh[i] > p.eps ? 1.0f : 0.0f;
+        if (bsrc != 0) wetb = G(f.hh)[i] > p.eps ? 1.0f : 0.0f;
     }
     if (NT > 0) {
 #pragma unroll
@@ -869,7 +874,7 @@ XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     if (tid == 0) {
         float m = wred[0];
         for (int w = 1; w < WTHREADS / 64; w++) m = fmn(m, wred[w]);
-        f.red_wave[tile] = m;
+        G(f.red_wave)[tile] = m;
     }
 }
 template <int NT, bool CUR, bool RQ>
@@ -892,14 +897,14 @@ XDEV void wave_rows_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     const size_t N = p.N;
     int y = side ? ny - 1 : 0, ys = side ? ny - 2 : 1;
     size_t i = (size_t)x + (size_t)y * nx, is = (size_t)x + (size_t)ys * nx;
-    float wet = f.hh[i] > p.eps ? 1.0f : 0.0f;
+    float wet = G(f.hh)[i] > p.eps ? 1.0f : 0.0f;
     float e[A], r[A], efin[A];
 #pragma unroll
     for (int th = 0; th < A; th++) {
         if (th >= nt) break;
-        e[th] = f.bnd_e1[((size_t)side * nt + th) * nx + x] * wet;
-        r[th] = p.roller == 1 ? f.bnd_r1[((size_t)side * nt + th) * nx + x] * wet : act_rr(p, f, x, y, th);
-        efin[th] = f.ee_out[is + (size_t)th * N] * wet;
+        e[th] = G(f.bnd_e1)[((size_t)side * nt + th) * nx + x] * wet;
+        r[th] = p.roller == 1 ? G(f.bnd_r1)[((size_t)side * nt + th) * nx + x] * wet : act_rr(p, f, x, y, th);
+        efin[th] = G(f.ee_out)[is + (size_t)th * N] * wet;
     }
     float Rv, DRv;
     dissip_cell<NT, CUR>(p, s, f, i, e, r, Rv, DRv);
@@ -915,11 +920,11 @@ XDEV void wavforce_at(const KP& p, const Fld& f, int x, int y, float& Fxv, float
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     auto I = [nx](int a, int b) { return (size_t)a + (size_t)b * nx; };
-    float Sxxi = f.Sxx[I(x, y)], Sxxr = f.Sxx[I(xp, y)];
-    float Sxyt = f.Sxy[I(x, yp)], Sxytr = f.Sxy[I(xp, yp)];
-    float Sxyb = f.Sxy[I(x, ym)], Sxybr = f.Sxy[I(xp, ym)];
-    float Syyi = f.Syy[I(x, y)], Syyt = f.Syy[I(x, yp)];
-    float Sxyr = f.Sxy[I(xp, y)], Sxyl = f.Sxy[I(xm, y)], Sxytl = f.Sxy[I(xm, yp)];
+    float Sxxi = G(f.Sxx)[I(x, y)], Sxxr = G(f.Sxx)[I(xp, y)];
+    float Sxyt = G(f.Sxy)[I(x, yp)], Sxytr = G(f.Sxy)[I(xp, yp)];
+    float Sxyb = G(f.Sxy)[I(x, ym)], Sxybr = G(f.Sxy)[I(xp, ym)];
+    float Syyi = G(f.Syy)[I(x, y)], Syyt = G(f.Syy)[I(x, yp)];
+    float Sxyr = G(f.Sxy)[I(xp, y)], Sxyl = G(f.Sxy)[I(xm, y)], Sxytl = G(f.Sxy)[I(xm, yp)];
     Fxv = -1 * (Sxxr - Sxxi) / p.dx - 0.5f * (Sxyt + Sxytr - Sxyb - Sxybr) / (2.0f * p.dx);
     Fyv = -1 * (Syyt - Syyi) / p.dx - 0.5f * (Sxyr + Sxytr - Sxyl - Sxytl) / (2.0f * p.dx);
 }
@@ -930,7 +935,7 @@ XDEV void wavforce_at(const KP& p, const Fld& f, int x, int y, float& Fxv, float
 XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     size_t i = (size_t)x + (size_t)y * nx;
-    float wet = f.hh[i] > p.eps ? 1.0f : 0.0f;
+    float wet = G(f.hh)[i] > p.eps ? 1.0f : 0.0f;
     int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
     float Fxv, Fyv;
     wavforce_at(p, f, x, ys, Fxv, Fyv);
@@ -940,17 +945,17 @@ XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
         float a, b;
         wavforce_at(p, f, p1(0, nx), y, a, b);
         fy = b * wet;
-        ust = f.ust_raw[(size_t)p1(0, nx) + (size_t)y * nx] * wet;
+        ust = G(f.ust_raw)[(size_t)p1(0, nx) + (size_t)y * nx] * wet;
     } else if (ys != y) {
         fy = Fyv * wet;
-        ust = f.ust_raw[(size_t)x + (size_t)ys * nx] * wet;
+        ust = G(f.ust_raw)[(size_t)x + (size_t)ys * nx] * wet;
     } else {
         fy = Fyv;
-        ust = f.ust_raw[i];
+        ust = G(f.ust_raw)[i];
     }
-    f.Fx[i] = fx;
-    f.Fy[i] = fy;
-    f.ust[i] = ust;
+    G(f.Fx)[i] = fx;
+    G(f.Fy)[i] = fy;
+    G(f.ust)[i] = ust;
 }
 __global__ __launch_bounds__(256) void k_wave_force(KP p, Fld f) {
     int x = gx(BI{(int)blockIdx.x, (int)blockIdx.y}), y = gy(BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -964,13 +969,13 @@ __global__ __launch_bounds__(256) void k_wave_force(KP p, Fld f) {
 XDEV size_t ix2(const KP& p, int x, int y) { return (size_t)x + (size_t)y * p.nx; }
 // udepthmomcont / vdepthmomcont (Flow_kernel.cu:546-651)
 XDEV float hum_at(const KP& p, const Fld& f, int x, int y) {
-    return fmx(0.5f * (f.hh[ix2(p, x, y)] + f.hh[ix2(p, p1(x, p.nx), y)]), p.eps);
+    return fmx(0.5f * (G(f.hh)[ix2(p, x, y)] + G(f.hh)[ix2(p, p1(x, p.nx), y)]), p.eps);
 }
 XDEV float hvm_at(const KP& p, const Fld& f, int x, int y) {
-    return fmx(0.5f * (f.hh[ix2(p, x, y)] + f.hh[ix2(p, x, p1(y, p.ny))]), p.eps);
+    return fmx(0.5f * (G(f.hh)[ix2(p, x, y)] + G(f.hh)[ix2(p, x, p1(y, p.ny))]), p.eps);
 }
-XDEV float wetu_at(const KP& p, const Fld& f, int x, int y) { return (float)(f.wet[ix2(p, x, y)] & 1); }
-XDEV float wetv_at(const KP& p, const Fld& f, int x, int y) { return (float)((f.wet[ix2(p, x, y)] >> 1) & 1); }
+XDEV float wetu_at(const KP& p, const Fld& f, int x, int y) { return (float)(G(f.wet)[ix2(p, x, y)] & 1); }
+XDEV float wetv_at(const KP& p, const Fld& f, int x, int y) { return (float)((G(f.wet)[ix2(p, x, y)] >> 1) & 1); }
 
 // smago (Flow_kernel.cu:1282-1317) and the wet masks of
 // udepthmomcont/vdepthmomcont from the previous step's hu/hv.  With force
@@ -983,25 +988,25 @@ XDEV void flow_nuh_body(const KP& p, const Fld& f, int force, BI bi) {
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     size_t i = ix2(p, x, y);
-    float h = f.hh[i];
-    float humi = 0.5f * (h + f.hh[ix2(p, xp, y)]);
-    float hvmi = 0.5f * (h + f.hh[ix2(p, x, yp)]);
+    float h = G(f.hh)[i];
+    float humi = 0.5f * (h + G(f.hh)[ix2(p, xp, y)]);
+    float hvmi = 0.5f * (h + G(f.hh)[ix2(p, x, yp)]);
     uint8_t m = 0;
-    if (f.hu[i] > p.eps && humi > p.eps) m |= 1;
-    if (f.hv[i] > p.eps && hvmi > p.eps) m |= 2;
-    f.wet[i] = m;
-    float turb = p.nuhfac * h * xm::powf(f.DR[i] / p.rho, 1.0f / 3.0f);
+    if (G(f.hu)[i] > p.eps && humi > p.eps) m |= 1;
+    if (G(f.hv)[i] > p.eps && hvmi > p.eps) m |= 2;
+    G(f.wet)[i] = m;
+    float turb = p.nuhfac * h * xm::powf(G(f.DR)[i] / p.rho, 1.0f / 3.0f);
     if (p.usesmago == 1) {
-        const float* uu = f.uu;
-        const float* vv = f.vv;
+        const auto uu = G(f.uu);
+        const auto vv = G(f.vv);
         float dudx = (uu[i] - uu[ix2(p, xm, y)]) / p.dx;
         float dudy = 0.50f * (uu[ix2(p, x, yp)] - uu[ix2(p, x, ym)] + uu[ix2(p, xm, yp)] - uu[ix2(p, xm, ym)]) / p.dx;
         float dvdy = (vv[i] - vv[ix2(p, x, ym)]) / p.dx;
         float dvdx = 0.50f * (vv[ix2(p, xp, y)] - vv[ix2(p, xm, y)] + vv[ix2(p, xp, ym)] - vv[ix2(p, xm, ym)]) / p.dx;
         float tau = sqrtf(2.0f * dudx * dudx + 2.0f * dvdy * dvdy + xm::powf(dudy + dvdx, 2.0f));
-        f.nuh[i] = fmx(p.nuh * p.nuh * p.dx * p.dx * tau, turb);
+        G(f.nuh)[i] = fmx(p.nuh * p.nuh * p.dx * p.dx * tau, turb);
     } else {
-        f.nuh[i] = fmx(p.nuh, turb);
+        G(f.nuh)[i] = fmx(p.nuh, turb);
     }
 }
 __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
@@ -1013,23 +1018,23 @@ __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
 XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     size_t i = ix2(p, x, y);
-    if (x == 0) return f.uu[i];
-    if (!(f.wet[i] & 1)) return 0.0f;
+    if (x == 0) return G(f.uu)[i];
+    if (!(G(f.wet)[i] & 1)) return 0.0f;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
-    const float* uu = f.uu;
-    const float* vv = f.vv;
+    const auto uu = G(f.uu);
+    const auto vv = G(f.vv);
     const float dx = p.dx;
     size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), br = ix2(p, xp, ym);
     float hum = hum_at(p, f, x, y);
     // ududx_adv
     float res = 0.0f;
-    float qin = 0.5f * (f.hu[i] * uu[i] + f.hu[l] * uu[l]);
+    float qin = 0.5f * (G(f.hu)[i] * uu[i] + G(f.hu)[l] * uu[l]);
     float du = uu[i] - uu[l];
     if (qin > 0.0f) {
         if (du > 0.5f) res = res + 0.5f * (uu[l] + uu[i]) * (uu[i] - uu[l]) / dx;
         else res = res + qin / hum * (uu[i] - uu[l]) / dx;
     }
-    qin = -0.5f * (f.hu[i] * uu[i] + f.hu[r] * uu[r]);
+    qin = -0.5f * (G(f.hu)[i] * uu[i] + G(f.hu)[r] * uu[r]);
     du = uu[r] - uu[i];
     if (qin > 0.0f) {
         if (du > 0.5f) res = res - 0.5f * (uu[r] + uu[i]) * (uu[i] - uu[r]) / dx;
@@ -1038,13 +1043,13 @@ XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     float ududx = res;
     // vdudy_adv
     res = 0.0f;
-    qin = 0.5f * (vv[b] * f.hv[b] + vv[br] * f.hv[br]);
+    qin = 0.5f * (vv[b] * G(f.hv)[b] + vv[br] * G(f.hv)[br]);
     float dv = vv[i] - vv[b];
     if (qin > 0.0f) {
         if (dv > 0.5f) res = res + 0.5f * (vv[b] + vv[br]) * (uu[i] - uu[b]) / dx;
         else res = res + qin / hum * (uu[i] - uu[b]) / dx;
     }
-    qin = -0.5f * (vv[i] * f.hv[i] + vv[r] * f.hv[r]);
+    qin = -0.5f * (vv[i] * G(f.hv)[i] + vv[r] * G(f.hv)[r]);
     dv = vv[t] - vv[i];
     if (qin > 0.0f) {
         if (dv > 0.5f) res = res - 0.5f * (vv[i] + vv[r]) * (uu[i] - uu[t]) / dx;
@@ -1052,9 +1057,9 @@ XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     }
     float vdudy = res;
     // viscou1 + viscou2
-    const float* nuh = f.nuh;
-    float dudx1 = nuh[r] * f.hh[r] * (uu[r] - uu[i]) / dx;
-    float dudx2 = nuh[i] * f.hh[i] * (uu[i] - uu[l]) / dx;
+    const auto nuh = G(f.nuh);
+    float dudx1 = nuh[r] * G(f.hh)[r] * (uu[r] - uu[i]) / dx;
+    float dudx2 = nuh[i] * G(f.hh)[i] * (uu[i] - uu[l]) / dx;
     float viscu = (1.0f / hum) * ((dudx1 - dudx2) / (dx));
     float nuh1 = 0.25f * (nuh[i] + nuh[r] + nuh[ix2(p, xp, yp)] + nuh[t]);
     float nuh2 = 0.25f * (nuh[i] + nuh[r] + nuh[br] + nuh[b]);
@@ -1062,13 +1067,13 @@ XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     float dudy2 = nuh2 * 0.5f * (hvm_at(p, f, x, ym) + hvm_at(p, f, xp, ym)) * (uu[i] - uu[b]) / dx;
     viscu = viscu + (1.0f / hum) * ((dudy1 - dudy2) / dx * wetu_at(p, f, x, yp) * wetu_at(p, f, x, ym));
     // wlevslopes
-    float dzsdx = (f.zs[r] - f.zs[i]) / dx;
-    float taubx = cfm_of(p, f.stdep[i]) * p.rho * f.ueu[i] *
-                  sqrtf(1.3456f * f.urms[i] * f.urms[i] + f.vmageu[i] * f.vmageu[i]);
+    float dzsdx = (G(f.zs)[r] - G(f.zs)[i]) / dx;
+    float taubx = cfm_of(p, G(f.stdep)[i]) * p.rho * G(f.ueu)[i] *
+                  sqrtf(1.3456f * G(f.urms)[i] * G(f.urms)[i] + G(f.vmageu)[i] * G(f.vmageu)[i]);
     float hmin = 0.2f;
-    return uu[i] - s.dt * (ududx + vdudy - viscu + p.g * dzsdx + taubx / (p.rho * f.hu[i]) -
-                           f.Fx[i] / (p.rho * fmx(hum, hmin)) -
-                           1.25f * p.Cd * (float)s.signwu * s.windu * s.windu / (p.rho * hum) - p.fc * f.vu[i]);
+    return uu[i] - s.dt * (ududx + vdudy - viscu + p.g * dzsdx + taubx / (p.rho * G(f.hu)[i]) -
+                           G(f.Fx)[i] / (p.rho * fmx(hum, hmin)) -
+                           1.25f * p.Cd * (float)s.signwu * s.windu * s.windu / (p.rho * hum) - p.fc * G(f.vu)[i]);
 }
 
 // eulerustep + latbnd(uu) (Flow_kernel.cu:1778-1809): rows 0 / ny-1 copy
@@ -1077,7 +1082,7 @@ XDEV void flow_u_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
-    f.uu_b[ix2(p, x, y)] = eulerustep_at(p, s, f, x, ys);
+    G(f.uu_b)[ix2(p, x, y)] = eulerustep_at(p, s, f, x, ys);
 }
 __global__ __launch_bounds__(256) void k_flow_u(KP p, SP s, Fld f) {
     flow_u_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1087,9 +1092,9 @@ __global__ __launch_bounds__(256) void k_flow_u(KP p, SP s, Fld f) {
 XDEV float viscov1_at(const KP& p, const Fld& f, int x, int y) {
     int ym = m1(y), yp = p1(y, p.ny);
     size_t i = ix2(p, x, y), t = ix2(p, x, yp), b = ix2(p, x, ym);
-    const float* vv = f.vv;
-    float dvdy1 = f.nuh[t] * f.hh[t] * (vv[t] - vv[i]) / p.dx;
-    float dvdy2 = f.nuh[i] * f.hh[i] * (vv[i] - vv[b]) / p.dx;
+    const auto vv = G(f.vv);
+    float dvdy1 = G(f.nuh)[t] * G(f.hh)[t] * (vv[t] - vv[i]) / p.dx;
+    float dvdy2 = G(f.nuh)[i] * G(f.hh)[i] * (vv[i] - vv[b]) / p.dx;
     return (1.0f / hvm_at(p, f, x, y)) * (dvdy1 - dvdy2) / p.dx * wetv_at(p, f, x, yp) * wetv_at(p, f, x, ym);
 }
 
@@ -1099,23 +1104,23 @@ XDEV float viscov1_at(const KP& p, const Fld& f, int x, int y) {
 XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     size_t i = ix2(p, x, y);
-    if (x == 0 || y >= ny - 2) return f.vv[i];
-    if (!((f.wet[i] >> 1) & 1)) return 0.0f;
+    if (x == 0 || y >= ny - 2) return G(f.vv)[i];
+    if (!((G(f.wet)[i] >> 1) & 1)) return 0.0f;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
-    const float* un = f.uu_b;
-    const float* vv = f.vv;
+    const auto un = G(f.uu_b);
+    const auto vv = G(f.vv);
     const float dx = p.dx;
     size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), tl = ix2(p, xm, yp);
     float hvm = hvm_at(p, f, x, y);
     // udvdx_adv
     float res = 0.0f;
-    float qin = (float)(0.5 * (double)(un[l] * f.hu[l] + un[tl] * f.hu[tl]));
+    float qin = (float)(0.5 * (double)(un[l] * G(f.hu)[l] + un[tl] * G(f.hu)[tl]));
     float du = un[i] - un[l];
     if (qin > 0.0f) {
         if (du > 0.5f) res = res + 0.5f * (un[l] + un[tl]) * (vv[i] - vv[l]) / (dx);
         else res = res + qin / hvm * (vv[i] - vv[l]) / (dx);
     }
-    qin = (float)(-0.5 * (double)(un[i] * f.hu[i] + un[t] * f.hu[t]));
+    qin = (float)(-0.5 * (double)(un[i] * G(f.hu)[i] + un[t] * G(f.hu)[t]));
     du = un[r] - un[i];
     if (qin > 0.0f) {
         if (du > 0.5f) res = res - 0.5f * (un[i] + un[r]) * (vv[i] - vv[r]) / (dx);
@@ -1125,13 +1130,13 @@ XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     // vdvdy_adv (interior: ix>0, 0<iy<ny-2)
     res = 0.0f;
     if (y > 0) {
-        qin = 0.5f * (vv[i] * f.hv[i] + vv[b] * f.hv[b]);
+        qin = 0.5f * (vv[i] * G(f.hv)[i] + vv[b] * G(f.hv)[b]);
         float dv = vv[i] - vv[b];
         if (qin > 0.0f) {
             if (dv > 0.5f) res = res + 0.5f * (vv[b] - vv[i]) * (vv[i] - vv[b]) / (dx);
             else res = res + qin / hvm * (vv[i] - vv[b]) / (dx);
         }
-        qin = -0.5f * (f.hv[i] * vv[i] + f.hv[t] * vv[t]);
+        qin = -0.5f * (G(f.hv)[i] * vv[i] + G(f.hv)[t] * vv[t]);
         dv = vv[t] - vv[i];
         if (qin > 0.0f) {
             if (dv > 0.5f) res = res - 0.5f * (vv[t] + vv[i]) * (vv[i] - vv[t]) / (dx);
@@ -1141,18 +1146,18 @@ XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     float vdvdy = res;
     // viscov1 (row 0 takes row 1 through viscovbnd) + viscov2
     float viscv = viscov1_at(p, f, x, y == 0 ? p1(0, ny) : y);
-    const float* nuh = f.nuh;
+    const auto nuh = G(f.nuh);
     float nuh1 = 0.25f * (nuh[i] + nuh[r] + nuh[ix2(p, xp, yp)] + nuh[t]);
     float nuh2 = 0.25f * (nuh[i] + nuh[l] + nuh[tl] + nuh[t]);
     float dvdx1 = nuh1 * 0.5f * (hum_at(p, f, x, y) + hum_at(p, f, x, yp)) * (vv[r] - vv[i]) / dx;
     float dvdx2 = nuh2 * 0.5f * (hum_at(p, f, xm, y) + hum_at(p, f, xm, yp)) * (vv[i] - vv[l]) / dx;
     viscv = viscv + (1.0f / hvm) * ((dvdx1 - dvdx2) / (dx) * wetv_at(p, f, xp, y) * wetv_at(p, f, xm, y));
-    float dzsdy = (f.zs[t] - f.zs[i]) / dx;
-    float tauby = cfm_of(p, f.stdep[i]) * p.rho * f.vev[i] *
-                  sqrtf(1.3456f * f.urms[i] * f.urms[i] + f.vmagev[i] * f.vmagev[i]);
+    float dzsdy = (G(f.zs)[t] - G(f.zs)[i]) / dx;
+    float tauby = cfm_of(p, G(f.stdep)[i]) * p.rho * G(f.vev)[i] *
+                  sqrtf(1.3456f * G(f.urms)[i] * G(f.urms)[i] + G(f.vmagev)[i] * G(f.vmagev)[i]);
     float hmin = 0.2f;
-    return vv[i] - s.dt * (udvdx + vdvdy - viscv + p.g * dzsdy + tauby / (p.rho * f.hv[i]) -
-                           f.Fy[i] / (p.rho * fmx(hvm, hmin)) + p.fc * f.uv[i] -
+    return vv[i] - s.dt * (udvdx + vdvdy - viscv + p.g * dzsdy + tauby / (p.rho * G(f.hv)[i]) -
+                           G(f.Fy)[i] / (p.rho * fmx(hvm, hmin)) + p.fc * G(f.uv)[i] -
                            1.25f * p.Cd * (float)s.signwv * s.windv * s.windv / (p.rho * hvm));
 }
 
@@ -1162,7 +1167,7 @@ XDEV void flow_v_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     int ys = (y == 0) ? 1 : (y == p.ny - 2 ? p.ny - 3 : y);
-    f.vv_b[ix2(p, x, y)] = eulervstep_at(p, s, f, x, ys);
+    G(f.vv_b)[ix2(p, x, y)] = eulervstep_at(p, s, f, x, ys);
 }
 __global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
     flow_v_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1286,8 +1291,8 @@ XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         v = v - dts / hvm_at(p, f, x, y) *
                     ((qe * K(3, x, y) - qw * K(3, xm, y)) / dx + (qs * K(2, x, yp) - qn * K(2, x, y)) / dx);
     }
-    f.uu[i] = u;
-    f.vv[i] = v;
+    G(f.uu)[i] = u;
+    G(f.vv)[i] = v;
     const float ummn = 0.0f;
     float zsi = D(T_ZS, x, y), zbi = D(T_ZB, x, y);
     float zsr = D(T_ZS, x + 1, y), zbr = D(T_ZB, x + 1, y);
@@ -1297,12 +1302,12 @@ XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     if (u > ummn) h = zsi - hb;
     else if (u < -1.0f * ummn) h = zsr - hb;
     else h = fmx(fmx(zsi, zsr) - hb, p.eps);
-    f.hu[i] = fmx(h, 0.0f);
+    G(f.hu)[i] = fmx(h, 0.0f);
     hb = fmx(-1.0f * zbi, -1.0f * zbt);
     if (v > ummn) h = zsi - hb;
     else if (v < -1 * ummn) h = zst - hb;
     else h = fmx(fmx(zsi, zst) - hb, p.eps);
-    f.hv[i] = fmx(h, 0.0f);
+    G(f.hv)[i] = fmx(h, 0.0f);
 }
 __global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
     flow_uvcorr_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1312,11 +1317,11 @@ __global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
 // the water level after continuity; *dzdt_out gets dzsdt.
 XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, float* dzdt_out) {
     size_t i = ix2(p, x, y);
-    float z = f.zs[i];
+    float z = G(f.zs)[i];
     if (x > 0) {
         size_t l = ix2(p, x - 1, y), b = ix2(p, x, m1(y));
-        float qxi = f.uu[i] * f.hu[i], qxl = f.uu[l] * f.hu[l];
-        float qyi = f.vv[i] * f.hv[i], qyb = f.vv[b] * f.hv[b];
+        float qxi = G(f.uu)[i] * G(f.hu)[i], qxl = G(f.uu)[l] * G(f.hu)[l];
+        float qyi = G(f.vv)[i] * G(f.hv)[i], qyb = G(f.vv)[b] * G(f.hv)[b];
         float dzdt = (qxl - qxi + qyb - qyi) / p.dx;
         *dzdt_out = dzdt;
         z = z + dzdt * s.dt;
@@ -1350,7 +1355,7 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         float z = continuity_at(p, s, f, gxx, gyy, &dzdt);
         zt[r][c] = z;
         if (ux == gxx && uy == gyy && ux >= x0 && ux < x0 + BX && uy >= y0 && uy < y0 + CY && gxx > 0)
-            f.dzsdt[ix2(p, gxx, gyy)] = dzdt;
+            G(f.dzsdt)[ix2(p, gxx, gyy)] = dzdt;
     }
     __syncthreads();
     const float two_eps = p.eps * 2.0f;
@@ -1364,9 +1369,10 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
             int xm = m1(wx), xp = p1(wx, nx), xp2 = p2(wx, nx);
             int ym = m1(wy), yp = p1(wy, ny), yp2 = p2(wy, ny);
             auto I = [&](int a, int b) { return ix2(p, a, b); };
-            const float *zo = f.zsold, *zb = f.zb;
+            const auto zo = G(f.zsold);
+            const auto zb = G(f.zb);
             size_t i = I(wx, wy);
-            float u = f.uu[i], v = f.vv[i];
+            float u = G(f.uu)[i], v = G(f.vv)[i];
             float minzs = fmn(fmn(fmn(Z(-1, 0), Z(0, 0)), Z(1, 0)), Z(2, 0));
             float minzb = fmn(fmn(fmn(zb[I(xm, wy)], zb[i]), zb[I(xp, wy)]), zb[I(xp2, wy)]);
             if (wx > 0 && minzs + minzb > two_eps) {
@@ -1394,16 +1400,16 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         int rf = ys - (y0 - 2);
         z = z - s.dt * ((w1t[rf][cf] - w1t[rf][cf - 1]) / p.dx + (w2t[rf][cf] - w2t[rf - 1][cf]) / p.dx);
     }
-    float qxv = f.uu[i] * f.hu[i], qyv = f.vv[i] * f.hv[i];
+    float qxv = G(f.uu)[i] * G(f.hu)[i], qyv = G(f.vv)[i] * G(f.hv)[i];
     if (x > 0 && y > 0) {
         int rf = y - (y0 - 2);
         qxv = qxv + w1t[rf][cf];
         qyv = qyv + w2t[rf][cf];
     }
-    f.zs_out[i] = z;
-    f.qx[i] = qxv;
-    f.qy[i] = qyv;
-    f.hh[i] = fmx(f.zb[i] + z, p.eps);
+    G(f.zs_out)[i] = z;
+    G(f.qx)[i] = qxv;
+    G(f.qy)[i] = qyv;
+    G(f.hh)[i] = fmx(G(f.zb)[i] + z, p.eps);
 }
 __global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
     flow_zsall_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1414,13 +1420,14 @@ XDEV void calcuvvu_at(const KP& p, const Fld& f, int x, int y, float* o) {
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     size_t i = ix2(p, x, y), r = ix2(p, xp, y), t = ix2(p, x, yp);
-    const float *uu = f.uu, *vv = f.vv;
+    const auto uu = G(f.uu);
+    const auto vv = G(f.vv);
     float wu = wetu_at(p, f, x, y), wv = wetv_at(p, f, x, y);
     float vu = 0.25f * (vv[ix2(p, x, ym)] + vv[i] + vv[ix2(p, xp, ym)] + vv[r]) * wu;
     float uv = 0.25f * (uu[ix2(p, xm, y)] + uu[i] + uu[ix2(p, xm, yp)] + uu[t]) * wv;
-    float tmi = f.thetamean[i], tmr = f.thetamean[r], tmt = f.thetamean[t];
+    float tmi = G(f.thetamean)[i], tmr = G(f.thetamean)[r], tmt = G(f.thetamean)[t];
     float si = xm::sinf(tmi), ci = xm::cosf(tmi), sr = xm::sinf(tmr), cr = xm::cosf(tmr), st = xm::sinf(tmt), ctt = xm::cosf(tmt);
-    float usi = f.ust[i], usr = f.ust[r], ust_t = f.ust[t];
+    float usi = G(f.ust)[i], usr = G(f.ust)[r], ust_t = G(f.ust)[t];
     float vsu = 0.5f * (usi * si + usr * sr) * wu;
     float usu = 0.5f * (usi * ci + usr * cr) * wu;
     float ue = uu[i] - usu, ve = vu - vsu;
@@ -1448,16 +1455,16 @@ XDEV void flow_uvvu_body(const KP& p, const SP& s, const Fld& f, int want_ua, BI
     int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
     float o[6];
     calcuvvu_at(p, f, x, ys, o);
-    f.uuold[i] = f.uu[i];
-    f.vvold[i] = f.vv[i];
-    f.zsold[i] = f.zs[i];
-    f.uv[i] = o[0];
-    f.vu[i] = o[1];
-    f.ueu[i] = o[2];
-    f.vev[i] = o[3];
-    f.vmageu[i] = o[4];
-    f.vmagev[i] = o[5];
-    if (want_ua) f.ua[i] = rvr_at(p, s, f, i);
+    G(f.uuold)[i] = G(f.uu)[i];
+    G(f.vvold)[i] = G(f.vv)[i];
+    G(f.zsold)[i] = G(f.zs)[i];
+    G(f.uv)[i] = o[0];
+    G(f.vu)[i] = o[1];
+    G(f.ueu)[i] = o[2];
+    G(f.vev)[i] = o[3];
+    G(f.vmageu)[i] = o[4];
+    G(f.vmagev)[i] = o[5];
+    if (want_ua) G(f.ua)[i] = rvr_at(p, s, f, i);
 }
 __global__ __launch_bounds__(256) void k_flow_uvvu(KP p, SP s, Fld f, int want_ua) {
     flow_uvvu_body(p, s, f, want_ua, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1485,8 +1492,8 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
     const float m1c = 0.0f, m2c = 0.7939f, m5c = 0.6373f, m6c = 0.5995f;
     // alph = -log10(e)/m4, bet = exp(m3/m4) with m3 = -0.6065, m4 = 0.3539
     const float alph = p.rvr_alph, bet = p.rvr_bet;
-    float kk = (float)(2 * PI_ / (double)(f.c[i] * s.Trep));
-    float Ur = 3.0f / 8.0f * sqrtf(2.0f) * f.H[i] * kk / xm::powf(kk * f.hh[i], 3.0f);
+    float kk = (float)(2 * PI_ / (double)(G(f.c)[i] * s.Trep));
+    float Ur = 3.0f / 8.0f * sqrtf(2.0f) * G(f.H)[i] * kk / xm::powf(kk * G(f.hh)[i], 3.0f);
     Ur = fmx(Ur, 0.00001f);
     if (Ur == Ur) {  // Ur >= 1e-5 (or +inf): both powers from one log
         const double lu = xm::log((double)Ur);
@@ -1494,13 +1501,13 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
         float B1 = (float)((double)(-90.0f + 90.0f * xm::tanhf(m5c / powf_pos(Ur, m6c, lu))) * PI_ / 180.0);
         float Sk = Bm * xm::cosf(B1);
         float As = Bm * xm::sinf(B1);
-        return (p.facsk * Sk - p.facas * As) * f.urms[i];
+        return (p.facsk * Sk - p.facas * As) * G(f.urms)[i];
     }
     float Bm = m1c + (m2c - m1c) / (1.0f + bet * xm::powf(Ur, alph));
     float B1 = (float)((double)(-90.0f + 90.0f * xm::tanhf(m5c / xm::powf(Ur, m6c))) * PI_ / 180.0);
     float Sk = Bm * xm::cosf(B1);
     float As = Bm * xm::sinf(B1);
-    return (p.facsk * Sk - p.facas * As) * f.urms[i];
+    return (p.facsk * Sk - p.facas * As) * G(f.urms)[i];
 }
 
 // longturb (Sediment_kernel.cu:27-165: kturb is set to 0, rolthick is the
@@ -1535,32 +1542,32 @@ XDEV void sed_flux_body(const KP& p, const SP& s, const Fld& f, int ua_ready, BI
     int x = x0 + cx, y = y0 + cy;
     if (x >= nx || y >= ny) return;
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, nx), y), t = ix2(p, x, p1(y, ny));
-    float rol = f.rolthick[i] + s.dt * (fabsf(f.dzsdt[i]) - p.beta * f.c[i]);
-    f.rolthick[i] = fmx(rol, 0.0f);
+    float rol = G(f.rolthick)[i] + s.dt * (fabsf(G(f.dzsdt)[i]) - p.beta * G(f.c)[i]);
+    G(f.rolthick)[i] = fmx(rol, 0.0f);
     float uai, uar, uat;
     if (ua_ready) {
-        uai = f.ua[i];
-        uar = f.ua[r];
-        uat = f.ua[t];
+        uai = G(f.ua)[i];
+        uar = G(f.ua)[r];
+        uat = G(f.ua)[t];
     } else {
         uai = uat_[cy][cx];
         uar = uat_[cy][cx + 1];
         uat = uat_[cy + 1][cx];
     }
-    float cci = f.C[i], ccr = f.C[r], cct = f.C[t];
-    float tm = f.thetamean[i];
+    float cci = G(f.C)[i], ccr = G(f.C)[r], cct = G(f.C)[t];
+    float tm = G(f.thetamean)[i];
     float ctm = xm::cosf(tm), stm = xm::sinf(tm);
     float uau = (float)(0.5 * (double)ctm * (double)(uai + uar));
     float uav = (float)(0.5 * (double)stm * (double)(uar + uai));
-    float urep = f.ueu[i] + uau;
-    float vmagu = sqrtf(xm::powf((f.uu[i] + uau), 2.0f) + xm::powf((f.vu[i] + uav), 2.0f));
+    float urep = G(f.ueu)[i] + uau;
+    float vmagu = sqrtf(xm::powf((G(f.uu)[i] + uau), 2.0f) + xm::powf((G(f.vu)[i] + uav), 2.0f));
     uau = (float)(0.5 * (double)ctm * (double)(uai + uat));
     uav = (float)(0.5 * (double)stm * (double)(uat + uai));
-    float vrep = f.vev[i] + uav;
-    float vmagv = sqrtf(xm::powf(f.uv[i] + uau, 2.0f) + xm::powf(f.vv[i] + uav, 2.0f));
-    float dzbdx = -1.0f * (f.zb[r] - f.zb[i]) / p.dx;
-    float dzbdy = -1.0f * (f.zb[t] - f.zb[i]) / p.dx;
-    float hu = f.hu[i], hv = f.hv[i];
+    float vrep = G(f.vev)[i] + uav;
+    float vmagv = sqrtf(xm::powf(G(f.uv)[i] + uau, 2.0f) + xm::powf(G(f.vv)[i] + uav, 2.0f));
+    float dzbdx = -1.0f * (G(f.zb)[r] - G(f.zb)[i]) / p.dx;
+    float dzbdy = -1.0f * (G(f.zb)[t] - G(f.zb)[i]) / p.dx;
+    float hu = G(f.hu)[i], hv = G(f.hv)[i];
     float wu = hu > p.eps ? 1.0f : 0.0f, wv = hv > p.eps ? 1.0f : 0.0f;
     float cu, cv;
     if (urep > 0.0f) cu = cci;
@@ -1571,10 +1578,10 @@ XDEV void sed_flux_body(const KP& p, const SP& s, const Fld& f, int ua_ready, BI
     else if (vrep < 0.0f) cv = cct;
     else cv = 0.50f * (cci + cct);
     float dcsdy = (cct - cci) / p.dx;
-    float Dc = p.nuh + p.nuhfac * f.hh[i] * xm::powf(f.DR[i] / p.rho, 1.0f / 3.0f);
+    float Dc = p.nuh + p.nuhfac * G(f.hh)[i] * xm::powf(G(f.DR)[i] / p.rho, 1.0f / 3.0f);
     const float facsl = 1.6f;
-    f.Sus[i] = p.sus * (cu * urep * hu - Dc * hu * dcsdx - facsl * cu * vmagu * hu * dzbdx) * wu;
-    f.Svs[i] = p.sus * (cv * vrep * hv - Dc * hv * dcsdy - facsl * cv * vmagv * hv * dzbdy) * wv;
+    G(f.Sus)[i] = p.sus * (cu * urep * hu - Dc * hu * dcsdx - facsl * cu * vmagu * hu * dzbdx) * wu;
+    G(f.Svs)[i] = p.sus * (cv * vrep * hv - Dc * hv * dcsdy - facsl * cv * vmagv * hv * dzbdy) * wv;
 }
 __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f, int ua_ready) {
     sed_flux_body(p, s, f, ua_ready, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1586,17 +1593,17 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     const int nx = p.nx, ny = p.ny;
     size_t i = ix2(p, x, y);
     const float zo = 0.006f, sedcal = 1.0f, bulk = 1.0f;
-    float h = f.hh[i];
+    float h = G(f.hh)[i];
     float hhi = fmx(h, 0.01f);
     const float drho = p.sed_drho;
-    float ue = (float)(0.5 * (double)(f.ueu[ix2(p, m1(x), y)] + f.ueu[i]));
-    float ve = (float)(0.5 * (double)(f.vev[ix2(p, x, m1(y))] + f.vev[i]));
+    float ue = (float)(0.5 * (double)(G(f.ueu)[ix2(p, m1(x), y)] + G(f.ueu)[i]));
+    float ve = (float)(0.5 * (double)(G(f.vev)[ix2(p, x, m1(y))] + G(f.vev)[i]));
     float vmags = ue * ue + ve * ve;
-    float ML = fmx(fmn(sqrtf(2.0f * f.R[i] * s.Trep / (p.rho * f.c[i])), hhi), 0.01f);
+    float ML = fmx(fmn(sqrtf(2.0f * G(f.R)[i] * s.Trep / (p.rho * G(f.c)[i])), hhi), 0.01f);
     float dcfin = xm::expf(fmn(hhi, 100.0f) / ML);
     float dcf = fmn(1 / (dcfin - 1), 1.0f);
-    float kb = p.nuhfac * xm::powf(f.DR[i] / p.rho, 0.66666666667f) * dcf;
-    float Urms2 = f.urms[i] * f.urms[i] + 1.45f * (kb + 0.0f);
+    float kb = p.nuhfac * xm::powf(G(f.DR)[i] / p.rho, 0.66666666667f) * dcf;
+    float Urms2 = G(f.urms)[i] * G(f.urms)[i] + 1.45f * (kb + 0.0f);
     float Tsg = fmx(0.1f * hhi / p.wws, 0.5f);
     float Ucr = p.sed_ucrfac * xm::log10f(4.0f * hhi / p.D90);
     float hdrag = fmx(hhi, 10.0f * zo);
@@ -1604,7 +1611,7 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     Cdd = Cdd * Cdd;
     float Asb = 0.005f * hhi * xm::powf(1 / hhi / (drho * p.g), 1.2f);
     const float Ass = p.sed_Ass;
-    float cfm = cfm_of(p, f.stdep[i]);
+    float cfm = cfm_of(p, G(f.stdep)[i]);
     float T1 = vmags + 0.018f / Cdd * Urms2;
     T1 = fmn(T1, 100000.0f * p.g / cfm * p.D50 * drho);
     T1 = sqrtf(T1);
@@ -1619,13 +1626,13 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     ceqs_out = ceqsg;
     // Erosus
     float exp_ero = p.morfac * s.dt / (1.0f - p.por) * h * (ceqsg * 1.0f / Tsg + ceqbg * 1.0f / s.dt);
-    float facero = fmn(1.0f, f.stdep[i] * 1.0f / fmx(0.000001f, exp_ero));
+    float facero = fmn(1.0f, G(f.stdep)[i] * 1.0f / fmx(0.000001f, exp_ero));
     // Conc
     float wz = h > p.eps ? 1.0f : 0.0f;
     float ero = facero * h * ceqsg * 1.0f / Tsg;
-    float dsusdx = (f.Sus[i] - f.Sus[ix2(p, m1(x), y)]) / p.dx;
-    float dsvsdy = (f.Svs[i] - f.Svs[ix2(p, x, m1(y))]) / p.dx;
-    float cs = (s.dt * Tsg) / (s.dt + Tsg) * (h * f.C[i] / s.dt - (dsusdx + dsvsdy - ero)) * wz;
+    float dsusdx = (G(f.Sus)[i] - G(f.Sus)[ix2(p, m1(x), y)]) / p.dx;
+    float dsvsdy = (G(f.Svs)[i] - G(f.Svs)[ix2(p, x, m1(y))]) / p.dx;
+    float cs = (s.dt * Tsg) / (s.dt + Tsg) * (h * G(f.C)[i] / s.dt - (dsusdx + dsvsdy - ero)) * wz;
     cs = fmx(cs, 0.0f);
     cs = fmn(cs, 0.1f * h);
     float depo = cs / Tsg;
@@ -1665,7 +1672,7 @@ XDEV void sed_conc_body(const KP& p, const SP& s, const Fld& f, int morph, BI bi
     size_t i = ix2(p, x, y);
     float Cn, dzg, ceq;
     conc_at(p, s, f, x, y, Cn, dzg, ceq);
-    f.ceqsg[i] = ceq;
+    G(f.ceqsg)[i] = ceq;
     float Cfin;
     int cy = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
     if (x == 0) {
@@ -1676,7 +1683,7 @@ XDEV void sed_conc_body(const KP& p, const SP& s, const Fld& f, int morph, BI bi
     } else {
         Cfin = Cn;
     }
-    f.C_out[i] = Cfin;
+    G(f.C_out)[i] = Cfin;
     if (!morph) return;
     int sx, sy;
     zsrc(p, x, y, sx, sy);
@@ -1686,21 +1693,21 @@ XDEV void sed_conc_body(const KP& p, const SP& s, const Fld& f, int morph, BI bi
         float c2, c3;
         conc_at(p, s, f, sx, sy, c2, dzs, c3);
     }
-    f.hh_b[i] = f.hh[i] + dzg;
-    f.dzb[i] = -1 * dzg;
-    f.stdep_b[i] = f.stdep[is] - dzs;
+    G(f.hh_b)[i] = G(f.hh)[i] + dzg;
+    G(f.dzb)[i] = -1 * dzg;
+    G(f.stdep_b)[i] = G(f.stdep)[is] - dzs;
     if (p.zb_accum == 2) {
-        double z = f.zb64[is] + (double)dzs;
-        f.zb64_b[i] = z;
-        f.zb_b[i] = (float)z;
+        double z = G(f.zb64)[is] + (double)dzs;
+        G(f.zb64_b)[i] = z;
+        G(f.zb_b)[i] = (float)z;
     } else if (p.zb_accum == 1) {
-        float zb0 = f.zb[is], c0 = f.zbc[is];
+        float zb0 = G(f.zb)[is], c0 = G(f.zbc)[is];
         float yk = dzs - c0;
         float tk = zb0 + yk;
-        f.zbc_b[i] = (tk - zb0) - yk;
-        f.zb_b[i] = tk;
+        G(f.zbc_b)[i] = (tk - zb0) - yk;
+        G(f.zb_b)[i] = tk;
     } else {
-        f.zb_b[i] = f.zb[is] + dzs;
+        G(f.zb_b)[i] = G(f.zb)[is] + dzs;
     }
 }
 __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) {
@@ -1711,8 +1718,9 @@ __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) 
 // exchange with the +x and +y neighbours.
 XDEV void aval_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& bxo, float& byo) {
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
-    const float *zb = f.zb_b, *sd = f.stdep_b;
-    float dzmax = f.hh_b[i] > p.eps ? p.wetdzmax : p.drydzmax;
+    const auto zb = G(f.zb_b);
+    const auto sd = G(f.stdep_b);
+    float dzmax = G(f.hh_b)[i] > p.eps ? p.wetdzmax : p.drydzmax;
     float maxchg = p.maxslpchg, dt = s.dt, dx = p.dx;
     float bx = 0.0f, by = 0.0f;
     float sxv = (zb[r] - zb[i]) / dx;
@@ -1781,22 +1789,22 @@ XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     zsrc(p, x, y, sx, sy);
     size_t is = ix2(p, sx, sy);
     float ds = (is == i) ? d : ddzb(sx, sy);
-    f.dzb[i] = f.dzb[i] - d;
-    f.zs[i] = f.zs[i] - d;
-    f.hh[i] = f.hh_b[i] - d;
-    f.stdep[i] = f.stdep_b[is] - ds;
+    G(f.dzb)[i] = G(f.dzb)[i] - d;
+    G(f.zs)[i] = G(f.zs)[i] - d;
+    G(f.hh)[i] = G(f.hh_b)[i] - d;
+    G(f.stdep)[i] = G(f.stdep_b)[is] - ds;
     if (p.zb_accum == 2) {
-        double z = f.zb64_b[is] + (double)ds;
-        f.zb64[i] = z;
-        f.zb[i] = (float)z;
+        double z = G(f.zb64_b)[is] + (double)ds;
+        G(f.zb64)[i] = z;
+        G(f.zb)[i] = (float)z;
     } else if (p.zb_accum == 1) {
-        float zb0 = f.zb_b[is], c0 = f.zbc_b[is];
+        float zb0 = G(f.zb_b)[is], c0 = G(f.zbc_b)[is];
         float yk = ds - c0;
         float tk = zb0 + yk;
-        f.zbc[i] = (tk - zb0) - yk;
-        f.zb[i] = tk;
+        G(f.zbc)[i] = (tk - zb0) - yk;
+        G(f.zb)[i] = tk;
     } else {
-        f.zb[i] = f.zb_b[is] + ds;
+        G(f.zb)[i] = G(f.zb_b)[is] + ds;
     }
 }
 __global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
@@ -1812,20 +1820,21 @@ __global__ __launch_bounds__(256) void k_init_depth(KP p, Fld f) {
     int x = gx(BI{(int)blockIdx.x, (int)blockIdx.y}), y = gy(BI{(int)blockIdx.x, (int)blockIdx.y});
     if (x >= p.nx || y >= p.ny) return;
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
-    const float *zs = f.zs, *zb = f.zb;
-    float u = f.uu[i], v = f.vv[i];
+    const auto zs = G(f.zs);
+    const auto zb = G(f.zb);
+    float u = G(f.uu)[i], v = G(f.vv)[i];
     float hb = fmx(-1.0f * zb[i], -1.0f * zb[r]);
     float h;
     if (u > 0.0f) h = zs[i] - hb;
     else if (u < -0.0f) h = zs[r] - hb;
     else h = fmx(fmx(zs[i], zs[r]) - hb, p.eps);
-    f.hu[i] = fmx(h, 0.0f);
+    G(f.hu)[i] = fmx(h, 0.0f);
     hb = fmx(-1.0f * zb[i], -1.0f * zb[t]);
     if (v > 0.0f) h = zs[i] - hb;
     else if (v < -0.0f) h = zs[t] - hb;
     else h = fmx(fmx(zs[i], zs[t]) - hb, p.eps);
-    f.hv[i] = fmx(h, 0.0f);
-    if (p.zb_accum == 2) f.zb64[i] = (double)zb[i];
+    G(f.hv)[i] = fmx(h, 0.0f);
+    if (p.zb_accum == 2) G(f.zb64)[i] = (double)zb[i];
 }
 
 // FLOWDT (Wave_kernel.cu:161-170) and the WAVEDT block partials reduced to
@@ -1852,7 +1861,7 @@ __global__ __launch_bounds__(256) void k_cfl(KP p, Fld f, float cfl, int nwave, 
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             size_t i = b + (size_t)k * stride;
-            h[k] = i < p.N ? f.hh[i] : 0.0f;
+            h[k] = i < p.N ? G(f.hh)[i] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -1862,7 +1871,7 @@ __global__ __launch_bounds__(256) void k_cfl(KP p, Fld f, float cfl, int nwave, 
             else v = fmn(v, d);
         }
     }
-    for (size_t j = i0; j < (size_t)nwave; j += stride) w = fmn(w, f.red_wave[j]);
+    for (size_t j = i0; j < (size_t)nwave; j += stride) w = fmn(w, G(f.red_wave)[j]);
     for (int o = 32; o > 0; o >>= 1) {
         v = fmn(v, __shfl_xor(v, o, 64));
         w = fmn(w, __shfl_xor(w, o, 64));
@@ -1887,13 +1896,13 @@ __global__ __launch_bounds__(256) void k_means(KP p, Fld f, float* Hm, float* um
                                                float* zmax, float* Cm) {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= p.N) return;
-    Hm[i] = Hm[i] + f.H[i];
-    um[i] = um[i] + f.uu[i];
-    vm[i] = vm[i] + f.vv[i];
-    hm[i] = hm[i] + f.hh[i];
-    zm[i] = zm[i] + f.zs[i];
-    zmax[i] = fmx(zmax[i], f.zs[i]);
-    Cm[i] = Cm[i] + f.C[i];
+    Hm[i] = Hm[i] + G(f.H)[i];
+    um[i] = um[i] + G(f.uu)[i];
+    vm[i] = vm[i] + G(f.vv)[i];
+    hm[i] = hm[i] + G(f.hh)[i];
+    zm[i] = zm[i] + G(f.zs)[i];
+    zmax[i] = fmx(zmax[i], G(f.zs)[i]);
+    Cm[i] = Cm[i] + G(f.C)[i];
 }
 
 __global__ __launch_bounds__(256) void k_scale(float* a, size_t n, float div) {
@@ -2844,12 +2853,12 @@ __global__ __launch_bounds__(256) void k_wave_from_sur(KP p, SP s, Fld f, const 
     const int x = gx(bi), yy = gy(bi);
     if (x >= p.nx || yy >= p.ny) return;
     const size_t i = (size_t)x + (size_t)yy * p.nx;
-    const float h = f.hh[i];
+    const float h = G(f.hh)[i];
     float kk, cc, cgg;
     dispersion_cell(p, s.sigm, h, kk, cc, cgg);
-    f.k[i] = kk;
-    f.c[i] = cc;
-    f.cg[i] = cgg;
+    G(f.k)[i] = kk;
+    G(f.c)[i] = cc;
+    G(f.cg)[i] = cgg;
     const int ys = p.ny < 3 ? yy : (yy == 0 ? 1 : (yy == p.ny - 1 ? p.ny - 2 : yy));
     const float* q = y + ((size_t)x + (size_t)ys * p.nx) * 5;
     const float wet = h > p.eps ? 1.0f : 0.0f;
@@ -2869,16 +2878,16 @@ __global__ __launch_bounds__(256) void k_wave_from_sur(KP p, SP s, Fld f, const 
     const float hhi = fmx(h, hmin);
     const float cib = fmx(cc, sqrtf(hmin * p.g));
     const float uorb = (float)(PI_ * (double)H / (double)s.Trep / (double)xm::sinhf(fmn(fmx(kk, 0.01f) * hhi, 10.0f)));
-    f.H[i] = H;
-    f.E[i] = E;
-    f.Dw[i] = Dw;
-    f.DR[i] = DR;
-    f.R[i] = R;
-    f.Fx[i] = Fx;
-    f.Fy[i] = Fy;
-    f.urms[i] = uorb / (sqrtf(2.0f));
-    f.ust[i] = (E / cib / p.rho / hhi + 2.0f * R / cib / p.rho / hhi) * wet;
-    f.thetamean[i] = dir;
+    G(f.H)[i] = H;
+    G(f.E)[i] = E;
+    G(f.Dw)[i] = Dw;
+    G(f.DR)[i] = DR;
+    G(f.R)[i] = R;
+    G(f.Fx)[i] = Fx;
+    G(f.Fy)[i] = Fy;
+    G(f.urms)[i] = uorb / (sqrtf(2.0f));
+    G(f.ust)[i] = (E / cib / p.rho / hhi + 2.0f * R / cib / p.rho / hhi) * wet;
+    G(f.thetamean)[i] = dir;
 }
 
 }  // namespace
@@ -3275,7 +3284,7 @@ __global__ __launch_bounds__(256) void k_ens_dt(DomDesc* tab, const float* part)
         if (!(a == a)) bad = 1;
         else v = fmn(v, a);
     }
-    for (int j = tid; j < D.nwave; j += 256) w = fmn(w, f.red_wave[j]);
+    for (int j = tid; j < D.nwave; j += 256) w = fmn(w, G(f.red_wave)[j]);
     for (int o = 32; o > 0; o >>= 1) {
         v = fmn(v, __shfl_xor(v, o, 64));
         w = fmn(w, __shfl_xor(w, o, 64));
@@ -3354,11 +3363,15 @@ __global__ __launch_bounds__(256) void k_ens_dt(DomDesc* tab, const float* part)
     float* St = (float*)g.St;
     float* qo = (float*)g.qbo;
     float* qn = (float*)g.qbn;
+    // one workgroup per domain streams the records: unrolled so that many
+    // loads are in flight at once
+#pragma unroll 8
     for (int j = tid; j < D.nst; j += 256) {
         double so = D.Stfile[(size_t)rec * D.nst + j];
         double sn = D.Stfile[(size_t)(rec + 1) * D.nst + j];
         St[j] = (float)(so + (tsl) / (tnext) * (sn - so));
     }
+#pragma unroll 8
     for (int j = tid; j < D.nq; j += 256) {
         qo[j] = (float)D.qfile[(size_t)rec * D.nq + j];
         qn[j] = (float)D.qfile[(size_t)(rec + 1) * D.nq + j];
@@ -3379,14 +3392,14 @@ __global__ __launch_bounds__(256) void kb_window_add(DTab tab, TMap map, int nti
             const Fld& f = D.f[D.par ^ 1][PH_START];
             const size_t i = (size_t)x + (size_t)y * D.p.nx, N = D.p.N;
             float* w = D.win;
-            w[i] += f.zs[i];
-            w[N + i] += f.uu[i];
-            w[2 * N + i] += f.vv[i];
-            w[3 * N + i] += f.H[i];
-            w[4 * N + i] += f.Dw[i];
-            w[5 * N + i] += f.DR[i];
-            w[6 * N + i] += f.Fx[i];
-            w[7 * N + i] += f.Fy[i];
+            w[i] += G(f.zs)[i];
+            w[N + i] += G(f.uu)[i];
+            w[2 * N + i] += G(f.vv)[i];
+            w[3 * N + i] += G(f.H)[i];
+            w[4 * N + i] += G(f.Dw)[i];
+            w[5 * N + i] += G(f.DR)[i];
+            w[6 * N + i] += G(f.Fx)[i];
+            w[7 * N + i] += G(f.Fy)[i];
         }
     }
 }
@@ -3406,7 +3419,7 @@ __global__ __launch_bounds__(256) void kb_window_sample(DTab tab, TMap map, int 
             const size_t i = (size_t)x + (size_t)y * D.p.nx, N = D.p.N;
             float* w = D.win;
             float* o = out + D.wout + i * 12;
-            o[0] = f.zb[i];
+            o[0] = G(f.zb)[i];
             for (int k = 0; k < 3; k++) o[1 + k] = w[k * N + i] * inv;
             for (int k = 0; k < 3; k++) o[4 + k] = meta[3 * e.dom + k];
             for (int k = 3; k < WIN_F; k++) o[4 + k] = w[k * N + i] * inv;
