@@ -877,9 +877,17 @@ XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         G(f.red_wave)[tile] = m;
     }
 }
+// With offshore set (current-free wave step only: it reads none of the
+// fields ubnd1D writes, and ubnd1D none that it writes) one workgroup more
+// than the tiles runs the offshore boundary beside the wave tiles.
 template <int NT, bool CUR, bool RQ>
-__global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f) {
-    wave_body<NT, CUR, RQ>(p, s, f, BI{xcd_tile(blockIdx.x, p.wtiles_x * p.wtiles_y), 0});
+__global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f, int offshore) {
+    const int ntiles = p.wtiles_x * p.wtiles_y;
+    if (!CUR && offshore && (int)blockIdx.x == ntiles) {
+        offshore_wg(p, s, f, 1, threadIdx.x + threadIdx.y * BX, WTHREADS);
+        return;
+    }
+    wave_body<NT, CUR, RQ>(p, s, f, BI{xcd_tile(blockIdx.x, ntiles), 0});
 }
 
 // Rows 0 and ny-1 (rollerlatbnd, wave_CPU.cpp:830-871): the post-Euler
@@ -1996,6 +2004,7 @@ struct xbg_ctx {
           *zsmax = nullptr, *Cmean = nullptr;
     float Trep = 10.0f, tsl = 0.0f, tnext = 1.0f, zsbndi = 0.0f, zsbndn = 0.0f, windu = 0.0f, windv = 0.0f;
     int nblk_wave = 0, nblk_flow = 0, nblk_sum = 0, acc_cur = 0;
+    bool pre_disp = false;  // the coming step's dispersion is already queued (xbg_advance_resident)
     float* host_red = nullptr;
     double* host_sum = nullptr;
     std::string err;
@@ -2201,26 +2210,34 @@ const char kFieldNames[] =
 // defer_force: the flow step follows and forms the wave forces in its first
 // launch
 template <int NT, bool CUR>
-int launch_k_wave(xbg_ctx* c, const SP& s, bool rq, dim3 gw, dim3 bw) {
-    if (rq) LAUNCH(c, "k_wave", (k_wave<NT, CUR, true>), gw, bw, c->kp, s, c->f);
-    else LAUNCH(c, "k_wave", (k_wave<NT, CUR, false>), gw, bw, c->kp, s, c->f);
+int launch_k_wave(xbg_ctx* c, const SP& s, bool rq, dim3 gw, dim3 bw, int offshore = 0) {
+    if (offshore) gw.x += 1;
+    if (rq) LAUNCH(c, "k_wave", (k_wave<NT, CUR, true>), gw, bw, c->kp, s, c->f, offshore);
+    else LAUNCH(c, "k_wave", (k_wave<NT, CUR, false>), gw, bw, c->kp, s, c->f, offshore);
     return 0;
 }
 
 int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0, int defer_force = 0) {
     KP& p = c->kp;
     dim3 g = grid2(p);
-    dim3 gd((p.nx + 4 * DPW - 1) / (4 * DPW), (p.ny + DPH - 1) / DPH + (offshore ? 1 : 0), 1);
-    LAUNCH(c, "k_dispersion", k_dispersion, gd, blk2, p, s, c->f, offshore);
+    if (!c->pre_disp) {
+        dim3 gd((p.nx + 4 * DPW - 1) / (4 * DPW), (p.ny + DPH - 1) / DPH + (offshore ? 1 : 0), 1);
+        LAUNCH(c, "k_dispersion", k_dispersion, gd, blk2, p, s, c->f, offshore);
+    }
+    // the dispersion ran ahead of dt: the offshore boundary rides in the
+    // current-free wave launch, else goes on its own before the wave step
+    const bool cur = p.wci != 0.0f;
+    const int off_w = c->pre_disp && offshore && !cur ? 1 : 0;
+    if (c->pre_disp && offshore && cur)
+        LAUNCH(c, "k_bnd_offshore", k_bnd_offshore, dim3(1), dim3(1024), p, s, c->f, 1);
     dim3 gr((p.nx + 255) / 256, 2, 1);
     dim3 gw(p.wtiles_x * p.wtiles_y, 1, 1), bw(BX, WTY, 1);
     // wci == 0 (the reference default): the current-free kernel
-    const bool cur = p.wci != 0.0f;
     const bool rq = s.sigm_recip && p.dth_recip;
     int rc = 0;
     switch (p.nt) {
         case 12:
-            rc = cur ? launch_k_wave<12, true>(c, s, rq, gw, bw) : launch_k_wave<12, false>(c, s, rq, gw, bw);
+            rc = cur ? launch_k_wave<12, true>(c, s, rq, gw, bw) : launch_k_wave<12, false>(c, s, rq, gw, bw, off_w);
             if (rc) return rc;
             if (!p.fold_rows) {
                 if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<12, true>), gr, dim3(256), p, s, c->f);
@@ -2228,7 +2245,7 @@ int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0, int defer_force = 0) 
             }
             break;
         case 24:
-            rc = cur ? launch_k_wave<24, true>(c, s, rq, gw, bw) : launch_k_wave<24, false>(c, s, rq, gw, bw);
+            rc = cur ? launch_k_wave<24, true>(c, s, rq, gw, bw) : launch_k_wave<24, false>(c, s, rq, gw, bw, off_w);
             if (rc) return rc;
             if (!p.fold_rows) {
                 if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<24, true>), gr, dim3(256), p, s, c->f);
@@ -2236,7 +2253,7 @@ int launch_wave(xbg_ctx* c, const SP& s, int offshore = 0, int defer_force = 0) 
             }
             break;
         default:
-            rc = cur ? launch_k_wave<0, true>(c, s, rq, gw, bw) : launch_k_wave<0, false>(c, s, rq, gw, bw);
+            rc = cur ? launch_k_wave<0, true>(c, s, rq, gw, bw) : launch_k_wave<0, false>(c, s, rq, gw, bw, off_w);
             if (rc) return rc;
             if (cur) LAUNCH(c, "k_wave_rows", (k_wave_rows<0, true>), gr, dim3(256), p, s, c->f);
             else LAUNCH(c, "k_wave_rows", (k_wave_rows<0, false>), gr, dim3(256), p, s, c->f);
@@ -2586,20 +2603,47 @@ int xbg_use_bnd_record(xbg_ctx* c, int rec, double timesincelast, double timenex
 // step, without returning to the caller between the dt read-back and the
 // launches of the step.  The offshore sea level and the wind stay as the
 // last xbg_set_flow_forcing / xbg_use_bnd_record left them.
+}  // extern "C"
+
+namespace {
+int cfl_launch(xbg_ctx* c, int initial, int* nb_out);
+int cfl_collect(xbg_ctx* c, int nb, double* dt_flow, double* dt_wave);
+}  // namespace
+
+extern "C" {
+
 int xbg_advance_resident(xbg_ctx* c, double t0, double dtbc, double Trep, double nextoutputtime, double endtime,
                          double* totaltime, double* dt) {
     if (!c || !totaltime || !dt || !(dtbc > 0.0)) return XBG_E_ARG;
     double dtf, dtw, dtn;
-    int rc = xbg_cfl_dt(c, 0, &dtf, &dtw);
+    int nb = 0;
+    int rc = cfl_launch(c, 0, &nb);
     if (rc) return rc;
-    if ((rc = xbg_next_dt(dtf, dtw, c->P.swave == 1, *dt, *totaltime, nextoutputtime, endtime, &dtn)) != XBG_OK)
-        return fail(c, rc, "Model crashed: non-finite or zero flow time step");
+    // The coming step's dispersion depends on the depths and the
+    // representative period only: queued now, it runs while the host waits
+    // for the CFL minima, chooses dt and launches the rest of the step (its
+    // offshore boundary then goes in a launch of its own: same sums, same
+    // order, same results).
+    if (c->P.swave == 1) {
+        c->Trep = (float)Trep;
+        const SP s0 = make_sp(c, 0.0, *totaltime);
+        const KP& p = c->kp;
+        dim3 gd((p.nx + 4 * DPW - 1) / (4 * DPW), (p.ny + DPH - 1) / DPH, 1);
+        LAUNCH(c, "k_dispersion", k_dispersion, gd, blk2, p, s0, c->f, 0);
+        c->pre_disp = true;
+    }
+    rc = cfl_collect(c, nb, &dtf, &dtw);
+    if (!rc && (rc = xbg_next_dt(dtf, dtw, c->P.swave == 1, *dt, *totaltime, nextoutputtime, endtime, &dtn)) != XBG_OK)
+        rc = fail(c, rc, "Model crashed: non-finite or zero flow time step");
     const double t = *totaltime + dtn;
-    const int rec = (int)std::floor((t - t0) / dtbc);
-    const double tsl = t - (rec * dtbc + t0);
-    if ((rc = xbg_use_bnd_record(c, rec, tsl, dtbc, Trep, c->zsbndi, c->zsbndn, c->windu, c->windv)) != XBG_OK)
-        return rc;
-    if ((rc = xbg_step(c, dtn, t)) != XBG_OK) return rc;
+    if (!rc) {
+        const int rec = (int)std::floor((t - t0) / dtbc);
+        const double tsl = t - (rec * dtbc + t0);
+        rc = xbg_use_bnd_record(c, rec, tsl, dtbc, Trep, c->zsbndi, c->zsbndn, c->windu, c->windv);
+    }
+    if (!rc) rc = xbg_step(c, dtn, t);
+    c->pre_disp = false;
+    if (rc) return rc;
     *totaltime = t;
     *dt = dtn;
     return XBG_OK;
@@ -2714,8 +2758,12 @@ int xbg_step(xbg_ctx* c, double dt, double totaltime) {
     return XBG_OK;
 }
 
-int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
-    if (!c) return XBG_E_ARG;
+}  // extern "C"
+
+namespace {
+// The CFL partial minima and their copy to the host, queued; *nb partial
+// pairs.
+int cfl_launch(xbg_ctx* c, int initial, int* nb_out) {
     float cfl = initial ? 0.5f * c->kp.CFL : c->kp.CFL;
     // about 8 cells per thread, at most CFL_PARTS workgroups
     size_t want = (c->N + 2047) / 2048;
@@ -2723,9 +2771,27 @@ int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
     LAUNCH(c, "k_cfl", k_cfl, dim3(nb), dim3(256), c->kp, c->f, cfl, c->nblk_wave, c->f.red_part);
     HIPC(c, hipMemcpyAsync(c->host_red, c->f.red_part, 2 * (size_t)nb * sizeof(float), hipMemcpyDeviceToHost,
                            c->stream));
+    HIPC(c, hipEventRecord(c->cfl_done, c->stream));
+    *nb_out = nb;
+    return XBG_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
+    if (!c) return XBG_E_ARG;
+    int nb = 0, rc = cfl_launch(c, initial, &nb);
+    if (rc) return rc;
+    return cfl_collect(c, nb, dt_flow, dt_wave);
+}
+
+}  // extern "C"
+
+namespace {
+int cfl_collect(xbg_ctx* c, int nb, double* dt_flow, double* dt_wave) {
     // the step cannot be launched before dt is known: poll for the copy
     // rather than block, so the next launches go out as soon as it lands
-    HIPC(c, hipEventRecord(c->cfl_done, c->stream));
     hipError_t q;
     while ((q = hipEventQuery(c->cfl_done)) == hipErrorNotReady) {
     }
@@ -2745,6 +2811,9 @@ int xbg_cfl_dt(xbg_ctx* c, int initial, double* dt_flow, double* dt_wave) {
         return fail(c, XBG_E_CRASH, "Model crashed: non-finite or zero flow time step");
     return XBG_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int xbg_accumulate_means(xbg_ctx* c) {
     if (!c) return XBG_E_ARG;
