@@ -236,6 +236,8 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ 
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     float a = 0.0f;
+    // unrolled: many loads in flight, the sum still in split order
+#pragma unroll 16
     for (int sp = 0; sp < nsplit; sp++) a += part[(size_t)sp * n + i];
     out[i] = a;
 }
@@ -255,13 +257,31 @@ __global__ __launch_bounds__(256) void k_flip_weights(const bf16* __restrict__ W
 }
 
 // Per-channel sums of a [P][C] BF16 tensor (bias gradient), partials per
-// workgroup over a range of pixels: part[blockIdx.x][c].
+// workgroup over a range of pixels: part[blockIdx.x][c].  With C dividing
+// 256 the workgroup's threads cover 256 / C pixel lanes of all C channels;
+// each lane sums its pixels in order and the lanes are added in lane order.
 __global__ __launch_bounds__(256) void k_colsum(const bf16* __restrict__ G, size_t P, int C, int ppb,
                                                 float* __restrict__ part) {
+    __shared__ float red[256];
     const size_t p0 = (size_t)blockIdx.x * ppb;
     const size_t p1 = p0 + ppb < P ? p0 + ppb : P;
+    if (C <= 256 && 256 % C == 0) {
+        const int lanes = 256 / C, c = threadIdx.x % C, l = threadIdx.x / C;
+        float a = 0.0f;
+#pragma unroll 8
+        for (size_t p = p0 + l; p < p1; p += lanes) a += (float)G[p * C + c];
+        red[threadIdx.x] = a;
+        __syncthreads();
+        if (l == 0) {
+            float t = 0.0f;
+            for (int k = 0; k < lanes; k++) t += red[k * C + c];
+            part[(size_t)blockIdx.x * C + c] = t;
+        }
+        return;
+    }
     for (int c = threadIdx.x; c < C; c += 256) {
         float a = 0.0f;
+#pragma unroll 16
         for (size_t p = p0; p < p1; p++) a += (float)G[p * C + c];
         part[(size_t)blockIdx.x * C + c] = a;
     }
@@ -387,20 +407,64 @@ __global__ __launch_bounds__(256) void k_head_dgrad(const float* __restrict__ dY
     dX[i] = to_bf16(a);
 }
 // partials of dW[k][c] = sum_p dY[p][k] X[p][c] and db[k] = sum_p dY[p][k]
-// over ppb pixels per workgroup: part[blk][K*C + K]
+// over ppb pixels per workgroup: part[blk][K*C + K].  With C dividing 256
+// the threads cover 256 / C pixel lanes of all C channels; each lane sums
+// its pixels in order and the lanes are added in lane order.
 template <int K>
 __global__ __launch_bounds__(256) void k_head_wgrad(const float* __restrict__ dY, const bf16* __restrict__ X,
                                                     size_t P, int C, int ppb, float* __restrict__ part) {
     const size_t p0 = (size_t)blockIdx.x * ppb;
     const size_t p1 = p0 + ppb < P ? p0 + ppb : P;
     const int n = K * C + K;
+    if (C <= 256 && 256 % C == 0) {
+        __shared__ float red[K][256];
+        __shared__ float redb[K][256];
+        const int lanes = 256 / C, c = threadIdx.x % C, l = threadIdx.x / C;
+        float a[K], b[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) a[k] = b[k] = 0.0f;
+#pragma unroll 4
+        for (size_t p = p0 + l; p < p1; p += lanes) {
+            const float x = (float)X[p * C + c];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const float d = dY[p * K + k];
+                a[k] += d * x;
+                b[k] += d;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            red[k][threadIdx.x] = a[k];
+            redb[k][threadIdx.x] = b[k];
+        }
+        __syncthreads();
+        if (l == 0) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                float t = 0.0f;
+                for (int q = 0; q < lanes; q++) t += red[k][q * C + c];
+                part[(size_t)blockIdx.x * n + k * C + c] = t;
+            }
+        }
+        if ((int)threadIdx.x < K) {
+            // the bias sums of the lanes' channel-0 threads
+            const int k = threadIdx.x;
+            float t = 0.0f;
+            for (int q = 0; q < lanes; q++) t += redb[k][q * C];
+            part[(size_t)blockIdx.x * n + K * C + k] = t;
+        }
+        return;
+    }
     for (int j = threadIdx.x; j < n; j += 256) {
         float a = 0.0f;
         if (j < K * C) {
             int k = j / C, c = j % C;
+#pragma unroll 16
             for (size_t p = p0; p < p1; p++) a += dY[p * K + k] * (float)X[p * C + c];
         } else {
             int k = j - K * C;
+#pragma unroll 16
             for (size_t p = p0; p < p1; p++) a += dY[p * K + k];
         }
         part[(size_t)blockIdx.x * n + j] = a;
