@@ -32,7 +32,8 @@ def close(a, b, rel):
 
 @pytest.mark.parametrize("n,H,W,cin,cout,taps,relu", [(2, 8, 64, 64, 32, 9, 1), (1, 6, 128, 32, 64, 9, 0),
                                                       (2, 4, 32, 128, 256, 9, 1), (2, 8, 64, 256, 128, 1, 0),
-                                                      (3, 10, 32, 64, 32, 1, 1)])
+                                                      (3, 10, 32, 64, 32, 1, 1), (2, 16, 32, 64, 64, 9, 1),
+                                                      (1, 12, 64, 32, 32, 9, 0)])
 def test_conv_forward_dgrad_wgrad(need_gpu, n, H, W, cin, cout, taps, relu):
     L = lib()
     g = torch.Generator().manual_seed(n * 1000 + H + cin + cout)

@@ -47,17 +47,20 @@ constexpr int LROW = KB + 8;  // LDS row (pixel) stride in bf16: 80 B, keeps 16-
 
 // ---------------------------------------------------------------------------
 // Forward convolution, 'same' padding (TAPS 9: 3x3, TAPS 1: 1x1).
-// Workgroup: 4 waves, TW pixels along x times 64/TW rows (64 pixels) and BN
-// output channels; wave w computes 16 pixels x BN channels.  Per K block
-// the input tile (+1-pixel halo for 3x3) and the block's weights are
-// staged in LDS.  Epilogue: + bias, optional ReLU, BF16 or FP32 store.
-// Requires Cin % 32 == 0, Cout % BN == 0, W % TW == 0, H % (64/TW) == 0.
+// Workgroup: 4 waves, TW pixels along x times 64 MB / TW rows (64 MB pixels)
+// and BN output channels; wave w computes MB blocks of 16 pixels (pixels
+// 64 m + 16 w + 0..15 of the tile, m < MB) x BN channels, so each staged
+// weight block feeds MB times the matrix work.  Per K block the input tile
+// (+1-pixel halo for 3x3) and the block's weights are staged in LDS.  The
+// sum over K blocks and taps runs in the same order for any MB.  Epilogue:
+// + bias, optional ReLU, BF16 or FP32 store.
+// Requires Cin % 32 == 0, Cout % BN == 0, W % TW == 0, H % (64 MB / TW) == 0.
 // ---------------------------------------------------------------------------
-template <int TAPS, int TW, int BN, bool RELU, bool OUTF32>
+template <int TAPS, int TW, int BN, int MB, bool RELU, bool OUTF32>
 __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ X, const bf16* __restrict__ Wt,
                                                   const float* __restrict__ bias, void* __restrict__ Yv, int H,
                                                   int W, int Cin, int Cout) {
-    constexpr int TR = 64 / TW;                 // rows per tile
+    constexpr int TR = 64 * MB / TW;            // rows per tile
     constexpr int HALO = TAPS == 9 ? 1 : 0;
     constexpr int SR = TR + 2 * HALO, SC = TW + 2 * HALO;
     __shared__ __attribute__((aligned(16))) bf16 sX[SR * SC * LROW];
@@ -67,12 +70,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ X, co
     const int xt = blockIdx.x % ntx, cb = blockIdx.x / ntx;
     const int x0 = xt * TW, y0 = blockIdx.y * TR, n = blockIdx.z;
     const int co0 = cb * BN;
-    f32x4 acc[BN / 16];
+    f32x4 acc[MB][BN / 16];
 #pragma unroll
-    for (int j = 0; j < BN / 16; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int q = 16 * w + (lane & 15);  // this lane's pixel (A row) within the tile
-    const int qr = q / TW, qc = q % TW;
-    const int s = lane >> 4;             // K slice of 8 channels
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int j = 0; j < BN / 16; j++) acc[m][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int s = lane >> 4;  // K slice of 8 channels
     const size_t img = (size_t)n * H * W;
     for (int c0 = 0; c0 < Cin; c0 += KB) {
         __syncthreads();
@@ -97,29 +100,38 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ X, co
 #pragma unroll
         for (int tap = 0; tap < TAPS; tap++) {
             const int dy = TAPS == 9 ? tap / 3 : 0, dx = TAPS == 9 ? tap % 3 : 0;
-            bf16x8 a = *(const bf16x8*)(sX + ((qr + dy) * SC + qc + dx) * LROW + s * 8);
+            bf16x8 a[MB];
+#pragma unroll
+            for (int m = 0; m < MB; m++) {
+                const int q = 64 * m + 16 * w + (lane & 15);  // this lane's pixel (A row) within the tile
+                a[m] = *(const bf16x8*)(sX + ((q / TW + dy) * SC + q % TW + dx) * LROW + s * 8);
+            }
 #pragma unroll
             for (int j = 0; j < BN / 16; j++) {
                 bf16x8 b = *(const bf16x8*)(sW + (tap * BN + 16 * j + (lane & 15)) * LROW + s * 8);
-                acc[j] = mfma16(a, b, acc[j]);
+#pragma unroll
+                for (int m = 0; m < MB; m++) acc[m][j] = mfma16(a[m], b, acc[m][j]);
             }
         }
     }
-    // epilogue: lane holds rows (pixels) 16w + 4s + r, column (channel) lane & 15
+    // epilogue: lane holds rows (pixels) 64 m + 16 w + 4 s + r, column
+    // (channel) lane & 15
 #pragma unroll
     for (int j = 0; j < BN / 16; j++) {
         const int co = co0 + 16 * j + (lane & 15);
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int qq = 16 * w + 4 * s + r;
-            const int yy = y0 + qq / TW, xx = x0 + qq % TW;
-            float v = acc[j][r] + bv;
-            if (RELU) v = v > 0.0f ? v : 0.0f;
-            const size_t o = (img + (size_t)yy * W + xx) * Cout + co;
-            if (OUTF32) ((float*)Yv)[o] = v;
-            else ((bf16*)Yv)[o] = to_bf16(v);
-        }
+        for (int m = 0; m < MB; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int qq = 64 * m + 16 * w + 4 * s + r;
+                const int yy = y0 + qq / TW, xx = x0 + qq % TW;
+                float v = acc[m][j][r] + bv;
+                if (RELU) v = v > 0.0f ? v : 0.0f;
+                const size_t o = (img + (size_t)yy * W + xx) * Cout + co;
+                if (OUTF32) ((float*)Yv)[o] = v;
+                else ((bf16*)Yv)[o] = to_bf16(v);
+            }
     }
 }
 
@@ -586,8 +598,14 @@ int conv_fwd(hipStream_t st, const bf16* X, const bf16* Wt, const float* bias, v
     int TW = W % 64 == 0 ? 64 : (W == 32 ? 32 : 0);
     if (!TW || H % (64 / TW)) return XBG_E_ARG;
     int BN = Cout % 64 == 0 ? 64 : 32;
-    dim3 g((W / TW) * (Cout / BN), H / (64 / TW), n);
-#define CF(T_, TW_, BN_, R_) k_conv_fwd<T_, TW_, BN_, R_, false><<<g, dim3(256), 0, st>>>(X, Wt, bias, Y, H, W, Cin, Cout)
+    // four 16-pixel blocks per wave where the rows divide, else one
+    const int MB = H % (4 * 64 / TW) == 0 ? 4 : 1;
+    dim3 g((W / TW) * (Cout / BN), H / (MB * 64 / TW), n);
+#define CF(T_, TW_, BN_, R_)                                                                                       \
+    do {                                                                                                       \
+        if (MB == 4) k_conv_fwd<T_, TW_, BN_, 4, R_, false><<<g, dim3(256), 0, st>>>(X, Wt, bias, Y, H, W, Cin, Cout); \
+        else k_conv_fwd<T_, TW_, BN_, 1, R_, false><<<g, dim3(256), 0, st>>>(X, Wt, bias, Y, H, W, Cin, Cout);    \
+    } while (0)
 #define CF_R(T_, TW_, BN_) do { if (relu) CF(T_, TW_, BN_, true); else CF(T_, TW_, BN_, false); } while (0)
 #define CF_BN(T_, TW_) do { if (BN == 64) CF_R(T_, TW_, 64); else CF_R(T_, TW_, 32); } while (0)
 #define CF_TW(T_) do { if (TW == 64) CF_BN(T_, 64); else CF_BN(T_, 32); } while (0)
