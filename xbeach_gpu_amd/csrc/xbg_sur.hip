@@ -141,12 +141,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const bf16* __restrict__ X, co
 // smaller) block of (co, ci), one row of taps (dy) and a range of image
 // rows; per 64-pixel segment dZ and the three x-shifted copies of X are
 // staged transposed (channel-major) in LDS so that a lane's 8 consecutive
-// pixels are one 16-B read.  Wave w owns output-channel tile w.  Partials
-// go to part[split][tap][co][ci] (FP32).
+// pixels are one 16-B read; each thread stages four consecutive pixels of
+// eight channels and writes each channel's four values as one 8-B store.
+// Waves take the output-channel tiles in turn; with fewer than four tiles
+// (32 output channels) two waves share a tile and split its input-channel
+// tiles.  Partials go to part[split][tap][co][ci] (FP32).
 // Requires W % 64 == 0 or W == 32 (TW), Cout % CBO == 0, Cin % CBI == 0.
 // ---------------------------------------------------------------------------
 constexpr int WPX = 64;           // pixels per staged segment
 constexpr int WLD = WPX + 8;      // LDS stride of a transposed channel row
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 template <int TAPS, int CBO, int CBI>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ X, const bf16* __restrict__ dZ,
                                                     float* __restrict__ part, int N, int H, int W, int Cin,
@@ -163,17 +167,21 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ X, 
     const int dy = blockIdx.y;  // 0..2 (TAPS 9) or 0
     const int split = blockIdx.z;
     const int co0 = cob * CBO, ci0 = cib * CBI;
-    constexpr int MT = CBO / 16;  // output-channel tiles, spread over the waves
-    constexpr int NT = CBI / 16;
+    constexpr int MT = CBO / 16;  // output-channel tiles
+    constexpr int NT = CBI / 16;  // input-channel tiles
+    constexpr int WPM = MT >= 4 ? 1 : 4 / MT;        // waves per output-channel tile
     constexpr int MT_PER_W = MT >= 4 ? MT / 4 : 1;
-    const bool wave_on = w < MT;
-    f32x4 acc[MT_PER_W][NDX][NT];
+    constexpr int NT_W = NT / WPM;                   // input-channel tiles per wave
+    static_assert(MT >= 4 ? MT % 4 == 0 : 4 % MT == 0, "output-channel tiles over four waves");
+    static_assert(NT % WPM == 0, "input-channel tiles over the waves of an output tile");
+    const int t0 = MT >= 4 ? 0 : (w / MT) * NT_W;
+    f32x4 acc[MT_PER_W][NDX][NT_W];
 #pragma unroll
     for (int a = 0; a < MT_PER_W; a++)
 #pragma unroll
         for (int d = 0; d < NDX; d++)
 #pragma unroll
-            for (int t = 0; t < NT; t++) acc[a][d][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < NT_W; t++) acc[a][d][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int total_rows = N * H;
     const int r0 = split * rows_per_split, r1 = min(total_rows, r0 + rows_per_split);
     const int segs = (W + WPX - 1) / WPX;
@@ -184,59 +192,67 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16* __restrict__ X, 
             const int x0 = sg * WPX;
             const int np = min(WPX, W - x0);
             __syncthreads();
-            // dZ segment, transposed: sD[co][p]
-            for (int k = tid; k < WPX * (CBO / 8); k += 256) {
-                int p = k / (CBO / 8), ch = (k % (CBO / 8)) * 8;
-                bf16x8 v = {};
-                if (p < np) v = *(const bf16x8*)(dZ + (((size_t)n * H + y) * W + x0 + p) * Cout + co0 + ch);
+            // dZ segment, transposed: sD[co][p], four pixels per thread
+            for (int k = tid; k < (WPX / 4) * (CBO / 8); k += 256) {
+                const int p = (k / (CBO / 8)) * 4, ch = (k % (CBO / 8)) * 8;
+                bf16x8 v[4];
 #pragma unroll
-                for (int e = 0; e < 8; e++) sD[(ch + e) * WLD + p] = v[e];
+                for (int u = 0; u < 4; u++) {
+                    v[u] = bf16x8{};
+                    if (p + u < np) v[u] = *(const bf16x8*)(dZ + (((size_t)n * H + y) * W + x0 + p + u) * Cout + co0 + ch);
+                }
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    *(bf16x4*)(sD + (ch + e) * WLD + p) = bf16x4{v[0][e], v[1][e], v[2][e], v[3][e]};
             }
             // X of the shifted row, one copy per dx, transposed: sXt[dx][ci][p]
-            for (int k = tid; k < NDX * WPX * (CBI / 8); k += 256) {
-                int d = k / (WPX * (CBI / 8));
-                int r = k % (WPX * (CBI / 8));
-                int p = r / (CBI / 8), ch = (r % (CBI / 8)) * 8;
-                int xs = x0 + p + (TAPS == 9 ? d - 1 : 0);
-                bf16x8 v = {};
-                if (p < np && ys >= 0 && ys < H && xs >= 0 && xs < W)
-                    v = *(const bf16x8*)(X + (((size_t)n * H + ys) * W + xs) * Cin + ci0 + ch);
+            for (int k = tid; k < NDX * (WPX / 4) * (CBI / 8); k += 256) {
+                const int d = k / ((WPX / 4) * (CBI / 8));
+                const int r = k % ((WPX / 4) * (CBI / 8));
+                const int p = (r / (CBI / 8)) * 4, ch = (r % (CBI / 8)) * 8;
+                bf16x8 v[4];
 #pragma unroll
-                for (int e = 0; e < 8; e++) sXt[(d * CBI + ch + e) * WLD + p] = v[e];
+                for (int u = 0; u < 4; u++) {
+                    const int xs = x0 + p + u + (TAPS == 9 ? d - 1 : 0);
+                    v[u] = bf16x8{};
+                    if (p + u < np && ys >= 0 && ys < H && xs >= 0 && xs < W)
+                        v[u] = *(const bf16x8*)(X + (((size_t)n * H + ys) * W + xs) * Cin + ci0 + ch);
+                }
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    *(bf16x4*)(sXt + (d * CBI + ch + e) * WLD + p) = bf16x4{v[0][e], v[1][e], v[2][e], v[3][e]};
             }
             __syncthreads();
-            if (wave_on) {
 #pragma unroll
-                for (int kk = 0; kk < WPX / 32; kk++) {
+            for (int kk = 0; kk < WPX / 32; kk++) {
 #pragma unroll
-                    for (int a = 0; a < MT_PER_W; a++) {
-                        const int mt = w + 4 * a;
-                        bf16x8 av = *(const bf16x8*)(sD + (mt * 16 + (lane & 15)) * WLD + kk * 32 + s * 8);
+                for (int a = 0; a < MT_PER_W; a++) {
+                    const int mt = MT >= 4 ? w + 4 * a : w % MT;
+                    bf16x8 av = *(const bf16x8*)(sD + (mt * 16 + (lane & 15)) * WLD + kk * 32 + s * 8);
 #pragma unroll
-                        for (int d = 0; d < NDX; d++)
+                    for (int d = 0; d < NDX; d++)
 #pragma unroll
-                            for (int t = 0; t < NT; t++) {
-                                bf16x8 bv =
-                                    *(const bf16x8*)(sXt + (d * CBI + t * 16 + (lane & 15)) * WLD + kk * 32 + s * 8);
-                                acc[a][d][t] = mfma16(av, bv, acc[a][d][t]);
-                            }
-                    }
+                        for (int t = 0; t < NT_W; t++) {
+                            bf16x8 bv = *(const bf16x8*)(sXt + (d * CBI + (t0 + t) * 16 + (lane & 15)) * WLD +
+                                                         kk * 32 + s * 8);
+                            acc[a][d][t] = mfma16(av, bv, acc[a][d][t]);
+                        }
                 }
             }
         }
     }
-    if (!wave_on) return;
-    // lane holds rows (co) 4s + r of tile mt, column (ci) lane & 15 of tile t
+    // lane holds rows (co) 4s + r of tile mt, column (ci) lane & 15 of tile t0 + t
 #pragma unroll
     for (int a = 0; a < MT_PER_W; a++)
 #pragma unroll
         for (int d = 0; d < NDX; d++)
 #pragma unroll
-            for (int t = 0; t < NT; t++)
+            for (int t = 0; t < NT_W; t++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int co = co0 + (w + 4 * a) * 16 + 4 * s + r;
-                    const int ci = ci0 + t * 16 + (lane & 15);
+                    const int mt = MT >= 4 ? w + 4 * a : w % MT;
+                    const int co = co0 + mt * 16 + 4 * s + r;
+                    const int ci = ci0 + (t0 + t) * 16 + (lane & 15);
                     const int tap = TAPS == 9 ? dy * 3 + d : 0;
                     part[(((size_t)split * TAPS + tap) * Cout + co) * Cin + ci] = acc[a][d][t][r];
                 }
