@@ -13,10 +13,12 @@ of H, wave dissipation, roller dissipation, Fx, Fy.  Two families:
             beach) under sea states drawn from the storm's range with seeds
             other than the storm run's, cut into 256 x 256 crops
 
-Targets are made dimensionless with the offshore wave height before the
-normalisation: H / Hm0 and the energy-like products (Dw, DR, Fx, Fy) over
-Hm0^2 (``TARGET_POW``), so the network does not have to learn the scale of
-the sea state; inference multiplies back (``phys_norm``).  In the site
+In the site family the targets are made dimensionless with the offshore
+wave height before the normalisation: H / Hm0 and the energy-like products
+(Dw, DR, Fx, Fy) over Hm0^2 (``TARGET_POW``), so the network does not have
+to learn the scale of the sea state; inference multiplies back
+(``phys_norm``).  The ensemble family keeps per-channel-normalised
+physical targets.  In the site
 family the zs, uu, vv inputs of a sample are the means of an earlier window
 (``ref_every``), matching the morphology run.  Samples stay in HBM as BF16
 (NHWC), normalised per channel with the mean and standard deviation of the
@@ -143,16 +145,17 @@ def crops(raw, size: int = 256):
 
 class Dataset:
     """Normalised BF16 samples in HBM; channel statistics from the first
-    batch added."""
+    batch added.  scale: targets divided by Hm0 ** TARGET_POW first."""
 
-    def __init__(self):
+    def __init__(self, scale: bool = False):
         self.X, self.T = [], []
         self.mean = self.std = None
+        self.target_pow = np.asarray(TARGET_POW if scale else (0.0,) * OUT_CH, np.float32)
 
     def add(self, raw):
         import torch
 
-        pw = torch.tensor(TARGET_POW, dtype=raw.dtype, device=raw.device)
+        pw = torch.tensor(self.target_pow, dtype=raw.dtype, device=raw.device)
         tgt = raw[..., 7:] / raw[..., 4:5].clamp_min(0.1) ** pw
         if self.mean is None:
             fi = raw[..., :7].reshape(-1, 7)
@@ -261,7 +264,7 @@ def main():
     ap.add_argument("--lr-final", type=float, default=None, help="cosine decay to this learning rate")
     ap.add_argument("--save", default=None, help="write parameters and normalisation (.npz)")
     a = ap.parse_args()
-    ds = Dataset()
+    ds = Dataset(scale=a.family == "site")
     info = {"gen_gpu_s": 0.0, "gen_host_s": 0.0, "gen_cell_steps": 0.0}
 
     def take(g, h, c):
@@ -293,7 +296,7 @@ def main():
            "train_loss": hist, "heldout_loss": held, "norm14": ds.norm14().tolist(), "norm10": ds.norm10().tolist()}
     if a.save:
         np.savez(a.save, params=sur.get_params(), norm14=ds.norm14(), norm10=ds.norm10(), heldout=held,
-                 target_pow=np.asarray(TARGET_POW, np.float32))
+                 target_pow=ds.target_pow)
     print(json.dumps(out))
 
 
