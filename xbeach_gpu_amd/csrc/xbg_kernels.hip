@@ -878,16 +878,18 @@ XDEV void wave_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     }
 }
 // With offshore set (current-free wave step only: it reads none of the
-// fields ubnd1D writes, and ubnd1D none that it writes) one workgroup more
-// than the tiles runs the offshore boundary beside the wave tiles.
+// fields ubnd1D writes, and ubnd1D none that it writes) the first workgroup
+// runs the offshore boundary beside the wave tiles, which take the
+// following ones (their XCD bands rotate by one XCD and stay contiguous).
 template <int NT, bool CUR, bool RQ>
 __global__ __launch_bounds__(WTHREADS, 4) void k_wave(KP p, SP s, Fld f, int offshore) {
     const int ntiles = p.wtiles_x * p.wtiles_y;
-    if (!CUR && offshore && (int)blockIdx.x == ntiles) {
+    const int off = !CUR && offshore ? 1 : 0;
+    if (off && blockIdx.x == 0) {
         offshore_wg(p, s, f, 1, threadIdx.x + threadIdx.y * BX, WTHREADS);
         return;
     }
-    wave_body<NT, CUR, RQ>(p, s, f, BI{xcd_tile(blockIdx.x, ntiles), 0});
+    wave_body<NT, CUR, RQ>(p, s, f, BI{xcd_tile((int)blockIdx.x - off, ntiles), 0});
 }
 
 // Rows 0 and ny-1 (rollerlatbnd, wave_CPU.cpp:830-871): the post-Euler
