@@ -227,7 +227,9 @@ int xbg_cfl_dt(xbg_ctx* ctx, int initial, double* dt_flow, double* dt_wave);
  * last xbg_set_flow_forcing / xbg_use_bnd_record set them), then xbg_step.  On return
  * *totaltime and *dt hold the new time and step.  The same calls a host
  * loop would make, without a return to the caller between the CFL read-back
- * and the launches of the step.
+ * and the launches of the step; the coming step's dispersion (which needs
+ * only the depths and Trep) is queued before the read-back, so it runs
+ * while the host chooses dt.  Results equal those calls bit for bit.
  */
 int xbg_advance_resident(xbg_ctx* ctx, double t0, double dtbc, double Trep, double nextoutputtime,
                          double endtime, double* totaltime, double* dt);
@@ -265,7 +267,9 @@ int xbg_mass_balance(xbg_ctx* ctx, double* water_volume, double* bed_volume);
  * the wave step (k, c, cg, H, E, Dw, DR, R, Fx, Fy, urms, ust, thetamean)
  * formed from surrogate predictions y (device FP32 [ny][nx][5]: H, Dw, DR,
  * Fx, Fy, normalised; norm10 = 5 means then 5 standard deviations, host)
- * and the offshore mean direction dir (grid radians).  xbg_step_nowave: a
+ * and the offshore mean direction dir (grid radians); H at most gammax h
+ * with the energy-like products scaled as energint scales the energy, rows
+ * 0 and ny-1 from rows 1 and ny-2.  xbg_step_nowave: a
  * step with those products held (offshore boundary, flow, sediment and
  * bed; no wave action solve).  xbg_advance_window: main-loop iterations on
  * the resident records until t_end (steps aligned on it), full steps
