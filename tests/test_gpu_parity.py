@@ -275,14 +275,18 @@ def test_lake_at_rest_full_size():
 
 @pytest.mark.parametrize("kw", [dict(), dict(dtheta=5.0, zb_accum=2), dict(roller=0, breakmodel=2),
                                 dict(wci=1.0, hwci=0.5, morfac=10.0), dict(nx=130, ny=37, dtheta=120.0 / 7),
-                                dict(nx=96, ny=33, dtheta=5.0), dict(dx=2.0, dtheta=5.0, zb_accum=2)],
+                                dict(nx=96, ny=33, dtheta=5.0), dict(dx=2.0, dtheta=5.0, zb_accum=2),
+                                dict(dx=4.0, morfac=10.0, zb_accum=1), dict(dx=8.0, wci=1.0, hwci=0.5, roller=0)],
                          ids=["default", "24bins-fp64zb", "noroller-baldock", "wci-morfac10", "130x37-7bins",
-                              "96x33-24bins-rowlaunch", "dx2-24bins"])
+                              "96x33-24bins-rowlaunch", "dx2-24bins", "dx4-morfac10", "dx8-wci-noroller"])
 def test_bitwise_identical_to_oracle(kw):
     """The kernels and the oracle evaluate every elementary function with the
     same deterministic implementation and keep the reference's operation
     order, so the GPU step reproduces the FP32 oracle bit for bit, which is
-    stronger than the tolerances above; checked over 50 steps."""
+    stronger than the tolerances above; checked over 50 steps.  The dx = 2,
+    4 and 8 cases run the power-of-two spacing
+    instantiations (products with the exact 1/dx), the others the
+    divisions."""
     nx, ny = kw.pop("nx", 200), kw.pop("ny", 50)
     p, d, o, frc = pair(nx=nx, ny=ny, **kw)
     advance(d, o, frc, 50)
@@ -291,12 +295,13 @@ def test_bitwise_identical_to_oracle(kw):
     d.close()
 
 
-@pytest.mark.parametrize("lat", [0.0, 35.0])
-def test_bitwise_with_tide_gradient_wind_and_coriolis(lat):
+@pytest.mark.parametrize("lat,dx", [(0.0, 5.0), (35.0, 5.0), (35.0, 4.0)])
+def test_bitwise_with_tide_gradient_wind_and_coriolis(lat, dx):
     """Non-zero offshore sea level varying alongshore (zsbndi != zsbndn),
     wind in both directions and, at lat 35, the Coriolis terms: 50 steps
-    bit for bit against the oracle."""
-    p, d, o, frc = pair(nx=120, ny=44, lat=lat)
+    bit for bit against the oracle (dx = 4: the power-of-two spacing
+    instantiations)."""
+    p, d, o, frc = pair(nx=120, ny=44, lat=lat, dx=dx)
     assert (p.fc != 0.0) == (lat != 0.0)
     advance(d, o, frc, 50, zsbndi=0.3, zsbndn=-0.15, windu=11.0, windv=-7.0)
     for k in MAIN + MORE:

@@ -44,6 +44,8 @@ struct KP {
     int fold_rows;           // rows 0 / ny-1 finished inside the wave kernel
     double rdtheta;          // 1/dtheta rounded once in double (div_uniform)
     int dth_recip;           // 0 < dtheta < 2: divide through rdtheta
+    float rdx;               // 1/dx, exact when dx_p2 is set
+    int dx_p2;               // dx is a power of two (Dx<true> applies)
     float theta[MAXT], cxsth[MAXT], sxnth[MAXT];
 };
 
@@ -66,6 +68,24 @@ struct SP {
 XDEV float div_uniform(float a, float b, double rb, int use_rb) {
     if (use_rb) return (float)((double)a * rb);
     return a / b;
+}
+
+// The grid spacing as a divisor.  With a power-of-two dx, v / dx and
+// v * (1/dx) are the same real number, so the IEEE quotient and the product
+// round to the same float (subnormal and overflowing results, zeros,
+// infinities and NaN included): Dx<true> multiplies, Dx<false> divides.
+// Kernels pick the instantiation from KP::dx_p2.
+template <bool P2>
+struct Dx {
+    float d, r;  // dx and its reciprocal (exact for P2)
+};
+template <bool P2>
+XDEV float operator/(float v, Dx<P2> q) {
+    return P2 ? v * q.r : v / q.d;
+}
+template <bool P2>
+XDEV Dx<P2> dx_of(const KP& p, float m = 1.0f) {  // m * dx, m a power of two
+    return Dx<P2>{m * p.dx, p.rdx / m};
 }
 
 // Friction maps derived from the sediment thickness, updatezom

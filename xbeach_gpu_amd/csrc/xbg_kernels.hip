@@ -926,8 +926,10 @@ __global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
 }
 
 // wavforceCPU (wave_CPU.cpp:1159-1223) at (x, y)
+template <bool P2 = false>
 XDEV void wavforce_at(const KP& p, const Fld& f, int x, int y, float& Fxv, float& Fyv) {
     const int nx = p.nx, ny = p.ny;
+    const Dx<P2> dx = dx_of<P2>(p), dx2 = dx_of<P2>(p, 2.0f);
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     auto I = [nx](int a, int b) { return (size_t)a + (size_t)b * nx; };
     float Sxxi = G(f.Sxx)[I(x, y)], Sxxr = G(f.Sxx)[I(xp, y)];
@@ -935,25 +937,26 @@ XDEV void wavforce_at(const KP& p, const Fld& f, int x, int y, float& Fxv, float
     float Sxyb = G(f.Sxy)[I(x, ym)], Sxybr = G(f.Sxy)[I(xp, ym)];
     float Syyi = G(f.Syy)[I(x, y)], Syyt = G(f.Syy)[I(x, yp)];
     float Sxyr = G(f.Sxy)[I(xp, y)], Sxyl = G(f.Sxy)[I(xm, y)], Sxytl = G(f.Sxy)[I(xm, yp)];
-    Fxv = -1 * (Sxxr - Sxxi) / p.dx - 0.5f * (Sxyt + Sxytr - Sxyb - Sxybr) / (2.0f * p.dx);
-    Fyv = -1 * (Syyt - Syyi) / p.dx - 0.5f * (Sxyr + Sxytr - Sxyl - Sxytl) / (2.0f * p.dx);
+    Fxv = -1 * (Sxxr - Sxxi) / dx - 0.5f * (Sxyt + Sxytr - Sxyb - Sxybr) / dx2;
+    Fyv = -1 * (Syyt - Syyi) / dx - 0.5f * (Sxyr + Sxytr - Sxyl - Sxytl) / dx2;
 }
 
 // wavforce + twodimbndnoix(Fx) + twodimbnd(Fy) + twodimbnd(ust)
 // (Wavestep.cu:1099-1125; CPU twins wave_CPU.cpp:1224-1331: the neighbour
 // value read is the pre-update one and the ix==0 rule wins).
+template <bool P2 = false>
 XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     size_t i = (size_t)x + (size_t)y * nx;
     float wet = G(f.hh)[i] > p.eps ? 1.0f : 0.0f;
     int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
     float Fxv, Fyv;
-    wavforce_at(p, f, x, ys, Fxv, Fyv);
+    wavforce_at<P2>(p, f, x, ys, Fxv, Fyv);
     float fx = ys != y ? Fxv * wet : Fxv;
     float fy, ust;
     if (x == 0) {
         float a, b;
-        wavforce_at(p, f, p1(0, nx), y, a, b);
+        wavforce_at<P2>(p, f, p1(0, nx), y, a, b);
         fy = b * wet;
         ust = G(f.ust_raw)[(size_t)p1(0, nx) + (size_t)y * nx] * wet;
     } else if (ys != y) {
@@ -970,7 +973,8 @@ XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
 __global__ __launch_bounds__(256) void k_wave_force(KP p, Fld f) {
     int x = gx(BI{(int)blockIdx.x, (int)blockIdx.y}), y = gy(BI{(int)blockIdx.x, (int)blockIdx.y});
     if (x >= p.nx || y >= p.ny) return;
-    wave_force_cell(p, f, x, y);
+    if (p.dx_p2) wave_force_cell<true>(p, f, x, y);
+    else wave_force_cell<false>(p, f, x, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -991,10 +995,12 @@ XDEV float wetv_at(const KP& p, const Fld& f, int x, int y) { return (float)((G(
 // udepthmomcont/vdepthmomcont from the previous step's hu/hv.  With force
 // set, also the wave forces of the step (k_wave_force's work: neither reads
 // what the other writes), one launch where a whole step runs both.
+template <bool P2 = false>
 XDEV void flow_nuh_body(const KP& p, const Fld& f, int force, BI bi) {
     int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
-    if (force) wave_force_cell(p, f, x, y);
+    if (force) wave_force_cell<P2>(p, f, x, y);
+    const Dx<P2> dx = dx_of<P2>(p);
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     size_t i = ix2(p, x, y);
@@ -1009,10 +1015,10 @@ XDEV void flow_nuh_body(const KP& p, const Fld& f, int force, BI bi) {
     if (p.usesmago == 1) {
         const auto uu = G(f.uu);
         const auto vv = G(f.vv);
-        float dudx = (uu[i] - uu[ix2(p, xm, y)]) / p.dx;
-        float dudy = 0.50f * (uu[ix2(p, x, yp)] - uu[ix2(p, x, ym)] + uu[ix2(p, xm, yp)] - uu[ix2(p, xm, ym)]) / p.dx;
-        float dvdy = (vv[i] - vv[ix2(p, x, ym)]) / p.dx;
-        float dvdx = 0.50f * (vv[ix2(p, xp, y)] - vv[ix2(p, xm, y)] + vv[ix2(p, xp, ym)] - vv[ix2(p, xm, ym)]) / p.dx;
+        float dudx = (uu[i] - uu[ix2(p, xm, y)]) / dx;
+        float dudy = 0.50f * (uu[ix2(p, x, yp)] - uu[ix2(p, x, ym)] + uu[ix2(p, xm, yp)] - uu[ix2(p, xm, ym)]) / dx;
+        float dvdy = (vv[i] - vv[ix2(p, x, ym)]) / dx;
+        float dvdx = 0.50f * (vv[ix2(p, xp, y)] - vv[ix2(p, xm, y)] + vv[ix2(p, xp, ym)] - vv[ix2(p, xm, ym)]) / dx;
         float tau = sqrtf(2.0f * dudx * dudx + 2.0f * dvdy * dvdy + xm::powf(dudy + dvdx, 2.0f));
         G(f.nuh)[i] = fmx(p.nuh * p.nuh * p.dx * p.dx * tau, turb);
     } else {
@@ -1020,11 +1026,13 @@ XDEV void flow_nuh_body(const KP& p, const Fld& f, int force, BI bi) {
     }
 }
 __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
-    flow_nuh_body(p, f, force, BI{(int)blockIdx.x, (int)blockIdx.y});
+    if (p.dx_p2) flow_nuh_body<true>(p, f, force, BI{(int)blockIdx.x, (int)blockIdx.y});
+    else flow_nuh_body<false>(p, f, force, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // eulerustep (Flow_kernel.cu:1487-1544) at (x, y) with ududx_adv (752-823),
 // vdudy_adv (887-949), viscou1/viscou2 (1319-1397), wlevslopes (410-466).
+template <bool P2 = false>
 XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     size_t i = ix2(p, x, y);
@@ -1033,7 +1041,7 @@ XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     const auto uu = G(f.uu);
     const auto vv = G(f.vv);
-    const float dx = p.dx;
+    const Dx<P2> dx = dx_of<P2>(p);
     size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), br = ix2(p, xp, ym);
     float hum = hum_at(p, f, x, y);
     // ududx_adv
@@ -1088,29 +1096,34 @@ XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
 
 // eulerustep + latbnd(uu) (Flow_kernel.cu:1778-1809): rows 0 / ny-1 copy
 // the updated rows 1 / ny-2.
+template <bool P2 = false>
 XDEV void flow_u_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
-    G(f.uu_b)[ix2(p, x, y)] = eulerustep_at(p, s, f, x, ys);
+    G(f.uu_b)[ix2(p, x, y)] = eulerustep_at<P2>(p, s, f, x, ys);
 }
 __global__ __launch_bounds__(256) void k_flow_u(KP p, SP s, Fld f) {
-    flow_u_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    if (p.dx_p2) flow_u_body<true>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    else flow_u_body<false>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // viscov1 (Flow_kernel.cu:1399-1429) at (x, y)
+template <bool P2 = false>
 XDEV float viscov1_at(const KP& p, const Fld& f, int x, int y) {
     int ym = m1(y), yp = p1(y, p.ny);
+    const Dx<P2> dx = dx_of<P2>(p);
     size_t i = ix2(p, x, y), t = ix2(p, x, yp), b = ix2(p, x, ym);
     const auto vv = G(f.vv);
-    float dvdy1 = G(f.nuh)[t] * G(f.hh)[t] * (vv[t] - vv[i]) / p.dx;
-    float dvdy2 = G(f.nuh)[i] * G(f.hh)[i] * (vv[i] - vv[b]) / p.dx;
-    return (1.0f / hvm_at(p, f, x, y)) * (dvdy1 - dvdy2) / p.dx * wetv_at(p, f, x, yp) * wetv_at(p, f, x, ym);
+    float dvdy1 = G(f.nuh)[t] * G(f.hh)[t] * (vv[t] - vv[i]) / dx;
+    float dvdy2 = G(f.nuh)[i] * G(f.hh)[i] * (vv[i] - vv[b]) / dx;
+    return (1.0f / hvm_at(p, f, x, y)) * (dvdy1 - dvdy2) / dx * wetv_at(p, f, x, yp) * wetv_at(p, f, x, ym);
 }
 
 // eulervstep (Flow_kernel.cu:1546-1599) at (x, y) with udvdx_adv
 // (1159-1210, on the updated uu), vdvdy_adv (1024-1074), viscov1 +
 // viscovbnd + viscov2 (1399-1483), wlevslopes.
+template <bool P2 = false>
 XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     size_t i = ix2(p, x, y);
@@ -1119,7 +1132,7 @@ XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     const auto un = G(f.uu_b);
     const auto vv = G(f.vv);
-    const float dx = p.dx;
+    const Dx<P2> dx = dx_of<P2>(p);
     size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), tl = ix2(p, xm, yp);
     float hvm = hvm_at(p, f, x, y);
     // udvdx_adv
@@ -1155,7 +1168,7 @@ XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     }
     float vdvdy = res;
     // viscov1 (row 0 takes row 1 through viscovbnd) + viscov2
-    float viscv = viscov1_at(p, f, x, y == 0 ? p1(0, ny) : y);
+    float viscv = viscov1_at<P2>(p, f, x, y == 0 ? p1(0, ny) : y);
     const auto nuh = G(f.nuh);
     float nuh1 = 0.25f * (nuh[i] + nuh[r] + nuh[ix2(p, xp, yp)] + nuh[t]);
     float nuh2 = 0.25f * (nuh[i] + nuh[l] + nuh[tl] + nuh[t]);
@@ -1173,14 +1186,16 @@ XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
 
 // eulervstep + vvlatbnd (Flow_kernel.cu:1812-1844): rows 0 / ny-2 copy the
 // updated rows 1 / ny-3.
+template <bool P2 = false>
 XDEV void flow_v_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     int ys = (y == 0) ? 1 : (y == p.ny - 2 ? p.ny - 3 : y);
-    G(f.vv_b)[ix2(p, x, y)] = eulervstep_at(p, s, f, x, ys);
+    G(f.vv_b)[ix2(p, x, y)] = eulervstep_at<P2>(p, s, f, x, ys);
 }
 __global__ __launch_bounds__(256) void k_flow_v(KP p, SP s, Fld f) {
-    flow_v_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    if (p.dx_p2) flow_v_body<true>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    else flow_v_body<false>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // wrkuu2Ocorr (1960-2044), wrkvv2Ocorr (2046-2128), uu2Ocorr (2210-2245),
@@ -1197,9 +1212,20 @@ constexpr int CY = 16;
 constexpr int DW = BX + 6, DH = CY + 6;  // data tile:  x0-3 .. x0+BX+2, y0-3 .. y0+CY+2
 constexpr int KW = BX + 2, KH = CY + 2;  // flux tile:  x0-1 .. x0+BX,   y0-1 .. y0+CY
 enum { T_ZS, T_ZB, T_QX, T_QY, T_U, T_UO, T_V, T_VO, T_N };
+// LDS of the tiled kernels is declared in non-template functions, so the two
+// spacing instantiations a kernel holds share one allocation.
+struct UvcorrLds {
+    float d[T_N][DH][DW];
+    float k[4][KH][KW];
+};
+XDEV UvcorrLds& uvcorr_lds() {
+    __shared__ UvcorrLds l;
+    return l;
+}
+template <bool P2 = false>
 XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
-    __shared__ float dt_[T_N][DH][DW];
-    __shared__ float kt[4][KH][KW];
+    float (&dt_)[T_N][DH][DW] = uvcorr_lds().d;
+    float (&kt)[4][KH][KW] = uvcorr_lds().k;
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
     const int x0 = bi.x * BX, y0 = bi.y * CY;
@@ -1282,7 +1308,8 @@ XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     if (x >= nx || y >= ny) return;
     auto K = [&](int k, int ux, int uy) { return kt[k][uy - (y0 - 1)][ux - (x0 - 1)]; };
     size_t i = ix2(p, x, y);
-    const float dx = p.dx, dts = s.dt;
+    const Dx<P2> dx = dx_of<P2>(p);
+    const float dts = s.dt;
     float u = D(T_U, x, y), v = D(T_V, x, y);
     if (x > 0 && y > 0) {
         float qe = 0.5f * (D(T_QX, x + 1, y) + D(T_QX, x, y));
@@ -1320,11 +1347,13 @@ XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     G(f.hv)[i] = fmx(h, 0.0f);
 }
 __global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
-    flow_uvcorr_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    if (p.dx_p2) flow_uvcorr_body<true>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    else flow_uvcorr_body<false>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // CalcQFlow (Flow_kernel.cu:1917-1958) + continuity (1626-1681) at (x, y):
 // the water level after continuity; *dzdt_out gets dzsdt.
+template <bool P2 = false>
 XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, float* dzdt_out) {
     size_t i = ix2(p, x, y);
     float z = G(f.zs)[i];
@@ -1332,7 +1361,7 @@ XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, f
         size_t l = ix2(p, x - 1, y), b = ix2(p, x, m1(y));
         float qxi = G(f.uu)[i] * G(f.hu)[i], qxl = G(f.uu)[l] * G(f.hu)[l];
         float qyi = G(f.vv)[i] * G(f.hv)[i], qyb = G(f.vv)[b] * G(f.hv)[b];
-        float dzdt = (qxl - qxi + qyb - qyi) / p.dx;
+        float dzdt = (qxl - qxi + qyb - qyi) / dx_of<P2>(p);
         *dzdt_out = dzdt;
         z = z + dzdt * s.dt;
     }
@@ -1351,9 +1380,19 @@ XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, f
 // zs buffer: other tiles still read the old one in step 1.
 constexpr int ZW = BX + 4, ZH = CY + 5;  // level tile: x0-2 .. x0+BX+1, y0-3 .. y0+CY+1
 constexpr int FW = BX + 1, FH = CY + 2;  // flux tile:  x0-1 .. x0+BX-1, y0-2 .. y0+CY-1
+struct ZsallLds {
+    float z[ZH][ZW];
+    float w1[FH][FW], w2[FH][FW];
+};
+XDEV ZsallLds& zsall_lds() {
+    __shared__ ZsallLds l;
+    return l;
+}
+template <bool P2 = false>
 XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
-    __shared__ float zt[ZH][ZW];
-    __shared__ float w1t[FH][FW], w2t[FH][FW];
+    float (&zt)[ZH][ZW] = zsall_lds().z;
+    float (&w1t)[FH][FW] = zsall_lds().w1;
+    float (&w2t)[FH][FW] = zsall_lds().w2;
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
     const int x0 = bi.x * BX, y0 = bi.y * CY;
@@ -1362,7 +1401,7 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         int ux = x0 - 2 + c, uy = y0 - 3 + r;
         int gxx = min(max(ux, 0), nx - 1), gyy = min(max(uy, 0), ny - 1);
         float dzdt = 0.0f;
-        float z = continuity_at(p, s, f, gxx, gyy, &dzdt);
+        float z = continuity_at<P2>(p, s, f, gxx, gyy, &dzdt);
         zt[r][c] = z;
         if (ux == gxx && uy == gyy && ux >= x0 && ux < x0 + BX && uy >= y0 && uy < y0 + CY && gxx > 0)
             G(f.dzsdt)[ix2(p, gxx, gyy)] = dzdt;
@@ -1408,7 +1447,8 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     float z = zt[ys - (y0 - 3)][cz];
     if (x > 0 && ys > 0) {
         int rf = ys - (y0 - 2);
-        z = z - s.dt * ((w1t[rf][cf] - w1t[rf][cf - 1]) / p.dx + (w2t[rf][cf] - w2t[rf - 1][cf]) / p.dx);
+        const Dx<P2> dx = dx_of<P2>(p);
+        z = z - s.dt * ((w1t[rf][cf] - w1t[rf][cf - 1]) / dx + (w2t[rf][cf] - w2t[rf - 1][cf]) / dx);
     }
     float qxv = G(f.uu)[i] * G(f.hu)[i], qyv = G(f.vv)[i] * G(f.hv)[i];
     if (x > 0 && y > 0) {
@@ -1422,7 +1462,8 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     G(f.hh)[i] = fmx(G(f.zb)[i] + z, p.eps);
 }
 __global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
-    flow_zsall_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    if (p.dx_p2) flow_zsall_body<true>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    else flow_zsall_body<false>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // calcuvvu (Flow_kernel.cu:467-541) at (x, y)
@@ -1726,34 +1767,36 @@ __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) 
 
 // avalanching (Sediment_kernel.cu:1020-1137) at (x, y): slope-limited
 // exchange with the +x and +y neighbours.
+template <bool P2 = false>
 XDEV void aval_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& bxo, float& byo) {
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
     const auto zb = G(f.zb_b);
     const auto sd = G(f.stdep_b);
     float dzmax = G(f.hh_b)[i] > p.eps ? p.wetdzmax : p.drydzmax;
     float maxchg = p.maxslpchg, dt = s.dt, dx = p.dx;
+    const Dx<P2> ddx = dx_of<P2>(p);
     float bx = 0.0f, by = 0.0f;
-    float sxv = (zb[r] - zb[i]) / dx;
+    float sxv = (zb[r] - zb[i]) / ddx;
     if (fabsf(sxv) > dzmax) {
         float sg = sxv / fabsf(sxv);
         bx = sg * (fabsf(sxv) - dzmax) * dx;
         if (sg > 0) {
-            bx = fmn(bx, maxchg * dt / dx);
+            bx = fmn(bx, maxchg * dt / ddx);
             bx = fmn(bx, sd[i]);
         } else {
-            bx = fmx(bx, -1.0f * maxchg * dt / dx);
+            bx = fmx(bx, -1.0f * maxchg * dt / ddx);
             bx = fmx(bx, -1.0f * sd[r]);
         }
     }
-    float syv = (zb[t] - zb[i]) / dx;
+    float syv = (zb[t] - zb[i]) / ddx;
     if (fabsf(syv) > dzmax) {
         float sg = syv / fabsf(syv);
         by = sg * (fabsf(syv) - dzmax) * dx;
         if (sg > 0) {
-            by = fmn(by, maxchg * dt / dx);
+            by = fmn(by, maxchg * dt / ddx);
             by = fmn(by, sd[i]);
         } else {
-            by = fmx(by, -1 * maxchg * dt / dx);
+            by = fmx(by, -1 * maxchg * dt / ddx);
             by = fmx(by, -1 * sd[t]);
         }
     }
@@ -1769,8 +1812,17 @@ XDEV void aval_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& b
 // scatters the exchange into both cells without atomics; the change of each
 // cell is gathered here as (dzbx + dzby)_i - dzbx_{i-1} - dzby_{i-nx}.
 constexpr int MW = BX + 1, MH = BY + 2;  // exchange tile: x0-1 .. x0+BX-1, y0-2 .. y0+BY-1
+struct MorphLds {
+    float x[MH][MW], y[MH][MW];
+};
+XDEV MorphLds& morph_lds() {
+    __shared__ MorphLds l;
+    return l;
+}
+template <bool P2 = false>
 XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
-    __shared__ float ex[MH][MW], ey[MH][MW];
+    float (&ex)[MH][MW] = morph_lds().x;
+    float (&ey)[MH][MW] = morph_lds().y;
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
     const int x0 = bi.x * BX, y0 = bi.y * BY;
@@ -1778,7 +1830,7 @@ XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         int r = l / MW, c = l - r * MW;
         int hx = x0 - 1 + c, hy = y0 - 2 + r;
         float bx = 0.0f, by = 0.0f;
-        if (hx >= 0 && hy >= 0 && hx < nx && hy < ny) aval_at(p, s, f, hx, hy, bx, by);
+        if (hx >= 0 && hy >= 0 && hx < nx && hy < ny) aval_at<P2>(p, s, f, hx, hy, bx, by);
         ex[r][c] = bx;
         ey[r][c] = by;
     }
@@ -1818,7 +1870,8 @@ XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     }
 }
 __global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
-    sed_morph_body(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    if (p.dx_p2) sed_morph_body<true>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
+    else sed_morph_body<false>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
 // ---------------------------------------------------------------------------
@@ -2132,6 +2185,12 @@ void make_kp(xbg_ctx* c) {
     k.fold_rows = P.ny > WTY && (P.ny - 1) % WTY != 0 ? 1 : 0;
     k.rdtheta = 1.0 / (double)k.dtheta;
     k.dth_recip = k.dtheta > 0.0f && k.dtheta < 2.0f ? 1 : 0;
+    {
+        int ex = 0;
+        const float m = std::frexp(k.dx, &ex);
+        k.dx_p2 = k.dx > 0.0f && m == 0.5f && std::isnormal(k.dx) ? 1 : 0;
+        k.rdx = k.dx_p2 ? std::ldexp(1.0f, 1 - ex) : 1.0f / k.dx;
+    }
     k.usesmago = P.usesmago; k.breakmodel = P.breakmodel; k.roller = P.roller;
     k.zb_accum = P.zb_accum; k.swave = P.swave;
     xbg_theta_tables(&P, k.theta, k.cxsth, k.sxnth);
@@ -2617,7 +2676,7 @@ extern "C" {
 int xbg_advance_resident(xbg_ctx* c, double t0, double dtbc, double Trep, double nextoutputtime, double endtime,
                          double* totaltime, double* dt) {
     if (!c || !totaltime || !dt || !(dtbc > 0.0)) return XBG_E_ARG;
-    double dtf, dtw, dtn;
+    double dtf = 0.0, dtw = 0.0, dtn = 0.0;
     int nb = 0;
     int rc = cfl_launch(c, 0, &nb);
     if (rc) return rc;
