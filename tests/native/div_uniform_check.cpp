@@ -5,6 +5,8 @@
 //   exhaustive <b>   every one of the 2^32 float bit patterns a
 //   random <n> <m>   n random divisors in (0, 2), m random a each
 //   tie              the counter-example for b >= 2 (an exact subnormal tie)
+//   pow2 <b>         b a power of two: a * (1/b) in float equals a / b for
+//                    every float a (the flow kernels' dx instantiations)
 // Prints "mismatches N" and exits non-zero on a mismatch (or, for "tie",
 // when the expected mismatch is absent).
 #include <cmath>
@@ -40,6 +42,27 @@ int main(int argc, char** argv) {
         long n = 0;
         for (long v : bad) n += v;
         std::printf("b=%a mismatches %ld\n", b, n);
+        return n != 0;
+    }
+    if (!std::strcmp(argv[1], "pow2")) {
+        const float b = std::strtof(argv[2], nullptr);
+        int ex = 0;
+        if (std::frexp(b, &ex) != 0.5f) return 2;
+        const float rb = std::ldexp(1.0f, 1 - ex);
+        const int T = 8;
+        std::vector<long> bad(T, 0);
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; t++)
+            th.emplace_back([&, t] {
+                for (uint64_t u = t; u < (1ull << 32); u += T) {
+                    float a = fb((uint32_t)u);
+                    if (!same(a * rb, a / b)) bad[t]++;
+                }
+            });
+        for (auto& x : th) x.join();
+        long n = 0;
+        for (long v : bad) n += v;
+        std::printf("pow2 b=%a mismatches %ld\n", b, n);
         return n != 0;
     }
     if (!std::strcmp(argv[1], "random")) {

@@ -51,6 +51,15 @@ def test_random_divisors(checker):
     assert rc == 0, out
 
 
+@pytest.mark.parametrize("dx", [0.5, 1.0, 2.0, 4.0, 8.0, 1024.0])
+def test_power_of_two_spacing_exhaustive(checker, dx):
+    """The flow kernels' instantiation for a power-of-two dx takes v * (1/dx)
+    for v / dx: the same real number, so the same rounding for all 2^32
+    numerators (subnormal results, zeros, infinities and NaN included)."""
+    rc, out = run(checker, "pow2", repr(dx))
+    assert rc == 0, out
+
+
 def test_divisor_two_or_more_can_tie(checker):
     """Why the kernel keeps the IEEE division for b >= 2."""
     rc, out = run(checker, "tie")
