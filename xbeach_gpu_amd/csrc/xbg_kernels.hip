@@ -1030,68 +1030,97 @@ __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
     else flow_nuh_body<false>(p, f, force, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
+// Values needed in registers at this point.  The momentum stencils read
+// their operands under data-dependent branches, where the compiler would
+// issue each load after the branch above it: one round trip to memory per
+// branch level.  The operands are loaded into locals and named here, ahead
+// of the first branch, so all loads are in flight together.  The asm emits
+// no instruction.
+XDEV void need(float a, float b, float c, float d) { asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d)); }
+XDEV void need(int a, int b, int c, int d) { asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d)); }
+
 // eulerustep (Flow_kernel.cu:1487-1544) at (x, y) with ududx_adv (752-823),
 // vdudy_adv (887-949), viscou1/viscou2 (1319-1397), wlevslopes (410-466).
 template <bool P2 = false>
 XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
-    size_t i = ix2(p, x, y);
-    if (x == 0) return G(f.uu)[i];
-    if (!(G(f.wet)[i] & 1)) return 0.0f;
-    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
-    const auto uu = G(f.uu);
-    const auto vv = G(f.vv);
+    const int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny), ymp = p1(ym, ny);
+    const size_t i = ix2(p, x, y), l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp),
+                 br = ix2(p, xp, ym), tr = ix2(p, xp, yp), bt = ix2(p, x, ymp), brt = ix2(p, xp, ymp);
+    // every operand, loaded ahead of the first branch (clamped in-domain cells)
+    const auto U = G(f.uu), V = G(f.vv), HU = G(f.hu), HV = G(f.hv), HH = G(f.hh), NU = G(f.nuh);
+    const float u_i = U[i], u_l = U[l], u_r = U[r], u_b = U[b], u_t = U[t];
+    const float hu_i = HU[i], hu_l = HU[l], hu_r = HU[r];
+    const float v_b = V[b], v_br = V[br], v_i = V[i], v_r = V[r], v_t = V[t];
+    const float hv_b = HV[b], hv_br = HV[br], hv_i = HV[i], hv_r = HV[r];
+    const float n_i = NU[i], n_r = NU[r], n_tr = NU[tr], n_t = NU[t], n_br = NU[br], n_b = NU[b];
+    const float h_i = HH[i], h_r = HH[r], h_t = HH[t], h_tr = HH[tr], h_b = HH[b], h_br = HH[br];
+    const float h_bt = HH[bt], h_brt = HH[brt];
+    const float zs_r = G(f.zs)[r], zs_i = G(f.zs)[i], sd_i = G(f.stdep)[i], ue_i = G(f.ueu)[i];
+    const float urms_i = G(f.urms)[i], vmag_i = G(f.vmageu)[i], fx_i = G(f.Fx)[i], vu_i = G(f.vu)[i];
+    const int w_i = G(f.wet)[i], w_t = G(f.wet)[t], w_b = G(f.wet)[b];
+    need(u_i, u_l, u_r, u_b);
+    need(u_t, hu_i, hu_l, hu_r);
+    need(v_b, v_br, v_i, v_r);
+    need(v_t, hv_b, hv_br, hv_i);
+    need(hv_r, n_i, n_r, n_tr);
+    need(n_t, n_br, n_b, h_i);
+    need(h_r, h_t, h_tr, h_b);
+    need(h_br, h_bt, h_brt, zs_r);
+    need(zs_i, sd_i, ue_i, urms_i);
+    need(vmag_i, fx_i, vu_i, u_i);
+    need(w_i, w_t, w_b, w_i);
+    if (x == 0) return u_i;
+    if (!(w_i & 1)) return 0.0f;
     const Dx<P2> dx = dx_of<P2>(p);
-    size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), br = ix2(p, xp, ym);
-    float hum = hum_at(p, f, x, y);
+    float hum = fmx(0.5f * (h_i + h_r), p.eps);  // udepthmomcont at (x, y)
     // ududx_adv
     float res = 0.0f;
-    float qin = 0.5f * (G(f.hu)[i] * uu[i] + G(f.hu)[l] * uu[l]);
-    float du = uu[i] - uu[l];
+    float qin = 0.5f * (hu_i * u_i + hu_l * u_l);
+    float du = u_i - u_l;
     if (qin > 0.0f) {
-        if (du > 0.5f) res = res + 0.5f * (uu[l] + uu[i]) * (uu[i] - uu[l]) / dx;
-        else res = res + qin / hum * (uu[i] - uu[l]) / dx;
+        if (du > 0.5f) res = res + 0.5f * (u_l + u_i) * (u_i - u_l) / dx;
+        else res = res + qin / hum * (u_i - u_l) / dx;
     }
-    qin = -0.5f * (G(f.hu)[i] * uu[i] + G(f.hu)[r] * uu[r]);
-    du = uu[r] - uu[i];
+    qin = -0.5f * (hu_i * u_i + hu_r * u_r);
+    du = u_r - u_i;
     if (qin > 0.0f) {
-        if (du > 0.5f) res = res - 0.5f * (uu[r] + uu[i]) * (uu[i] - uu[r]) / dx;
-        else res = res + qin / hum * (uu[i] - uu[r]) / dx;
+        if (du > 0.5f) res = res - 0.5f * (u_r + u_i) * (u_i - u_r) / dx;
+        else res = res + qin / hum * (u_i - u_r) / dx;
     }
     float ududx = res;
     // vdudy_adv
     res = 0.0f;
-    qin = 0.5f * (vv[b] * G(f.hv)[b] + vv[br] * G(f.hv)[br]);
-    float dv = vv[i] - vv[b];
+    qin = 0.5f * (v_b * hv_b + v_br * hv_br);
+    float dv = v_i - v_b;
     if (qin > 0.0f) {
-        if (dv > 0.5f) res = res + 0.5f * (vv[b] + vv[br]) * (uu[i] - uu[b]) / dx;
-        else res = res + qin / hum * (uu[i] - uu[b]) / dx;
+        if (dv > 0.5f) res = res + 0.5f * (v_b + v_br) * (u_i - u_b) / dx;
+        else res = res + qin / hum * (u_i - u_b) / dx;
     }
-    qin = -0.5f * (vv[i] * G(f.hv)[i] + vv[r] * G(f.hv)[r]);
-    dv = vv[t] - vv[i];
+    qin = -0.5f * (v_i * hv_i + v_r * hv_r);
+    dv = v_t - v_i;
     if (qin > 0.0f) {
-        if (dv > 0.5f) res = res - 0.5f * (vv[i] + vv[r]) * (uu[i] - uu[t]) / dx;
-        else res = res + qin / hum * (uu[i] - uu[t]) / dx;
+        if (dv > 0.5f) res = res - 0.5f * (v_i + v_r) * (u_i - u_t) / dx;
+        else res = res + qin / hum * (u_i - u_t) / dx;
     }
     float vdudy = res;
-    // viscou1 + viscou2
-    const auto nuh = G(f.nuh);
-    float dudx1 = nuh[r] * G(f.hh)[r] * (uu[r] - uu[i]) / dx;
-    float dudx2 = nuh[i] * G(f.hh)[i] * (uu[i] - uu[l]) / dx;
+    // viscou1 + viscou2; vdepthmomcont at (x, y), (xp, y), (x, ym), (xp, ym)
+    float dudx1 = n_r * h_r * (u_r - u_i) / dx;
+    float dudx2 = n_i * h_i * (u_i - u_l) / dx;
     float viscu = (1.0f / hum) * ((dudx1 - dudx2) / (dx));
-    float nuh1 = 0.25f * (nuh[i] + nuh[r] + nuh[ix2(p, xp, yp)] + nuh[t]);
-    float nuh2 = 0.25f * (nuh[i] + nuh[r] + nuh[br] + nuh[b]);
-    float dudy1 = nuh1 * 0.5f * (hvm_at(p, f, x, y) + hvm_at(p, f, xp, y)) * (uu[t] - uu[i]) / dx;
-    float dudy2 = nuh2 * 0.5f * (hvm_at(p, f, x, ym) + hvm_at(p, f, xp, ym)) * (uu[i] - uu[b]) / dx;
-    viscu = viscu + (1.0f / hum) * ((dudy1 - dudy2) / dx * wetu_at(p, f, x, yp) * wetu_at(p, f, x, ym));
+    float nuh1 = 0.25f * (n_i + n_r + n_tr + n_t);
+    float nuh2 = 0.25f * (n_i + n_r + n_br + n_b);
+    const float hvm0 = fmx(0.5f * (h_i + h_t), p.eps), hvm1 = fmx(0.5f * (h_r + h_tr), p.eps);
+    const float hvm2 = fmx(0.5f * (h_b + h_bt), p.eps), hvm3 = fmx(0.5f * (h_br + h_brt), p.eps);
+    float dudy1 = nuh1 * 0.5f * (hvm0 + hvm1) * (u_t - u_i) / dx;
+    float dudy2 = nuh2 * 0.5f * (hvm2 + hvm3) * (u_i - u_b) / dx;
+    viscu = viscu + (1.0f / hum) * ((dudy1 - dudy2) / dx * (float)(w_t & 1) * (float)(w_b & 1));
     // wlevslopes
-    float dzsdx = (G(f.zs)[r] - G(f.zs)[i]) / dx;
-    float taubx = cfm_of(p, G(f.stdep)[i]) * p.rho * G(f.ueu)[i] *
-                  sqrtf(1.3456f * G(f.urms)[i] * G(f.urms)[i] + G(f.vmageu)[i] * G(f.vmageu)[i]);
+    float dzsdx = (zs_r - zs_i) / dx;
+    float taubx = cfm_of(p, sd_i) * p.rho * ue_i * sqrtf(1.3456f * urms_i * urms_i + vmag_i * vmag_i);
     float hmin = 0.2f;
-    return uu[i] - s.dt * (ududx + vdudy - viscu + p.g * dzsdx + taubx / (p.rho * G(f.hu)[i]) -
-                           G(f.Fx)[i] / (p.rho * fmx(hum, hmin)) -
-                           1.25f * p.Cd * (float)s.signwu * s.windu * s.windu / (p.rho * hum) - p.fc * G(f.vu)[i]);
+    return u_i - s.dt * (ududx + vdudy - viscu + p.g * dzsdx + taubx / (p.rho * hu_i) - fx_i / (p.rho * fmx(hum, hmin)) -
+                         1.25f * p.Cd * (float)s.signwu * s.windu * s.windu / (p.rho * hum) - p.fc * vu_i);
 }
 
 // eulerustep + latbnd(uu) (Flow_kernel.cu:1778-1809): rows 0 / ny-1 copy
@@ -1126,62 +1155,92 @@ XDEV float viscov1_at(const KP& p, const Fld& f, int x, int y) {
 template <bool P2 = false>
 XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
-    size_t i = ix2(p, x, y);
-    if (x == 0 || y >= ny - 2) return G(f.vv)[i];
-    if (!((G(f.wet)[i] >> 1) & 1)) return 0.0f;
-    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
-    const auto un = G(f.uu_b);
-    const auto vv = G(f.vv);
+    const int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    const size_t i = ix2(p, x, y), l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp),
+                 tl = ix2(p, xm, yp), tr = ix2(p, xp, yp);
+    // every operand, loaded ahead of the first branch (clamped in-domain cells)
+    const auto UN = G(f.uu_b), V = G(f.vv), HU = G(f.hu), HV = G(f.hv), HH = G(f.hh), NU = G(f.nuh);
+    const float un_l = UN[l], un_tl = UN[tl], un_i = UN[i], un_t = UN[t], un_r = UN[r];
+    const float hu_l = HU[l], hu_tl = HU[tl], hu_i = HU[i], hu_t = HU[t];
+    const float v_i = V[i], v_l = V[l], v_r = V[r], v_b = V[b], v_t = V[t];
+    const float hv_i = HV[i], hv_b = HV[b], hv_t = HV[t];
+    const float h_i = HH[i], h_t = HH[t], h_r = HH[r], h_tr = HH[tr], h_l = HH[l], h_tl = HH[tl];
+    const float n_i = NU[i], n_r = NU[r], n_tr = NU[tr], n_t = NU[t], n_l = NU[l], n_tl = NU[tl];
+    const float zs_t = G(f.zs)[t], zs_i = G(f.zs)[i], sd_i = G(f.stdep)[i], ve_i = G(f.vev)[i];
+    const float urms_i = G(f.urms)[i], vmag_i = G(f.vmagev)[i], fy_i = G(f.Fy)[i], uv_i = G(f.uv)[i];
+    const int w_i = G(f.wet)[i], w_t = G(f.wet)[t], w_b = G(f.wet)[b], w_r = G(f.wet)[r], w_l = G(f.wet)[l];
+    need(un_l, un_tl, un_i, un_t);
+    need(un_r, hu_l, hu_tl, hu_i);
+    need(hu_t, v_i, v_l, v_r);
+    need(v_b, v_t, hv_i, hv_b);
+    need(hv_t, h_i, h_t, h_r);
+    need(h_tr, h_l, h_tl, n_i);
+    need(n_r, n_tr, n_t, n_l);
+    need(n_tl, zs_t, zs_i, sd_i);
+    need(ve_i, urms_i, vmag_i, fy_i);
+    need(uv_i, v_i, v_i, v_i);
+    need(w_i, w_t, w_b, w_r);
+    need(w_l, w_l, w_l, w_l);
+    if (x == 0 || y >= ny - 2) return v_i;
+    if (!((w_i >> 1) & 1)) return 0.0f;
     const Dx<P2> dx = dx_of<P2>(p);
-    size_t l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp), tl = ix2(p, xm, yp);
-    float hvm = hvm_at(p, f, x, y);
+    float hvm = fmx(0.5f * (h_i + h_t), p.eps);  // vdepthmomcont at (x, y)
     // udvdx_adv
     float res = 0.0f;
-    float qin = (float)(0.5 * (double)(un[l] * G(f.hu)[l] + un[tl] * G(f.hu)[tl]));
-    float du = un[i] - un[l];
+    float qin = (float)(0.5 * (double)(un_l * hu_l + un_tl * hu_tl));
+    float du = un_i - un_l;
     if (qin > 0.0f) {
-        if (du > 0.5f) res = res + 0.5f * (un[l] + un[tl]) * (vv[i] - vv[l]) / (dx);
-        else res = res + qin / hvm * (vv[i] - vv[l]) / (dx);
+        if (du > 0.5f) res = res + 0.5f * (un_l + un_tl) * (v_i - v_l) / (dx);
+        else res = res + qin / hvm * (v_i - v_l) / (dx);
     }
-    qin = (float)(-0.5 * (double)(un[i] * G(f.hu)[i] + un[t] * G(f.hu)[t]));
-    du = un[r] - un[i];
+    qin = (float)(-0.5 * (double)(un_i * hu_i + un_t * hu_t));
+    du = un_r - un_i;
     if (qin > 0.0f) {
-        if (du > 0.5f) res = res - 0.5f * (un[i] + un[r]) * (vv[i] - vv[r]) / (dx);
-        else res = res + qin / hvm * (vv[i] - vv[r]) / (dx);
+        if (du > 0.5f) res = res - 0.5f * (un_i + un_r) * (v_i - v_r) / (dx);
+        else res = res + qin / hvm * (v_i - v_r) / (dx);
     }
     float udvdx = res;
     // vdvdy_adv (interior: ix>0, 0<iy<ny-2)
     res = 0.0f;
     if (y > 0) {
-        qin = 0.5f * (vv[i] * G(f.hv)[i] + vv[b] * G(f.hv)[b]);
-        float dv = vv[i] - vv[b];
+        qin = 0.5f * (v_i * hv_i + v_b * hv_b);
+        float dv = v_i - v_b;
         if (qin > 0.0f) {
-            if (dv > 0.5f) res = res + 0.5f * (vv[b] - vv[i]) * (vv[i] - vv[b]) / (dx);
-            else res = res + qin / hvm * (vv[i] - vv[b]) / (dx);
+            if (dv > 0.5f) res = res + 0.5f * (v_b - v_i) * (v_i - v_b) / (dx);
+            else res = res + qin / hvm * (v_i - v_b) / (dx);
         }
-        qin = -0.5f * (G(f.hv)[i] * vv[i] + G(f.hv)[t] * vv[t]);
-        dv = vv[t] - vv[i];
+        qin = -0.5f * (hv_i * v_i + hv_t * v_t);
+        dv = v_t - v_i;
         if (qin > 0.0f) {
-            if (dv > 0.5f) res = res - 0.5f * (vv[t] + vv[i]) * (vv[i] - vv[t]) / (dx);
-            else res = res + qin / hvm * (vv[i] - vv[t]) / (dx);
+            if (dv > 0.5f) res = res - 0.5f * (v_t + v_i) * (v_i - v_t) / (dx);
+            else res = res + qin / hvm * (v_i - v_t) / (dx);
         }
     }
     float vdvdy = res;
-    // viscov1 (row 0 takes row 1 through viscovbnd) + viscov2
-    float viscv = viscov1_at<P2>(p, f, x, y == 0 ? p1(0, ny) : y);
-    const auto nuh = G(f.nuh);
-    float nuh1 = 0.25f * (nuh[i] + nuh[r] + nuh[ix2(p, xp, yp)] + nuh[t]);
-    float nuh2 = 0.25f * (nuh[i] + nuh[l] + nuh[tl] + nuh[t]);
-    float dvdx1 = nuh1 * 0.5f * (hum_at(p, f, x, y) + hum_at(p, f, x, yp)) * (vv[r] - vv[i]) / dx;
-    float dvdx2 = nuh2 * 0.5f * (hum_at(p, f, xm, y) + hum_at(p, f, xm, yp)) * (vv[i] - vv[l]) / dx;
-    viscv = viscv + (1.0f / hvm) * ((dvdx1 - dvdx2) / (dx) * wetv_at(p, f, xp, y) * wetv_at(p, f, xm, y));
-    float dzsdy = (G(f.zs)[t] - G(f.zs)[i]) / dx;
-    float tauby = cfm_of(p, G(f.stdep)[i]) * p.rho * G(f.vev)[i] *
-                  sqrtf(1.3456f * G(f.urms)[i] * G(f.urms)[i] + G(f.vmagev)[i] * G(f.vmagev)[i]);
+    // viscov1 (row 0 takes row 1 through viscovbnd: only y = 0, which reads
+    // its operands on its own) + viscov2
+    float viscv;
+    if (y == 0) {
+        viscv = viscov1_at<P2>(p, f, x, p1(0, ny));
+    } else {
+        float dvdy1 = n_t * h_t * (v_t - v_i) / dx;
+        float dvdy2 = n_i * h_i * (v_i - v_b) / dx;
+        viscv = (1.0f / hvm) * (dvdy1 - dvdy2) / dx * (float)((w_t >> 1) & 1) * (float)((w_b >> 1) & 1);
+    }
+    float nuh1 = 0.25f * (n_i + n_r + n_tr + n_t);
+    float nuh2 = 0.25f * (n_i + n_l + n_tl + n_t);
+    // udepthmomcont at (x, y), (x, yp), (xm, y), (xm, yp); x >= 1 here, so
+    // the +x neighbour of xm is x
+    const float hum0 = fmx(0.5f * (h_i + h_r), p.eps), hum1 = fmx(0.5f * (h_t + h_tr), p.eps);
+    const float hum2 = fmx(0.5f * (h_l + h_i), p.eps), hum3 = fmx(0.5f * (h_tl + h_t), p.eps);
+    float dvdx1 = nuh1 * 0.5f * (hum0 + hum1) * (v_r - v_i) / dx;
+    float dvdx2 = nuh2 * 0.5f * (hum2 + hum3) * (v_i - v_l) / dx;
+    viscv = viscv + (1.0f / hvm) * ((dvdx1 - dvdx2) / (dx) * (float)((w_r >> 1) & 1) * (float)((w_l >> 1) & 1));
+    float dzsdy = (zs_t - zs_i) / dx;
+    float tauby = cfm_of(p, sd_i) * p.rho * ve_i * sqrtf(1.3456f * urms_i * urms_i + vmag_i * vmag_i);
     float hmin = 0.2f;
-    return vv[i] - s.dt * (udvdx + vdvdy - viscv + p.g * dzsdy + tauby / (p.rho * G(f.hv)[i]) -
-                           G(f.Fy)[i] / (p.rho * fmx(hvm, hmin)) + p.fc * G(f.uv)[i] -
-                           1.25f * p.Cd * (float)s.signwv * s.windv * s.windv / (p.rho * hvm));
+    return v_i - s.dt * (udvdx + vdvdy - viscv + p.g * dzsdy + tauby / (p.rho * hv_i) - fy_i / (p.rho * fmx(hvm, hmin)) +
+                         p.fc * uv_i - 1.25f * p.Cd * (float)s.signwv * s.windv * s.windv / (p.rho * hvm));
 }
 
 // eulervstep + vvlatbnd (Flow_kernel.cu:1812-1844): rows 0 / ny-2 copy the
