@@ -925,50 +925,89 @@ __global__ __launch_bounds__(256) void k_wave_rows(KP p, SP s, Fld f) {
     wave_rows_body<NT, CUR>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
-// wavforceCPU (wave_CPU.cpp:1159-1223) at (x, y)
-template <bool P2 = false>
-XDEV void wavforce_at(const KP& p, const Fld& f, int x, int y, float& Fxv, float& Fyv) {
+// Values needed in registers at this point.  Stencils that read their
+// operands under data-dependent branches, or after stores the compiler
+// cannot prove independent, would issue each load after the branch or store
+// above it: one round trip to memory per level.  Their operands are loaded
+// into locals and named here, ahead of the first branch and store, so all
+// loads are in flight together.  The asm emits no instruction.
+XDEV void need(float a, float b, float c, float d) { asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d)); }
+XDEV void need(int a, int b, int c, int d) { asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d)); }
+
+// Operands of wavforceCPU (wave_CPU.cpp:1159-1223) at (x, y).
+struct WfOps {
+    float Sxxi, Sxxr, Sxyt, Sxytr, Sxyb, Sxybr, Syyi, Syyt, Sxyr, Sxyl, Sxytl;
+};
+XDEV WfOps wf_load(const KP& p, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
-    const Dx<P2> dx = dx_of<P2>(p), dx2 = dx_of<P2>(p, 2.0f);
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     auto I = [nx](int a, int b) { return (size_t)a + (size_t)b * nx; };
-    float Sxxi = G(f.Sxx)[I(x, y)], Sxxr = G(f.Sxx)[I(xp, y)];
-    float Sxyt = G(f.Sxy)[I(x, yp)], Sxytr = G(f.Sxy)[I(xp, yp)];
-    float Sxyb = G(f.Sxy)[I(x, ym)], Sxybr = G(f.Sxy)[I(xp, ym)];
-    float Syyi = G(f.Syy)[I(x, y)], Syyt = G(f.Syy)[I(x, yp)];
-    float Sxyr = G(f.Sxy)[I(xp, y)], Sxyl = G(f.Sxy)[I(xm, y)], Sxytl = G(f.Sxy)[I(xm, yp)];
-    Fxv = -1 * (Sxxr - Sxxi) / dx - 0.5f * (Sxyt + Sxytr - Sxyb - Sxybr) / dx2;
-    Fyv = -1 * (Syyt - Syyi) / dx - 0.5f * (Sxyr + Sxytr - Sxyl - Sxytl) / dx2;
+    WfOps o;
+    o.Sxxi = G(f.Sxx)[I(x, y)];
+    o.Sxxr = G(f.Sxx)[I(xp, y)];
+    o.Sxyt = G(f.Sxy)[I(x, yp)];
+    o.Sxytr = G(f.Sxy)[I(xp, yp)];
+    o.Sxyb = G(f.Sxy)[I(x, ym)];
+    o.Sxybr = G(f.Sxy)[I(xp, ym)];
+    o.Syyi = G(f.Syy)[I(x, y)];
+    o.Syyt = G(f.Syy)[I(x, yp)];
+    o.Sxyr = G(f.Sxy)[I(xp, y)];
+    o.Sxyl = G(f.Sxy)[I(xm, y)];
+    o.Sxytl = G(f.Sxy)[I(xm, yp)];
+    return o;
+}
+XDEV void need(const WfOps& o, float e) {
+    need(o.Sxxi, o.Sxxr, o.Sxyt, o.Sxytr);
+    need(o.Sxyb, o.Sxybr, o.Syyi, o.Syyt);
+    need(o.Sxyr, o.Sxyl, o.Sxytl, e);
+}
+// wavforceCPU on loaded operands
+template <bool P2>
+XDEV void wf_forces(const KP& p, const WfOps& o, float& Fxv, float& Fyv) {
+    const Dx<P2> dx = dx_of<P2>(p), dx2 = dx_of<P2>(p, 2.0f);
+    Fxv = -1 * (o.Sxxr - o.Sxxi) / dx - 0.5f * (o.Sxyt + o.Sxytr - o.Sxyb - o.Sxybr) / dx2;
+    Fyv = -1 * (o.Syyt - o.Syyi) / dx - 0.5f * (o.Sxyr + o.Sxytr - o.Sxyl - o.Sxytl) / dx2;
 }
 
 // wavforce + twodimbndnoix(Fx) + twodimbnd(Fy) + twodimbnd(ust)
 // (Wavestep.cu:1099-1125; CPU twins wave_CPU.cpp:1224-1331: the neighbour
-// value read is the pre-update one and the ix==0 rule wins).
-template <bool P2 = false>
-XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
-    const int nx = p.nx, ny = p.ny;
+// value read is the pre-update one and the ix==0 rule wins) from the forces
+// and ust_raw at row ys, the cell's own row or its lateral source row.
+template <bool P2>
+XDEV void wave_force_store(const KP& p, const Fld& f, int x, int y, int ys, float wet, float Fxv, float Fyv,
+                           float ust_ys) {
+    const int nx = p.nx;
     size_t i = (size_t)x + (size_t)y * nx;
-    float wet = G(f.hh)[i] > p.eps ? 1.0f : 0.0f;
-    int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
-    float Fxv, Fyv;
-    wavforce_at<P2>(p, f, x, ys, Fxv, Fyv);
     float fx = ys != y ? Fxv * wet : Fxv;
     float fy, ust;
     if (x == 0) {
         float a, b;
-        wavforce_at<P2>(p, f, p1(0, nx), y, a, b);
+        wf_forces<P2>(p, wf_load(p, f, p1(0, nx), y), a, b);
         fy = b * wet;
         ust = G(f.ust_raw)[(size_t)p1(0, nx) + (size_t)y * nx] * wet;
     } else if (ys != y) {
         fy = Fyv * wet;
-        ust = G(f.ust_raw)[(size_t)x + (size_t)ys * nx] * wet;
+        ust = ust_ys * wet;
     } else {
         fy = Fyv;
-        ust = G(f.ust_raw)[i];
+        ust = ust_ys;
     }
     G(f.Fx)[i] = fx;
     G(f.Fy)[i] = fy;
     G(f.ust)[i] = ust;
+}
+template <bool P2 = false>
+XDEV void wave_force_cell(const KP& p, const Fld& f, int x, int y) {
+    const int nx = p.nx, ny = p.ny;
+    const int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
+    const WfOps w = wf_load(p, f, x, ys);
+    const float ust_ys = G(f.ust_raw)[(size_t)x + (size_t)ys * nx];
+    const float h = G(f.hh)[(size_t)x + (size_t)y * nx];
+    need(w, ust_ys);
+    need(h, h, h, h);
+    float Fxv, Fyv;
+    wf_forces<P2>(p, w, Fxv, Fyv);
+    wave_force_store<P2>(p, f, x, y, ys, h > p.eps ? 1.0f : 0.0f, Fxv, Fyv, ust_ys);
 }
 __global__ __launch_bounds__(256) void k_wave_force(KP p, Fld f) {
     int x = gx(BI{(int)blockIdx.x, (int)blockIdx.y}), y = gy(BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -992,33 +1031,53 @@ XDEV float wetu_at(const KP& p, const Fld& f, int x, int y) { return (float)(G(f
 XDEV float wetv_at(const KP& p, const Fld& f, int x, int y) { return (float)((G(f.wet)[ix2(p, x, y)] >> 1) & 1); }
 
 // smago (Flow_kernel.cu:1282-1317) and the wet masks of
-// udepthmomcont/vdepthmomcont from the previous step's hu/hv.  With force
-// set, also the wave forces of the step (k_wave_force's work: neither reads
-// what the other writes), one launch where a whole step runs both.
-template <bool P2 = false>
-XDEV void flow_nuh_body(const KP& p, const Fld& f, int force, BI bi) {
+// udepthmomcont/vdepthmomcont from the previous step's hu/hv.  With FORCE,
+// also the wave forces of the step (k_wave_force's work: neither reads what
+// the other writes), one launch where a whole step runs both.  Every
+// operand is loaded ahead of the branches and the stores.
+template <bool P2 = false, bool FORCE = true>
+XDEV void flow_nuh_body(const KP& p, const Fld& f, BI bi) {
     int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
-    if (force) wave_force_cell<P2>(p, f, x, y);
-    const Dx<P2> dx = dx_of<P2>(p);
     const int nx = p.nx, ny = p.ny;
-    int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
-    size_t i = ix2(p, x, y);
-    float h = G(f.hh)[i];
-    float humi = 0.5f * (h + G(f.hh)[ix2(p, xp, y)]);
-    float hvmi = 0.5f * (h + G(f.hh)[ix2(p, x, yp)]);
+    const int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
+    const size_t i = ix2(p, x, y), l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp),
+                 lt = ix2(p, xm, yp), lb = ix2(p, xm, ym), rb = ix2(p, xp, ym);
+    const int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
+    WfOps w{};
+    float ust_ys = 0.0f;
+    if (FORCE) {
+        w = wf_load(p, f, x, ys);
+        ust_ys = G(f.ust_raw)[(size_t)x + (size_t)ys * nx];
+    }
+    const auto HH = G(f.hh), U = G(f.uu), V = G(f.vv);
+    const float h = HH[i], h_r = HH[r], h_t = HH[t], hu_i = G(f.hu)[i], hv_i = G(f.hv)[i], dr_i = G(f.DR)[i];
+    const float u_i = U[i], u_l = U[l], u_t = U[t], u_b = U[b], u_lt = U[lt], u_lb = U[lb];
+    const float v_i = V[i], v_b = V[b], v_r = V[r], v_l = V[l], v_rb = V[rb], v_lb = V[lb];
+    if (FORCE) need(w, ust_ys);
+    need(h, h_r, h_t, hu_i);
+    need(hv_i, dr_i, u_i, u_l);
+    need(u_t, u_b, u_lt, u_lb);
+    need(v_i, v_b, v_r, v_l);
+    need(v_rb, v_lb, h, h);
+    if (FORCE) {
+        float Fxv, Fyv;
+        wf_forces<P2>(p, w, Fxv, Fyv);
+        wave_force_store<P2>(p, f, x, y, ys, h > p.eps ? 1.0f : 0.0f, Fxv, Fyv, ust_ys);
+    }
+    const Dx<P2> dx = dx_of<P2>(p);
+    float humi = 0.5f * (h + h_r);
+    float hvmi = 0.5f * (h + h_t);
     uint8_t m = 0;
-    if (G(f.hu)[i] > p.eps && humi > p.eps) m |= 1;
-    if (G(f.hv)[i] > p.eps && hvmi > p.eps) m |= 2;
+    if (hu_i > p.eps && humi > p.eps) m |= 1;
+    if (hv_i > p.eps && hvmi > p.eps) m |= 2;
     G(f.wet)[i] = m;
-    float turb = p.nuhfac * h * xm::powf(G(f.DR)[i] / p.rho, 1.0f / 3.0f);
+    float turb = p.nuhfac * h * xm::powf(dr_i / p.rho, 1.0f / 3.0f);
     if (p.usesmago == 1) {
-        const auto uu = G(f.uu);
-        const auto vv = G(f.vv);
-        float dudx = (uu[i] - uu[ix2(p, xm, y)]) / dx;
-        float dudy = 0.50f * (uu[ix2(p, x, yp)] - uu[ix2(p, x, ym)] + uu[ix2(p, xm, yp)] - uu[ix2(p, xm, ym)]) / dx;
-        float dvdy = (vv[i] - vv[ix2(p, x, ym)]) / dx;
-        float dvdx = 0.50f * (vv[ix2(p, xp, y)] - vv[ix2(p, xm, y)] + vv[ix2(p, xp, ym)] - vv[ix2(p, xm, ym)]) / dx;
+        float dudx = (u_i - u_l) / dx;
+        float dudy = 0.50f * (u_t - u_b + u_lt - u_lb) / dx;
+        float dvdy = (v_i - v_b) / dx;
+        float dvdx = 0.50f * (v_r - v_l + v_rb - v_lb) / dx;
         float tau = sqrtf(2.0f * dudx * dudx + 2.0f * dvdy * dvdy + xm::powf(dudy + dvdx, 2.0f));
         G(f.nuh)[i] = fmx(p.nuh * p.nuh * p.dx * p.dx * tau, turb);
     } else {
@@ -1026,18 +1085,15 @@ XDEV void flow_nuh_body(const KP& p, const Fld& f, int force, BI bi) {
     }
 }
 __global__ __launch_bounds__(256) void k_flow_nuh(KP p, Fld f, int force) {
-    if (p.dx_p2) flow_nuh_body<true>(p, f, force, BI{(int)blockIdx.x, (int)blockIdx.y});
-    else flow_nuh_body<false>(p, f, force, BI{(int)blockIdx.x, (int)blockIdx.y});
+    const BI bi{(int)blockIdx.x, (int)blockIdx.y};
+    if (p.dx_p2) {
+        if (force) flow_nuh_body<true, true>(p, f, bi);
+        else flow_nuh_body<true, false>(p, f, bi);
+    } else {
+        if (force) flow_nuh_body<false, true>(p, f, bi);
+        else flow_nuh_body<false, false>(p, f, bi);
+    }
 }
-
-// Values needed in registers at this point.  The momentum stencils read
-// their operands under data-dependent branches, where the compiler would
-// issue each load after the branch above it: one round trip to memory per
-// branch level.  The operands are loaded into locals and named here, ahead
-// of the first branch, so all loads are in flight together.  The asm emits
-// no instruction.
-XDEV void need(float a, float b, float c, float d) { asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d)); }
-XDEV void need(int a, int b, int c, int d) { asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d)); }
 
 // eulerustep (Flow_kernel.cu:1487-1544) at (x, y) with ududx_adv (752-823),
 // vdudy_adv (887-949), viscou1/viscou2 (1319-1397), wlevslopes (410-466).
@@ -1410,21 +1466,55 @@ __global__ __launch_bounds__(BX * CY) void k_flow_uvcorr(KP p, SP s, Fld f) {
     else flow_uvcorr_body<false>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
-// CalcQFlow (Flow_kernel.cu:1917-1958) + continuity (1626-1681) at (x, y):
-// the water level after continuity; *dzdt_out gets dzsdt.
+// CalcQFlow (Flow_kernel.cu:1917-1958) + continuity (1626-1681) at (x, y)
+// on loaded operands: the water level after continuity; *dzdt_out gets
+// dzsdt.
+struct CtOps {
+    float zs_i, u_i, hu_i, u_l, hu_l, v_i, hv_i, v_b, hv_b;
+};
+XDEV CtOps ct_load(const KP& p, const Fld& f, int x, int y) {
+    size_t i = ix2(p, x, y), l = ix2(p, m1(x), y), b = ix2(p, x, m1(y));
+    return CtOps{G(f.zs)[i], G(f.uu)[i], G(f.hu)[i], G(f.uu)[l], G(f.hu)[l], G(f.vv)[i], G(f.hv)[i], G(f.vv)[b], G(f.hv)[b]};
+}
+XDEV void need(const CtOps& o) {
+    need(o.zs_i, o.u_i, o.hu_i, o.u_l);
+    need(o.hu_l, o.v_i, o.hv_i, o.v_b);
+    need(o.hv_b, o.zs_i, o.zs_i, o.zs_i);
+}
 template <bool P2 = false>
-XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, float* dzdt_out) {
-    size_t i = ix2(p, x, y);
-    float z = G(f.zs)[i];
+XDEV float continuity_ops(const KP& p, const SP& s, const CtOps& o, int x, float* dzdt_out) {
+    float z = o.zs_i;
     if (x > 0) {
-        size_t l = ix2(p, x - 1, y), b = ix2(p, x, m1(y));
-        float qxi = G(f.uu)[i] * G(f.hu)[i], qxl = G(f.uu)[l] * G(f.hu)[l];
-        float qyi = G(f.vv)[i] * G(f.hv)[i], qyb = G(f.vv)[b] * G(f.hv)[b];
+        float qxi = o.u_i * o.hu_i, qxl = o.u_l * o.hu_l;
+        float qyi = o.v_i * o.hv_i, qyb = o.v_b * o.hv_b;
         float dzdt = (qxl - qxi + qyb - qyi) / dx_of<P2>(p);
         *dzdt_out = dzdt;
         z = z + dzdt * s.dt;
     }
     return z;
+}
+
+// Operands of the level correction fluxes (wrkzs2Ocorr) at (x, y): u, v and
+// zb, zsold at the -1..+2 neighbours along x and y.
+struct WzOps {
+    float u, v, zb_xm, zb_i, zb_xp, zb_xp2, zb_ym, zb_yp, zb_yp2, zo_i, zo_xm, zo_xp, zo_xp2, zo_ym, zo_yp, zo_yp2;
+};
+XDEV WzOps wz_load(const KP& p, const Fld& f, int x, int y) {
+    const int nx = p.nx, ny = p.ny;
+    const int xm = m1(x), xp = p1(x, nx), xp2 = p2(x, nx), ym = m1(y), yp = p1(y, ny), yp2 = p2(y, ny);
+    const size_t i = ix2(p, x, y);
+    const auto zb = G(f.zb);
+    const auto zo = G(f.zsold);
+    return WzOps{G(f.uu)[i],         G(f.vv)[i],         zb[ix2(p, xm, y)],  zb[i],
+                 zb[ix2(p, xp, y)],  zb[ix2(p, xp2, y)], zb[ix2(p, x, ym)],  zb[ix2(p, x, yp)],
+                 zb[ix2(p, x, yp2)], zo[i],              zo[ix2(p, xm, y)],  zo[ix2(p, xp, y)],
+                 zo[ix2(p, xp2, y)], zo[ix2(p, x, ym)],  zo[ix2(p, x, yp)],  zo[ix2(p, x, yp2)]};
+}
+XDEV void need(const WzOps& o) {
+    need(o.u, o.v, o.zb_xm, o.zb_i);
+    need(o.zb_xp, o.zb_xp2, o.zb_ym, o.zb_yp);
+    need(o.zb_yp2, o.zo_i, o.zo_xm, o.zo_xp);
+    need(o.zo_xp2, o.zo_ym, o.zo_yp, o.zo_yp2);
 }
 
 // Continuity, wrkzs2Ocorr (Flow_kernel.cu:2130-2208), zs2Ocorr (2284-2319),
@@ -1436,9 +1526,12 @@ XDEV float continuity_at(const KP& p, const SP& s, const Fld& f, int x, int y, f
 //      the fluxes qx/qy with their corrections, and hh.
 // Halo slots hold the values at the clamped coordinates, which equal the
 // mminus/pplus/mminus2/pplus2 neighbours.  The new level goes to the other
-// zs buffer: other tiles still read the old one in step 1.
+// zs buffer: other tiles still read the old one in step 1.  The global
+// operands of all three steps are loaded before the first branch and store
+// (none of them is written by this pass): one round trip to memory.
 constexpr int ZW = BX + 4, ZH = CY + 5;  // level tile: x0-2 .. x0+BX+1, y0-3 .. y0+CY+1
 constexpr int FW = BX + 1, FH = CY + 2;  // flux tile:  x0-1 .. x0+BX-1, y0-2 .. y0+CY-1
+constexpr int ZSLOT = (ZW * ZH + BX * CY - 1) / (BX * CY), FSLOT = (FW * FH + BX * CY - 1) / (BX * CY);
 struct ZsallLds {
     float z[ZH][ZW];
     float w1[FH][FW], w2[FH][FW];
@@ -1455,52 +1548,69 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
     const int x0 = bi.x * BX, y0 = bi.y * CY;
-    for (int l = tid; l < ZW * ZH; l += BX * CY) {
-        int r = l / ZW, c = l - r * ZW;
-        int ux = x0 - 2 + c, uy = y0 - 3 + r;
-        int gxx = min(max(ux, 0), nx - 1), gyy = min(max(uy, 0), ny - 1);
-        float dzdt = 0.0f;
-        float z = continuity_at<P2>(p, s, f, gxx, gyy, &dzdt);
-        zt[r][c] = z;
-        if (ux == gxx && uy == gyy && ux >= x0 && ux < x0 + BX && uy >= y0 && uy < y0 + CY && gxx > 0)
-            G(f.dzsdt)[ix2(p, gxx, gyy)] = dzdt;
+    // operands of steps 1-3 at clamped coordinates
+    CtOps co[ZSLOT];
+#pragma unroll
+    for (int k = 0; k < ZSLOT; k++) {
+        int l = min(tid + k * BX * CY, ZW * ZH - 1), r = l / ZW, c = l - r * ZW;
+        co[k] = ct_load(p, f, min(max(x0 - 2 + c, 0), nx - 1), min(max(y0 - 3 + r, 0), ny - 1));
+    }
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    const size_t ic = ix2(p, min(x, nx - 1), min(y, ny - 1));
+    const float u_c = G(f.uu)[ic], hu_c = G(f.hu)[ic], v_c = G(f.vv)[ic], hv_c = G(f.hv)[ic], zb_c = G(f.zb)[ic];
+#pragma unroll
+    for (int k = 0; k < ZSLOT; k++) need(co[k]);
+    need(u_c, hu_c, v_c, hv_c);
+    need(zb_c, zb_c, zb_c, zb_c);
+#pragma unroll
+    for (int k = 0; k < ZSLOT; k++) {
+        int l = tid + k * BX * CY;
+        if (l < ZW * ZH) {
+            int r = l / ZW, c = l - r * ZW;
+            int ux = x0 - 2 + c, uy = y0 - 3 + r;
+            int gxx = min(max(ux, 0), nx - 1), gyy = min(max(uy, 0), ny - 1);
+            float dzdt = 0.0f;
+            float z = continuity_ops<P2>(p, s, co[k], gxx, &dzdt);
+            zt[r][c] = z;
+            if (ux == gxx && uy == gyy && ux >= x0 && ux < x0 + BX && uy >= y0 && uy < y0 + CY && gxx > 0)
+                G(f.dzsdt)[ix2(p, gxx, gyy)] = dzdt;
+        }
     }
     __syncthreads();
     const float two_eps = p.eps * 2.0f;
-    for (int l = tid; l < FW * FH; l += BX * CY) {
-        int r = l / FW, c = l - r * FW;
-        int wx = x0 - 1 + c, wy = y0 - 2 + r;
-        float w1 = 0.0f, w2 = 0.0f;
-        if (wx >= 0 && wy >= 0 && wx < nx && wy < ny) {
-            // level at unclamped offsets (dx, dy) from the flux cell
-            auto Z = [&](int dx_, int dy_) { return zt[wy + dy_ - (y0 - 3)][wx + dx_ - (x0 - 2)]; };
-            int xm = m1(wx), xp = p1(wx, nx), xp2 = p2(wx, nx);
-            int ym = m1(wy), yp = p1(wy, ny), yp2 = p2(wy, ny);
-            auto I = [&](int a, int b) { return ix2(p, a, b); };
-            const auto zo = G(f.zsold);
-            const auto zb = G(f.zb);
-            size_t i = I(wx, wy);
-            float u = G(f.uu)[i], v = G(f.vv)[i];
-            float minzs = fmn(fmn(fmn(Z(-1, 0), Z(0, 0)), Z(1, 0)), Z(2, 0));
-            float minzb = fmn(fmn(fmn(zb[I(xm, wy)], zb[i]), zb[I(xp, wy)]), zb[I(xp2, wy)]);
-            if (wx > 0 && minzs + minzb > two_eps) {
-                if (u > 0.0f) w1 = u * 0.5f * minmod(Z(1, 0) - zo[i], Z(0, 0) - zo[I(xm, wy)]);
-                if (u < 0.0f) w1 = -1.0f * u * 0.5f * minmod(zo[I(xp2, wy)] - Z(1, 0), zo[I(xp, wy)] - Z(0, 0));
+#pragma unroll
+    for (int k = 0; k < FSLOT; k++) {
+        int l = tid + k * BX * CY;
+        if (l < FW * FH) {
+            int r = l / FW, c = l - r * FW;
+            int wx = x0 - 1 + c, wy = y0 - 2 + r;
+            float w1 = 0.0f, w2 = 0.0f;
+            if (wx >= 0 && wy >= 0 && wx < nx && wy < ny) {
+                // level at unclamped offsets (dx, dy) from the flux cell
+                auto Z = [&](int dx_, int dy_) { return zt[wy + dy_ - (y0 - 3)][wx + dx_ - (x0 - 2)]; };
+                const WzOps o = wz_load(p, f, wx, wy);
+                need(o);
+                float u = o.u, v = o.v;
+                float minzs = fmn(fmn(fmn(Z(-1, 0), Z(0, 0)), Z(1, 0)), Z(2, 0));
+                float minzb = fmn(fmn(fmn(o.zb_xm, o.zb_i), o.zb_xp), o.zb_xp2);
+                if (wx > 0 && minzs + minzb > two_eps) {
+                    if (u > 0.0f) w1 = u * 0.5f * minmod(Z(1, 0) - o.zo_i, Z(0, 0) - o.zo_xm);
+                    if (u < 0.0f) w1 = -1.0f * u * 0.5f * minmod(o.zo_xp2 - Z(1, 0), o.zo_xp - Z(0, 0));
+                }
+                minzs = fmn(fmn(fmn(Z(0, -1), Z(0, 0)), Z(0, 1)), Z(0, 2));
+                minzb = fmn(fmn(fmn(o.zb_ym, o.zb_i), o.zb_yp), o.zb_yp2);
+                if (wx > 0 && wy > 0 && minzs + minzb > two_eps) {
+                    if (v > 0.0f) w2 = v * 0.5f * minmod(Z(0, 1) - o.zo_i, Z(0, 0) - o.zo_ym);
+                    if (v < 0.0f) w2 = v * 0.5f * minmod(o.zo_yp2 - Z(0, 1), o.zo_yp - Z(0, 0));
+                }
             }
-            minzs = fmn(fmn(fmn(Z(0, -1), Z(0, 0)), Z(0, 1)), Z(0, 2));
-            minzb = fmn(fmn(fmn(zb[I(wx, ym)], zb[i]), zb[I(wx, yp)]), zb[I(wx, yp2)]);
-            if (wx > 0 && wy > 0 && minzs + minzb > two_eps) {
-                if (v > 0.0f) w2 = v * 0.5f * minmod(Z(0, 1) - zo[i], Z(0, 0) - zo[I(wx, ym)]);
-                if (v < 0.0f) w2 = v * 0.5f * minmod(zo[I(wx, yp2)] - Z(0, 1), zo[I(wx, yp)] - Z(0, 0));
-            }
+            w1t[r][c] = w1;
+            w2t[r][c] = w2;
         }
-        w1t[r][c] = w1;
-        w2t[r][c] = w2;
     }
     __syncthreads();
-    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     if (x >= nx || y >= ny) return;
-    size_t i = ix2(p, x, y);
+    size_t i = ic;
     int ys = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
     const int cz = x - (x0 - 2), cf = x - (x0 - 1);
     float z = zt[ys - (y0 - 3)][cz];
@@ -1509,7 +1619,7 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         const Dx<P2> dx = dx_of<P2>(p);
         z = z - s.dt * ((w1t[rf][cf] - w1t[rf][cf - 1]) / dx + (w2t[rf][cf] - w2t[rf - 1][cf]) / dx);
     }
-    float qxv = G(f.uu)[i] * G(f.hu)[i], qyv = G(f.vv)[i] * G(f.hv)[i];
+    float qxv = u_c * hu_c, qyv = v_c * hv_c;
     if (x > 0 && y > 0) {
         int rf = y - (y0 - 2);
         qxv = qxv + w1t[rf][cf];
@@ -1518,7 +1628,7 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     G(f.zs_out)[i] = z;
     G(f.qx)[i] = qxv;
     G(f.qy)[i] = qyv;
-    G(f.hh)[i] = fmx(G(f.zb)[i] + z, p.eps);
+    G(f.hh)[i] = fmx(zb_c + z, p.eps);
 }
 __global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
     if (p.dx_p2) flow_zsall_body<true>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1824,39 +1934,52 @@ __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) 
     sed_conc_body(p, s, f, morph, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
-// avalanching (Sediment_kernel.cu:1020-1137) at (x, y): slope-limited
-// exchange with the +x and +y neighbours.
-template <bool P2 = false>
-XDEV void aval_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& bxo, float& byo) {
+// Operands of the avalanching exchange at (x, y) with its +x and +y
+// neighbours.
+struct AvOps {
+    float h_i, zb_i, zb_r, zb_t, sd_i, sd_r, sd_t;
+};
+XDEV AvOps av_load(const KP& p, const Fld& f, int x, int y) {
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
     const auto zb = G(f.zb_b);
     const auto sd = G(f.stdep_b);
-    float dzmax = G(f.hh_b)[i] > p.eps ? p.wetdzmax : p.drydzmax;
+    return AvOps{G(f.hh_b)[i], zb[i], zb[r], zb[t], sd[i], sd[r], sd[t]};
+}
+XDEV void need(const AvOps& o) {
+    need(o.h_i, o.zb_i, o.zb_r, o.zb_t);
+    need(o.sd_i, o.sd_r, o.sd_t, o.h_i);
+}
+
+// avalanching (Sediment_kernel.cu:1020-1137) on loaded operands:
+// slope-limited exchange with the +x and +y neighbours.
+template <bool P2 = false>
+XDEV void aval_ops(const KP& p, const SP& s, const AvOps& o, float& bxo, float& byo) {
+    float dzmax = o.h_i > p.eps ? p.wetdzmax : p.drydzmax;
     float maxchg = p.maxslpchg, dt = s.dt, dx = p.dx;
     const Dx<P2> ddx = dx_of<P2>(p);
     float bx = 0.0f, by = 0.0f;
-    float sxv = (zb[r] - zb[i]) / ddx;
+    float sxv = (o.zb_r - o.zb_i) / ddx;
     if (fabsf(sxv) > dzmax) {
         float sg = sxv / fabsf(sxv);
         bx = sg * (fabsf(sxv) - dzmax) * dx;
         if (sg > 0) {
             bx = fmn(bx, maxchg * dt / ddx);
-            bx = fmn(bx, sd[i]);
+            bx = fmn(bx, o.sd_i);
         } else {
             bx = fmx(bx, -1.0f * maxchg * dt / ddx);
-            bx = fmx(bx, -1.0f * sd[r]);
+            bx = fmx(bx, -1.0f * o.sd_r);
         }
     }
-    float syv = (zb[t] - zb[i]) / ddx;
+    float syv = (o.zb_t - o.zb_i) / ddx;
     if (fabsf(syv) > dzmax) {
         float sg = syv / fabsf(syv);
         by = sg * (fabsf(syv) - dzmax) * dx;
         if (sg > 0) {
             by = fmn(by, maxchg * dt / ddx);
-            by = fmn(by, sd[i]);
+            by = fmn(by, o.sd_i);
         } else {
             by = fmx(by, -1 * maxchg * dt / ddx);
-            by = fmx(by, -1 * sd[t]);
+            by = fmx(by, -1 * o.sd_t);
         }
     }
     bxo = bx;
@@ -1870,7 +1993,10 @@ XDEV void aval_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& b
 // takes the change of row ny-2, whose gather reads row ny-3).  The reference
 // scatters the exchange into both cells without atomics; the change of each
 // cell is gathered here as (dzbx + dzby)_i - dzbx_{i-1} - dzby_{i-nx}.
+// Every global operand of the pass (the exchange slots' and the cell's own)
+// is loaded before the first branch and store: one round trip to memory.
 constexpr int MW = BX + 1, MH = BY + 2;  // exchange tile: x0-1 .. x0+BX-1, y0-2 .. y0+BY-1
+constexpr int MSLOT = (MW * MH + BX * BY - 1) / (BX * BY);
 struct MorphLds {
     float x[MH][MW], y[MH][MW];
 };
@@ -1885,16 +2011,42 @@ XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
     const int x0 = bi.x * BX, y0 = bi.y * BY;
-    for (int l = tid; l < MW * MH; l += BX * BY) {
-        int r = l / MW, c = l - r * MW;
-        int hx = x0 - 1 + c, hy = y0 - 2 + r;
-        float bx = 0.0f, by = 0.0f;
-        if (hx >= 0 && hy >= 0 && hx < nx && hy < ny) aval_at<P2>(p, s, f, hx, hy, bx, by);
-        ex[r][c] = bx;
-        ey[r][c] = by;
+    AvOps o[MSLOT];
+#pragma unroll
+    for (int k = 0; k < MSLOT; k++) {
+        int l = min(tid + k * BX * BY, MW * MH - 1), r = l / MW, c = l - r * MW;
+        o[k] = av_load(p, f, min(max(x0 - 1 + c, 0), nx - 1), min(max(y0 - 2 + r, 0), ny - 1));
+    }
+    // the cell's own operands (clamped coordinates; used only inside the domain)
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    const int xc = min(x, nx - 1), yc = min(y, ny - 1);
+    size_t i = ix2(p, xc, yc);
+    int sx, sy;
+    zsrc(p, xc, yc, sx, sy);
+    size_t is = ix2(p, sx, sy);
+    const float dzb_i = G(f.dzb)[i], zs_i = G(f.zs)[i], hhb_i = G(f.hh_b)[i], sdb_s = G(f.stdep_b)[is];
+    double zb64_s = 0.0;
+    float zbb_s = 0.0f, zbcb_s = 0.0f;
+    if (p.zb_accum == 2) zb64_s = G(f.zb64_b)[is];
+    else zbb_s = G(f.zb_b)[is];
+    if (p.zb_accum == 1) zbcb_s = G(f.zbc_b)[is];
+#pragma unroll
+    for (int k = 0; k < MSLOT; k++) need(o[k]);
+    need(dzb_i, zs_i, hhb_i, sdb_s);
+    need(zbb_s, zbcb_s, (float)zb64_s, zbb_s);
+#pragma unroll
+    for (int k = 0; k < MSLOT; k++) {
+        int l = tid + k * BX * BY;
+        if (l < MW * MH) {
+            int r = l / MW, c = l - r * MW;
+            int hx = x0 - 1 + c, hy = y0 - 2 + r;
+            float bx = 0.0f, by = 0.0f;
+            if (hx >= 0 && hy >= 0 && hx < nx && hy < ny) aval_ops<P2>(p, s, o[k], bx, by);
+            ex[r][c] = bx;
+            ey[r][c] = by;
+        }
     }
     __syncthreads();
-    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     if (x >= nx || y >= ny) return;
     // bed change of cell (ux, uy) of the tile or its halo
     auto ddzb = [&](int ux, int uy) {
@@ -1904,28 +2056,24 @@ XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         if (uy > 0) d = d - ey[r - 1][c];
         return d;
     };
-    size_t i = ix2(p, x, y);
     float d = ddzb(x, y);
-    int sx, sy;
-    zsrc(p, x, y, sx, sy);
-    size_t is = ix2(p, sx, sy);
     float ds = (is == i) ? d : ddzb(sx, sy);
-    G(f.dzb)[i] = G(f.dzb)[i] - d;
-    G(f.zs)[i] = G(f.zs)[i] - d;
-    G(f.hh)[i] = G(f.hh_b)[i] - d;
-    G(f.stdep)[i] = G(f.stdep_b)[is] - ds;
+    G(f.dzb)[i] = dzb_i - d;
+    G(f.zs)[i] = zs_i - d;
+    G(f.hh)[i] = hhb_i - d;
+    G(f.stdep)[i] = sdb_s - ds;
     if (p.zb_accum == 2) {
-        double z = G(f.zb64_b)[is] + (double)ds;
+        double z = zb64_s + (double)ds;
         G(f.zb64)[i] = z;
         G(f.zb)[i] = (float)z;
     } else if (p.zb_accum == 1) {
-        float zb0 = G(f.zb_b)[is], c0 = G(f.zbc_b)[is];
+        float zb0 = zbb_s, c0 = zbcb_s;
         float yk = ds - c0;
         float tk = zb0 + yk;
         G(f.zbc)[i] = (tk - zb0) - yk;
         G(f.zb)[i] = tk;
     } else {
-        G(f.zb)[i] = G(f.zb_b)[is] + ds;
+        G(f.zb)[i] = zbb_s + ds;
     }
 }
 __global__ __launch_bounds__(256) void k_sed_morph(KP p, SP s, Fld f) {
@@ -3331,7 +3479,7 @@ __global__ __launch_bounds__(256) void kb_flow_nuh(DTab tab, TMap map, int ntile
     ENS_FOR_TILES(map, ntiles) {
         const TileEnt e = map[t_];
         const DomDesc& D = tab[e.dom];
-        if (D.status == ENS_ACTIVE) flow_nuh_body(D.p, D.f[D.par][PH_WAVE], 1, BI{e.bx, e.by});
+        if (D.status == ENS_ACTIVE) flow_nuh_body(D.p, D.f[D.par][PH_WAVE], BI{e.bx, e.by});
     }
 }
 
