@@ -1635,25 +1635,43 @@ __global__ __launch_bounds__(BX * CY) void k_flow_zsall(KP p, SP s, Fld f) {
     else flow_zsall_body<false>(p, s, f, BI{(int)blockIdx.x, (int)blockIdx.y});
 }
 
-// calcuvvu (Flow_kernel.cu:467-541) at (x, y)
-XDEV void calcuvvu_at(const KP& p, const Fld& f, int x, int y, float* o) {
+// Operands of calcuvvu at (x, y).
+struct CuOps {
+    float v_b, v_i, v_rb, v_r, u_l, u_i, u_lt, u_t, tm_i, tm_r, tm_t, us_i, us_r, us_t;
+    int w_i;
+};
+XDEV CuOps cu_load(const KP& p, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
     size_t i = ix2(p, x, y), r = ix2(p, xp, y), t = ix2(p, x, yp);
     const auto uu = G(f.uu);
     const auto vv = G(f.vv);
-    float wu = wetu_at(p, f, x, y), wv = wetv_at(p, f, x, y);
-    float vu = 0.25f * (vv[ix2(p, x, ym)] + vv[i] + vv[ix2(p, xp, ym)] + vv[r]) * wu;
-    float uv = 0.25f * (uu[ix2(p, xm, y)] + uu[i] + uu[ix2(p, xm, yp)] + uu[t]) * wv;
-    float tmi = G(f.thetamean)[i], tmr = G(f.thetamean)[r], tmt = G(f.thetamean)[t];
-    float si = xm::sinf(tmi), ci = xm::cosf(tmi), sr = xm::sinf(tmr), cr = xm::cosf(tmr), st = xm::sinf(tmt), ctt = xm::cosf(tmt);
-    float usi = G(f.ust)[i], usr = G(f.ust)[r], ust_t = G(f.ust)[t];
+    const auto tm = G(f.thetamean);
+    const auto us = G(f.ust);
+    return CuOps{vv[ix2(p, x, ym)], vv[i], vv[ix2(p, xp, ym)], vv[r], uu[ix2(p, xm, y)], uu[i], uu[ix2(p, xm, yp)],
+                 uu[t], tm[i], tm[r], tm[t], us[i], us[r], us[t], G(f.wet)[i]};
+}
+XDEV void need(const CuOps& o) {
+    need(o.v_b, o.v_i, o.v_rb, o.v_r);
+    need(o.u_l, o.u_i, o.u_lt, o.u_t);
+    need(o.tm_i, o.tm_r, o.tm_t, o.us_i);
+    need(o.us_r, o.us_t, o.us_i, o.us_i);
+    need(o.w_i, o.w_i, o.w_i, o.w_i);
+}
+// calcuvvu (Flow_kernel.cu:467-541) on loaded operands
+XDEV void calcuvvu_ops(const CuOps& q, float* o) {
+    float wu = (float)(q.w_i & 1), wv = (float)((q.w_i >> 1) & 1);
+    float vu = 0.25f * (q.v_b + q.v_i + q.v_rb + q.v_r) * wu;
+    float uv = 0.25f * (q.u_l + q.u_i + q.u_lt + q.u_t) * wv;
+    float si = xm::sinf(q.tm_i), ci = xm::cosf(q.tm_i), sr = xm::sinf(q.tm_r), cr = xm::cosf(q.tm_r);
+    float st = xm::sinf(q.tm_t), ctt = xm::cosf(q.tm_t);
+    float usi = q.us_i, usr = q.us_r, ust_t = q.us_t;
     float vsu = 0.5f * (usi * si + usr * sr) * wu;
     float usu = 0.5f * (usi * ci + usr * cr) * wu;
-    float ue = uu[i] - usu, ve = vu - vsu;
+    float ue = q.u_i - usu, ve = vu - vsu;
     float usv = 0.5f * (usi * ci + ust_t * ctt) * wv;
     float vsv = 0.5f * (usi * si + ust_t * st) * wv;
-    float vev = vv[i] - vsv, uev = uv - usv;
+    float vev = q.v_i - vsv, uev = uv - usv;
     o[0] = uv;
     o[1] = vu;
     o[2] = ue;
@@ -1662,29 +1680,43 @@ XDEV void calcuvvu_at(const KP& p, const Fld& f, int x, int y, float* o) {
     o[5] = sqrtf(uev * uev + vev * vev);
 }
 
+XDEV float rvr_ops(const KP& p, const SP& s, float c, float H, float h, float urms);
 XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i);
 
 // calcuvvu + uvlatbnd (Flow_kernel.cu:1712-1775): rows 0 / ny-1 copy rows
 // 1 / ny-2.  Also saves uuold/vvold/zsold.  With want_ua, Rvr's ua of the
 // cell for the sediment step that follows (its inputs c, H, hh, urms are
 // final here): once per cell instead of per cell and halo in k_sed_flux.
+// Every operand is loaded ahead of the stores.
 XDEV void flow_uvvu_body(const KP& p, const SP& s, const Fld& f, int want_ua, BI bi) {
     int x = gx(bi), y = gy(bi);
     if (x >= p.nx || y >= p.ny) return;
     size_t i = ix2(p, x, y);
     int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
+    const CuOps q = cu_load(p, f, x, ys);
+    const float u_i = G(f.uu)[i], v_i = G(f.vv)[i], zs_i = G(f.zs)[i];
+    float c_i = 0.0f, H_i = 0.0f, h_i = 0.0f, urms_i = 0.0f;
+    if (want_ua) {
+        c_i = G(f.c)[i];
+        H_i = G(f.H)[i];
+        h_i = G(f.hh)[i];
+        urms_i = G(f.urms)[i];
+    }
+    need(q);
+    need(u_i, v_i, zs_i, c_i);
+    need(H_i, h_i, urms_i, u_i);
     float o[6];
-    calcuvvu_at(p, f, x, ys, o);
-    G(f.uuold)[i] = G(f.uu)[i];
-    G(f.vvold)[i] = G(f.vv)[i];
-    G(f.zsold)[i] = G(f.zs)[i];
+    calcuvvu_ops(q, o);
+    G(f.uuold)[i] = u_i;
+    G(f.vvold)[i] = v_i;
+    G(f.zsold)[i] = zs_i;
     G(f.uv)[i] = o[0];
     G(f.vu)[i] = o[1];
     G(f.ueu)[i] = o[2];
     G(f.vev)[i] = o[3];
     G(f.vmageu)[i] = o[4];
     G(f.vmagev)[i] = o[5];
-    if (want_ua) G(f.ua)[i] = rvr_at(p, s, f, i);
+    if (want_ua) G(f.ua)[i] = rvr_ops(p, s, c_i, H_i, h_i, urms_i);
 }
 __global__ __launch_bounds__(256) void k_flow_uvvu(KP p, SP s, Fld f, int want_ua) {
     flow_uvvu_body(p, s, f, want_ua, BI{(int)blockIdx.x, (int)blockIdx.y});
@@ -1709,11 +1741,14 @@ XDEV float powf_pos(float x, float y, double lx) {
 
 // Rvr (Sediment_kernel.cu:477-510) at cell i
 XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
+    return rvr_ops(p, s, G(f.c)[i], G(f.H)[i], G(f.hh)[i], G(f.urms)[i]);
+}
+XDEV float rvr_ops(const KP& p, const SP& s, float c, float H, float h, float urms) {
     const float m1c = 0.0f, m2c = 0.7939f, m5c = 0.6373f, m6c = 0.5995f;
     // alph = -log10(e)/m4, bet = exp(m3/m4) with m3 = -0.6065, m4 = 0.3539
     const float alph = p.rvr_alph, bet = p.rvr_bet;
-    float kk = (float)(2 * PI_ / (double)(G(f.c)[i] * s.Trep));
-    float Ur = 3.0f / 8.0f * sqrtf(2.0f) * G(f.H)[i] * kk / xm::powf(kk * G(f.hh)[i], 3.0f);
+    float kk = (float)(2 * PI_ / (double)(c * s.Trep));
+    float Ur = 3.0f / 8.0f * sqrtf(2.0f) * H * kk / xm::powf(kk * h, 3.0f);
     Ur = fmx(Ur, 0.00001f);
     if (Ur == Ur) {  // Ur >= 1e-5 (or +inf): both powers from one log
         const double lu = xm::log((double)Ur);
@@ -1721,13 +1756,13 @@ XDEV float rvr_at(const KP& p, const SP& s, const Fld& f, size_t i) {
         float B1 = (float)((double)(-90.0f + 90.0f * xm::tanhf(m5c / powf_pos(Ur, m6c, lu))) * PI_ / 180.0);
         float Sk = Bm * xm::cosf(B1);
         float As = Bm * xm::sinf(B1);
-        return (p.facsk * Sk - p.facas * As) * G(f.urms)[i];
+        return (p.facsk * Sk - p.facas * As) * urms;
     }
     float Bm = m1c + (m2c - m1c) / (1.0f + bet * xm::powf(Ur, alph));
     float B1 = (float)((double)(-90.0f + 90.0f * xm::tanhf(m5c / xm::powf(Ur, m6c))) * PI_ / 180.0);
     float Sk = Bm * xm::cosf(B1);
     float As = Bm * xm::sinf(B1);
-    return (p.facsk * Sk - p.facas * As) * G(f.urms)[i];
+    return (p.facsk * Sk - p.facas * As) * urms;
 }
 
 // longturb (Sediment_kernel.cu:27-165: kturb is set to 0, rolthick is the
@@ -1762,8 +1797,8 @@ XDEV void sed_flux_body(const KP& p, const SP& s, const Fld& f, int ua_ready, BI
     int x = x0 + cx, y = y0 + cy;
     if (x >= nx || y >= ny) return;
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, nx), y), t = ix2(p, x, p1(y, ny));
-    float rol = G(f.rolthick)[i] + s.dt * (fabsf(G(f.dzsdt)[i]) - p.beta * G(f.c)[i]);
-    G(f.rolthick)[i] = fmx(rol, 0.0f);
+    // every operand loaded ahead of the stores
+    const float rol_i = G(f.rolthick)[i], dzsdt_i = G(f.dzsdt)[i], c_i = G(f.c)[i];
     float uai, uar, uat;
     if (ua_ready) {
         uai = G(f.ua)[i];
@@ -1774,20 +1809,30 @@ XDEV void sed_flux_body(const KP& p, const SP& s, const Fld& f, int ua_ready, BI
         uar = uat_[cy][cx + 1];
         uat = uat_[cy + 1][cx];
     }
-    float cci = G(f.C)[i], ccr = G(f.C)[r], cct = G(f.C)[t];
-    float tm = G(f.thetamean)[i];
+    const float cci = G(f.C)[i], ccr = G(f.C)[r], cct = G(f.C)[t];
+    const float tm = G(f.thetamean)[i], ueu_i = G(f.ueu)[i], uu_i = G(f.uu)[i], vu_i = G(f.vu)[i];
+    const float vev_i = G(f.vev)[i], uv_i = G(f.uv)[i], vv_i = G(f.vv)[i];
+    const float zb_r = G(f.zb)[r], zb_i = G(f.zb)[i], zb_t = G(f.zb)[t];
+    const float hu = G(f.hu)[i], hv = G(f.hv)[i], hh_i = G(f.hh)[i], DR_i = G(f.DR)[i];
+    need(rol_i, dzsdt_i, c_i, uai);
+    need(uar, uat, cci, ccr);
+    need(cct, tm, ueu_i, uu_i);
+    need(vu_i, vev_i, uv_i, vv_i);
+    need(zb_r, zb_i, zb_t, hu);
+    need(hv, hh_i, DR_i, hv);
+    float rol = rol_i + s.dt * (fabsf(dzsdt_i) - p.beta * c_i);
+    G(f.rolthick)[i] = fmx(rol, 0.0f);
     float ctm = xm::cosf(tm), stm = xm::sinf(tm);
     float uau = (float)(0.5 * (double)ctm * (double)(uai + uar));
     float uav = (float)(0.5 * (double)stm * (double)(uar + uai));
-    float urep = G(f.ueu)[i] + uau;
-    float vmagu = sqrtf(xm::powf((G(f.uu)[i] + uau), 2.0f) + xm::powf((G(f.vu)[i] + uav), 2.0f));
+    float urep = ueu_i + uau;
+    float vmagu = sqrtf(xm::powf((uu_i + uau), 2.0f) + xm::powf((vu_i + uav), 2.0f));
     uau = (float)(0.5 * (double)ctm * (double)(uai + uat));
     uav = (float)(0.5 * (double)stm * (double)(uat + uai));
-    float vrep = G(f.vev)[i] + uav;
-    float vmagv = sqrtf(xm::powf(G(f.uv)[i] + uau, 2.0f) + xm::powf(G(f.vv)[i] + uav, 2.0f));
-    float dzbdx = -1.0f * (G(f.zb)[r] - G(f.zb)[i]) / p.dx;
-    float dzbdy = -1.0f * (G(f.zb)[t] - G(f.zb)[i]) / p.dx;
-    float hu = G(f.hu)[i], hv = G(f.hv)[i];
+    float vrep = vev_i + uav;
+    float vmagv = sqrtf(xm::powf(uv_i + uau, 2.0f) + xm::powf(vv_i + uav, 2.0f));
+    float dzbdx = -1.0f * (zb_r - zb_i) / p.dx;
+    float dzbdy = -1.0f * (zb_t - zb_i) / p.dx;
     float wu = hu > p.eps ? 1.0f : 0.0f, wv = hv > p.eps ? 1.0f : 0.0f;
     float cu, cv;
     if (urep > 0.0f) cu = cci;
@@ -1798,7 +1843,7 @@ XDEV void sed_flux_body(const KP& p, const SP& s, const Fld& f, int ua_ready, BI
     else if (vrep < 0.0f) cv = cct;
     else cv = 0.50f * (cci + cct);
     float dcsdy = (cct - cci) / p.dx;
-    float Dc = p.nuh + p.nuhfac * G(f.hh)[i] * xm::powf(G(f.DR)[i] / p.rho, 1.0f / 3.0f);
+    float Dc = p.nuh + p.nuhfac * hh_i * xm::powf(DR_i / p.rho, 1.0f / 3.0f);
     const float facsl = 1.6f;
     G(f.Sus)[i] = p.sus * (cu * urep * hu - Dc * hu * dcsdx - facsl * cu * vmagu * hu * dzbdx) * wu;
     G(f.Svs)[i] = p.sus * (cv * vrep * hv - Dc * hv * dcsdy - facsl * cv * vmagv * hv * dzbdy) * wv;
@@ -1809,21 +1854,34 @@ __global__ __launch_bounds__(256) void k_sed_flux(KP p, SP s, Fld f, int ua_read
 
 // Sbvr (Sediment_kernel.cu:167-326) + Erosus (512-531) + Conc (721-779) at
 // (x, y): new concentration and bed change ero - depo.
-XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& Cnew, float& dzg, float& ceqs_out) {
-    const int nx = p.nx, ny = p.ny;
-    size_t i = ix2(p, x, y);
+// Operands of Sbvr + Erosus + Conc at (x, y).
+struct CcOps {
+    float h, ue_l, ue_i, ve_b, ve_i, R, c, DR, urms, sd, sus_i, sus_l, svs_i, svs_b, C;
+};
+XDEV CcOps cc_load(const KP& p, const Fld& f, int x, int y) {
+    const size_t i = ix2(p, x, y), l = ix2(p, m1(x), y), b = ix2(p, x, m1(y));
+    return CcOps{G(f.hh)[i],   G(f.ueu)[l], G(f.ueu)[i], G(f.vev)[b], G(f.vev)[i], G(f.R)[i],   G(f.c)[i], G(f.DR)[i],
+                 G(f.urms)[i], G(f.stdep)[i], G(f.Sus)[i], G(f.Sus)[l], G(f.Svs)[i], G(f.Svs)[b], G(f.C)[i]};
+}
+XDEV void need(const CcOps& o) {
+    need(o.h, o.ue_l, o.ue_i, o.ve_b);
+    need(o.ve_i, o.R, o.c, o.DR);
+    need(o.urms, o.sd, o.sus_i, o.sus_l);
+    need(o.svs_i, o.svs_b, o.C, o.h);
+}
+XDEV void conc_ops(const KP& p, const SP& s, const CcOps& o, float& Cnew, float& dzg, float& ceqs_out) {
     const float zo = 0.006f, sedcal = 1.0f, bulk = 1.0f;
-    float h = G(f.hh)[i];
+    float h = o.h;
     float hhi = fmx(h, 0.01f);
     const float drho = p.sed_drho;
-    float ue = (float)(0.5 * (double)(G(f.ueu)[ix2(p, m1(x), y)] + G(f.ueu)[i]));
-    float ve = (float)(0.5 * (double)(G(f.vev)[ix2(p, x, m1(y))] + G(f.vev)[i]));
+    float ue = (float)(0.5 * (double)(o.ue_l + o.ue_i));
+    float ve = (float)(0.5 * (double)(o.ve_b + o.ve_i));
     float vmags = ue * ue + ve * ve;
-    float ML = fmx(fmn(sqrtf(2.0f * G(f.R)[i] * s.Trep / (p.rho * G(f.c)[i])), hhi), 0.01f);
+    float ML = fmx(fmn(sqrtf(2.0f * o.R * s.Trep / (p.rho * o.c)), hhi), 0.01f);
     float dcfin = xm::expf(fmn(hhi, 100.0f) / ML);
     float dcf = fmn(1 / (dcfin - 1), 1.0f);
-    float kb = p.nuhfac * xm::powf(G(f.DR)[i] / p.rho, 0.66666666667f) * dcf;
-    float Urms2 = G(f.urms)[i] * G(f.urms)[i] + 1.45f * (kb + 0.0f);
+    float kb = p.nuhfac * xm::powf(o.DR / p.rho, 0.66666666667f) * dcf;
+    float Urms2 = o.urms * o.urms + 1.45f * (kb + 0.0f);
     float Tsg = fmx(0.1f * hhi / p.wws, 0.5f);
     float Ucr = p.sed_ucrfac * xm::log10f(4.0f * hhi / p.D90);
     float hdrag = fmx(hhi, 10.0f * zo);
@@ -1831,7 +1889,7 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     Cdd = Cdd * Cdd;
     float Asb = 0.005f * hhi * xm::powf(1 / hhi / (drho * p.g), 1.2f);
     const float Ass = p.sed_Ass;
-    float cfm = cfm_of(p, G(f.stdep)[i]);
+    float cfm = cfm_of(p, o.sd);
     float T1 = vmags + 0.018f / Cdd * Urms2;
     T1 = fmn(T1, 100000.0f * p.g / cfm * p.D50 * drho);
     T1 = sqrtf(T1);
@@ -1846,21 +1904,22 @@ XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& C
     ceqs_out = ceqsg;
     // Erosus
     float exp_ero = p.morfac * s.dt / (1.0f - p.por) * h * (ceqsg * 1.0f / Tsg + ceqbg * 1.0f / s.dt);
-    float facero = fmn(1.0f, G(f.stdep)[i] * 1.0f / fmx(0.000001f, exp_ero));
+    float facero = fmn(1.0f, o.sd * 1.0f / fmx(0.000001f, exp_ero));
     // Conc
     float wz = h > p.eps ? 1.0f : 0.0f;
     float ero = facero * h * ceqsg * 1.0f / Tsg;
-    float dsusdx = (G(f.Sus)[i] - G(f.Sus)[ix2(p, m1(x), y)]) / p.dx;
-    float dsvsdy = (G(f.Svs)[i] - G(f.Svs)[ix2(p, x, m1(y))]) / p.dx;
-    float cs = (s.dt * Tsg) / (s.dt + Tsg) * (h * G(f.C)[i] / s.dt - (dsusdx + dsvsdy - ero)) * wz;
+    float dsusdx = (o.sus_i - o.sus_l) / p.dx;
+    float dsvsdy = (o.svs_i - o.svs_b) / p.dx;
+    float cs = (s.dt * Tsg) / (s.dt + Tsg) * (h * o.C / s.dt - (dsusdx + dsvsdy - ero)) * wz;
     cs = fmx(cs, 0.0f);
     cs = fmn(cs, 0.1f * h);
     float depo = cs / Tsg;
     Cnew = cs / h;
     // bedupdate (Sediment_kernel.cu:881-964)
     dzg = p.morfac * s.dt / (1.0f - p.por) * (ero - depo);
-    (void)nx;
-    (void)ny;
+}
+XDEV void conc_at(const KP& p, const SP& s, const Fld& f, int x, int y, float& Cnew, float& dzg, float& ceqs_out) {
+    conc_ops(p, s, cc_load(p, f, x, y), Cnew, dzg, ceqs_out);
 }
 
 // zblatbnd source cell (Sediment_kernel.cu:970-1018): ix==0 wins.
@@ -1890,8 +1949,24 @@ XDEV void sed_conc_body(const KP& p, const SP& s, const Fld& f, int morph, BI bi
     if (x >= p.nx || y >= p.ny) return;
     const int ny = p.ny;
     size_t i = ix2(p, x, y);
+    // the cell's operands and those of its bed update, loaded ahead of the
+    // stores
+    const CcOps o = cc_load(p, f, x, y);
+    int sx, sy;
+    zsrc(p, x, y, sx, sy);
+    const size_t is = ix2(p, sx, sy);
+    float sd_s = 0.0f, zb_s = 0.0f, zbc_s = 0.0f;
+    double zb64_s = 0.0;
+    if (morph) {
+        sd_s = G(f.stdep)[is];
+        if (p.zb_accum == 2) zb64_s = G(f.zb64)[is];
+        else zb_s = G(f.zb)[is];
+        if (p.zb_accum == 1) zbc_s = G(f.zbc)[is];
+    }
+    need(o);
+    need(sd_s, zb_s, zbc_s, (float)zb64_s);
     float Cn, dzg, ceq;
-    conc_at(p, s, f, x, y, Cn, dzg, ceq);
+    conc_ops(p, s, o, Cn, dzg, ceq);
     G(f.ceqsg)[i] = ceq;
     float Cfin;
     int cy = (y == 0) ? p1(0, ny) : (y == ny - 1 ? m1(ny - 1) : y);
@@ -1905,29 +1980,26 @@ XDEV void sed_conc_body(const KP& p, const SP& s, const Fld& f, int morph, BI bi
     }
     G(f.C_out)[i] = Cfin;
     if (!morph) return;
-    int sx, sy;
-    zsrc(p, x, y, sx, sy);
-    size_t is = ix2(p, sx, sy);
     float dzs = dzg;
     if (is != i) {
         float c2, c3;
         conc_at(p, s, f, sx, sy, c2, dzs, c3);
     }
-    G(f.hh_b)[i] = G(f.hh)[i] + dzg;
+    G(f.hh_b)[i] = o.h + dzg;
     G(f.dzb)[i] = -1 * dzg;
-    G(f.stdep_b)[i] = G(f.stdep)[is] - dzs;
+    G(f.stdep_b)[i] = sd_s - dzs;
     if (p.zb_accum == 2) {
-        double z = G(f.zb64)[is] + (double)dzs;
+        double z = zb64_s + (double)dzs;
         G(f.zb64_b)[i] = z;
         G(f.zb_b)[i] = (float)z;
     } else if (p.zb_accum == 1) {
-        float zb0 = G(f.zb)[is], c0 = G(f.zbc)[is];
+        float zb0 = zb_s, c0 = zbc_s;
         float yk = dzs - c0;
         float tk = zb0 + yk;
         G(f.zbc_b)[i] = (tk - zb0) - yk;
         G(f.zb_b)[i] = tk;
     } else {
-        G(f.zb_b)[i] = G(f.zb)[is] + dzs;
+        G(f.zb_b)[i] = zb_s + dzs;
     }
 }
 __global__ __launch_bounds__(256) void k_sed_conc(KP p, SP s, Fld f, int morph) {
