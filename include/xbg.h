@@ -129,7 +129,9 @@ typedef struct xbg_ctx xbg_ctx;
  * allocates ~60 buffers at init and ~25 scratch buffers per step,
  * Wave_gpu.cu:2187-2280, Wavestep.cu:801-950).  Initial state is zero
  * except sigm; load it with xbg_set_field ("zb", "zs", "hh", "uu", "vv",
- * "stdep", "ee", "rr", ...), then call xbg_init_state.
+ * "stdep", "ee", "rr", ...), then call xbg_init_state.  Limits (XBG_E_ARG
+ * beyond them): nx, ny >= 4; 1 <= ntheta <= 64; nx*ny*ntheta < 2^31;
+ * nx*ny < 2^29 (each cell field, FP64 ones included, within 4 GiB).
  */
 int xbg_create(const xbg_params* p, xbg_ctx** out);
 int xbg_destroy(xbg_ctx* ctx);

@@ -119,3 +119,6 @@ def test_create_rejects_bad_sizes_without_gpu():
         Domain(default_params(nx=2, ny=8, dx=1.0))
     with pytest.raises(XbgError):
         Domain(default_params(nx=8, ny=8, dx=1.0, thetamin=-90, thetamax=90, dtheta=1.0))  # 180 bins
+    # 2^29 cells: a cell field of FP64 would pass 4 GiB (32-bit byte offsets)
+    with pytest.raises(XbgError, match="2\\^29"):
+        Domain(default_params(nx=32768, ny=16384, dx=1.0, thetamin=-10, thetamax=10, dtheta=20.0))

@@ -104,6 +104,21 @@ XDEV __attribute__((address_space(1))) T* G(T* q) {
     return (__attribute__((address_space(1))) T*)q;
 }
 
+// A field element through a 32-bit byte offset from the field's base: the
+// load takes the base from scalar registers and one 32-bit offset register
+// per lane (shared by every field read at the same index), where a 64-bit
+// index needs a 64-bit address per lane and load.  The library's size limit
+// keeps every field within 4 GiB.  Used where the operands of a pass are
+// loaded together and their 64-bit addresses would cost occupancy (the
+// level pass held 73 registers with them, 60 with these); the momentum and
+// viscosity passes measured faster with 64-bit addresses (29-31 against
+// 31-34 us) and keep them.
+template <class T>
+XDEV T ld32(const T* base, size_t idx) {
+    const uint32_t off = (uint32_t)idx * (uint32_t)sizeof(T);
+    return *(const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) char*)base + off);
+}
+
 // Block (tile) coordinates of a kernel body: blockIdx in a single-domain
 // launch, the tile-map entry in a batched ensemble launch.
 struct BI {
@@ -1048,12 +1063,14 @@ XDEV void flow_nuh_body(const KP& p, const Fld& f, BI bi) {
     float ust_ys = 0.0f;
     if (FORCE) {
         w = wf_load(p, f, x, ys);
-        ust_ys = G(f.ust_raw)[(size_t)x + (size_t)ys * nx];
+        ust_ys = ld32(f.ust_raw, (size_t)x + (size_t)ys * nx);
     }
-    const auto HH = G(f.hh), U = G(f.uu), V = G(f.vv);
-    const float h = HH[i], h_r = HH[r], h_t = HH[t], hu_i = G(f.hu)[i], hv_i = G(f.hv)[i], dr_i = G(f.DR)[i];
-    const float u_i = U[i], u_l = U[l], u_t = U[t], u_b = U[b], u_lt = U[lt], u_lb = U[lb];
-    const float v_i = V[i], v_b = V[b], v_r = V[r], v_l = V[l], v_rb = V[rb], v_lb = V[lb];
+    const float h = G(f.hh)[i], h_r = G(f.hh)[r], h_t = G(f.hh)[t], hu_i = G(f.hu)[i];
+    const float hv_i = G(f.hv)[i], dr_i = G(f.DR)[i];
+    const float u_i = G(f.uu)[i], u_l = G(f.uu)[l], u_t = G(f.uu)[t], u_b = G(f.uu)[b];
+    const float u_lt = G(f.uu)[lt], u_lb = G(f.uu)[lb];
+    const float v_i = G(f.vv)[i], v_b = G(f.vv)[b], v_r = G(f.vv)[r], v_l = G(f.vv)[l];
+    const float v_rb = G(f.vv)[rb], v_lb = G(f.vv)[lb];
     if (FORCE) need(w, ust_ys);
     need(h, h_r, h_t, hu_i);
     need(hv_i, dr_i, u_i, u_l);
@@ -1104,14 +1121,16 @@ XDEV float eulerustep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const size_t i = ix2(p, x, y), l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp),
                  br = ix2(p, xp, ym), tr = ix2(p, xp, yp), bt = ix2(p, x, ymp), brt = ix2(p, xp, ymp);
     // every operand, loaded ahead of the first branch (clamped in-domain cells)
-    const auto U = G(f.uu), V = G(f.vv), HU = G(f.hu), HV = G(f.hv), HH = G(f.hh), NU = G(f.nuh);
-    const float u_i = U[i], u_l = U[l], u_r = U[r], u_b = U[b], u_t = U[t];
-    const float hu_i = HU[i], hu_l = HU[l], hu_r = HU[r];
-    const float v_b = V[b], v_br = V[br], v_i = V[i], v_r = V[r], v_t = V[t];
-    const float hv_b = HV[b], hv_br = HV[br], hv_i = HV[i], hv_r = HV[r];
-    const float n_i = NU[i], n_r = NU[r], n_tr = NU[tr], n_t = NU[t], n_br = NU[br], n_b = NU[b];
-    const float h_i = HH[i], h_r = HH[r], h_t = HH[t], h_tr = HH[tr], h_b = HH[b], h_br = HH[br];
-    const float h_bt = HH[bt], h_brt = HH[brt];
+    const float u_i = G(f.uu)[i], u_l = G(f.uu)[l], u_r = G(f.uu)[r], u_b = G(f.uu)[b], u_t = G(f.uu)[t];
+    const float hu_i = G(f.hu)[i], hu_l = G(f.hu)[l], hu_r = G(f.hu)[r];
+    const float v_b = G(f.vv)[b], v_br = G(f.vv)[br], v_i = G(f.vv)[i], v_r = G(f.vv)[r];
+    const float v_t = G(f.vv)[t];
+    const float hv_b = G(f.hv)[b], hv_br = G(f.hv)[br], hv_i = G(f.hv)[i], hv_r = G(f.hv)[r];
+    const float n_i = G(f.nuh)[i], n_r = G(f.nuh)[r], n_tr = G(f.nuh)[tr], n_t = G(f.nuh)[t];
+    const float n_br = G(f.nuh)[br], n_b = G(f.nuh)[b];
+    const float h_i = G(f.hh)[i], h_r = G(f.hh)[r], h_t = G(f.hh)[t], h_tr = G(f.hh)[tr];
+    const float h_b = G(f.hh)[b], h_br = G(f.hh)[br];
+    const float h_bt = G(f.hh)[bt], h_brt = G(f.hh)[brt];
     const float zs_r = G(f.zs)[r], zs_i = G(f.zs)[i], sd_i = G(f.stdep)[i], ue_i = G(f.ueu)[i];
     const float urms_i = G(f.urms)[i], vmag_i = G(f.vmageu)[i], fx_i = G(f.Fx)[i], vu_i = G(f.vu)[i];
     const int w_i = G(f.wet)[i], w_t = G(f.wet)[t], w_b = G(f.wet)[b];
@@ -1215,16 +1234,19 @@ XDEV float eulervstep_at(const KP& p, const SP& s, const Fld& f, int x, int y) {
     const size_t i = ix2(p, x, y), l = ix2(p, xm, y), r = ix2(p, xp, y), b = ix2(p, x, ym), t = ix2(p, x, yp),
                  tl = ix2(p, xm, yp), tr = ix2(p, xp, yp);
     // every operand, loaded ahead of the first branch (clamped in-domain cells)
-    const auto UN = G(f.uu_b), V = G(f.vv), HU = G(f.hu), HV = G(f.hv), HH = G(f.hh), NU = G(f.nuh);
-    const float un_l = UN[l], un_tl = UN[tl], un_i = UN[i], un_t = UN[t], un_r = UN[r];
-    const float hu_l = HU[l], hu_tl = HU[tl], hu_i = HU[i], hu_t = HU[t];
-    const float v_i = V[i], v_l = V[l], v_r = V[r], v_b = V[b], v_t = V[t];
-    const float hv_i = HV[i], hv_b = HV[b], hv_t = HV[t];
-    const float h_i = HH[i], h_t = HH[t], h_r = HH[r], h_tr = HH[tr], h_l = HH[l], h_tl = HH[tl];
-    const float n_i = NU[i], n_r = NU[r], n_tr = NU[tr], n_t = NU[t], n_l = NU[l], n_tl = NU[tl];
+    const float un_l = G(f.uu_b)[l], un_tl = G(f.uu_b)[tl], un_i = G(f.uu_b)[i], un_t = G(f.uu_b)[t];
+    const float un_r = G(f.uu_b)[r];
+    const float hu_l = G(f.hu)[l], hu_tl = G(f.hu)[tl], hu_i = G(f.hu)[i], hu_t = G(f.hu)[t];
+    const float v_i = G(f.vv)[i], v_l = G(f.vv)[l], v_r = G(f.vv)[r], v_b = G(f.vv)[b], v_t = G(f.vv)[t];
+    const float hv_i = G(f.hv)[i], hv_b = G(f.hv)[b], hv_t = G(f.hv)[t];
+    const float h_i = G(f.hh)[i], h_t = G(f.hh)[t], h_r = G(f.hh)[r], h_tr = G(f.hh)[tr];
+    const float h_l = G(f.hh)[l], h_tl = G(f.hh)[tl];
+    const float n_i = G(f.nuh)[i], n_r = G(f.nuh)[r], n_tr = G(f.nuh)[tr], n_t = G(f.nuh)[t];
+    const float n_l = G(f.nuh)[l], n_tl = G(f.nuh)[tl];
     const float zs_t = G(f.zs)[t], zs_i = G(f.zs)[i], sd_i = G(f.stdep)[i], ve_i = G(f.vev)[i];
     const float urms_i = G(f.urms)[i], vmag_i = G(f.vmagev)[i], fy_i = G(f.Fy)[i], uv_i = G(f.uv)[i];
-    const int w_i = G(f.wet)[i], w_t = G(f.wet)[t], w_b = G(f.wet)[b], w_r = G(f.wet)[r], w_l = G(f.wet)[l];
+    const int w_i = G(f.wet)[i], w_t = G(f.wet)[t], w_b = G(f.wet)[b], w_r = G(f.wet)[r];
+    const int w_l = G(f.wet)[l];
     need(un_l, un_tl, un_i, un_t);
     need(un_r, hu_l, hu_tl, hu_i);
     need(hu_t, v_i, v_l, v_r);
@@ -1474,7 +1496,8 @@ struct CtOps {
 };
 XDEV CtOps ct_load(const KP& p, const Fld& f, int x, int y) {
     size_t i = ix2(p, x, y), l = ix2(p, m1(x), y), b = ix2(p, x, m1(y));
-    return CtOps{G(f.zs)[i], G(f.uu)[i], G(f.hu)[i], G(f.uu)[l], G(f.hu)[l], G(f.vv)[i], G(f.hv)[i], G(f.vv)[b], G(f.hv)[b]};
+    return CtOps{ld32(f.zs, i), ld32(f.uu, i), ld32(f.hu, i), ld32(f.uu, l), ld32(f.hu, l),
+                 ld32(f.vv, i), ld32(f.hv, i), ld32(f.vv, b), ld32(f.hv, b)};
 }
 XDEV void need(const CtOps& o) {
     need(o.zs_i, o.u_i, o.hu_i, o.u_l);
@@ -1502,13 +1525,12 @@ struct WzOps {
 XDEV WzOps wz_load(const KP& p, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     const int xm = m1(x), xp = p1(x, nx), xp2 = p2(x, nx), ym = m1(y), yp = p1(y, ny), yp2 = p2(y, ny);
-    const size_t i = ix2(p, x, y);
-    const auto zb = G(f.zb);
-    const auto zo = G(f.zsold);
-    return WzOps{G(f.uu)[i],         G(f.vv)[i],         zb[ix2(p, xm, y)],  zb[i],
-                 zb[ix2(p, xp, y)],  zb[ix2(p, xp2, y)], zb[ix2(p, x, ym)],  zb[ix2(p, x, yp)],
-                 zb[ix2(p, x, yp2)], zo[i],              zo[ix2(p, xm, y)],  zo[ix2(p, xp, y)],
-                 zo[ix2(p, xp2, y)], zo[ix2(p, x, ym)],  zo[ix2(p, x, yp)],  zo[ix2(p, x, yp2)]};
+    const size_t i = ix2(p, x, y), l = ix2(p, xm, y), r = ix2(p, xp, y), r2 = ix2(p, xp2, y), b = ix2(p, x, ym),
+                 t = ix2(p, x, yp), t2 = ix2(p, x, yp2);
+    return WzOps{ld32(f.uu, i),    ld32(f.vv, i),    ld32(f.zb, l),    ld32(f.zb, i),
+                 ld32(f.zb, r),    ld32(f.zb, r2),   ld32(f.zb, b),    ld32(f.zb, t),
+                 ld32(f.zb, t2),   ld32(f.zsold, i), ld32(f.zsold, l), ld32(f.zsold, r),
+                 ld32(f.zsold, r2), ld32(f.zsold, b), ld32(f.zsold, t), ld32(f.zsold, t2)};
 }
 XDEV void need(const WzOps& o) {
     need(o.u, o.v, o.zb_xm, o.zb_i);
@@ -1557,7 +1579,8 @@ XDEV void flow_zsall_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     }
     const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
     const size_t ic = ix2(p, min(x, nx - 1), min(y, ny - 1));
-    const float u_c = G(f.uu)[ic], hu_c = G(f.hu)[ic], v_c = G(f.vv)[ic], hv_c = G(f.hv)[ic], zb_c = G(f.zb)[ic];
+    const float u_c = ld32(f.uu, ic), hu_c = ld32(f.hu, ic), v_c = ld32(f.vv, ic), hv_c = ld32(f.hv, ic);
+    const float zb_c = ld32(f.zb, ic);
 #pragma unroll
     for (int k = 0; k < ZSLOT; k++) need(co[k]);
     need(u_c, hu_c, v_c, hv_c);
@@ -1643,13 +1666,12 @@ struct CuOps {
 XDEV CuOps cu_load(const KP& p, const Fld& f, int x, int y) {
     const int nx = p.nx, ny = p.ny;
     int xm = m1(x), xp = p1(x, nx), ym = m1(y), yp = p1(y, ny);
-    size_t i = ix2(p, x, y), r = ix2(p, xp, y), t = ix2(p, x, yp);
-    const auto uu = G(f.uu);
-    const auto vv = G(f.vv);
-    const auto tm = G(f.thetamean);
-    const auto us = G(f.ust);
-    return CuOps{vv[ix2(p, x, ym)], vv[i], vv[ix2(p, xp, ym)], vv[r], uu[ix2(p, xm, y)], uu[i], uu[ix2(p, xm, yp)],
-                 uu[t], tm[i], tm[r], tm[t], us[i], us[r], us[t], G(f.wet)[i]};
+    size_t i = ix2(p, x, y), r = ix2(p, xp, y), t = ix2(p, x, yp), b = ix2(p, x, ym), rb = ix2(p, xp, ym),
+           l = ix2(p, xm, y), lt = ix2(p, xm, yp);
+    return CuOps{ld32(f.vv, b),         ld32(f.vv, i),         ld32(f.vv, rb),        ld32(f.vv, r),
+                 ld32(f.uu, l),         ld32(f.uu, i),         ld32(f.uu, lt),        ld32(f.uu, t),
+                 ld32(f.thetamean, i),  ld32(f.thetamean, r),  ld32(f.thetamean, t), ld32(f.ust, i),
+                 ld32(f.ust, r),        ld32(f.ust, t),        ld32(f.wet, i)};
 }
 XDEV void need(const CuOps& o) {
     need(o.v_b, o.v_i, o.v_rb, o.v_r);
@@ -1694,13 +1716,13 @@ XDEV void flow_uvvu_body(const KP& p, const SP& s, const Fld& f, int want_ua, BI
     size_t i = ix2(p, x, y);
     int ys = (y == 0) ? p1(0, p.ny) : (y == p.ny - 1 ? m1(p.ny - 1) : y);
     const CuOps q = cu_load(p, f, x, ys);
-    const float u_i = G(f.uu)[i], v_i = G(f.vv)[i], zs_i = G(f.zs)[i];
+    const float u_i = ld32(f.uu, i), v_i = ld32(f.vv, i), zs_i = ld32(f.zs, i);
     float c_i = 0.0f, H_i = 0.0f, h_i = 0.0f, urms_i = 0.0f;
     if (want_ua) {
-        c_i = G(f.c)[i];
-        H_i = G(f.H)[i];
-        h_i = G(f.hh)[i];
-        urms_i = G(f.urms)[i];
+        c_i = ld32(f.c, i);
+        H_i = ld32(f.H, i);
+        h_i = ld32(f.hh, i);
+        urms_i = ld32(f.urms, i);
     }
     need(q);
     need(u_i, v_i, zs_i, c_i);
@@ -1798,22 +1820,22 @@ XDEV void sed_flux_body(const KP& p, const SP& s, const Fld& f, int ua_ready, BI
     if (x >= nx || y >= ny) return;
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, nx), y), t = ix2(p, x, p1(y, ny));
     // every operand loaded ahead of the stores
-    const float rol_i = G(f.rolthick)[i], dzsdt_i = G(f.dzsdt)[i], c_i = G(f.c)[i];
+    const float rol_i = ld32(f.rolthick, i), dzsdt_i = ld32(f.dzsdt, i), c_i = ld32(f.c, i);
     float uai, uar, uat;
     if (ua_ready) {
-        uai = G(f.ua)[i];
-        uar = G(f.ua)[r];
-        uat = G(f.ua)[t];
+        uai = ld32(f.ua, i);
+        uar = ld32(f.ua, r);
+        uat = ld32(f.ua, t);
     } else {
         uai = uat_[cy][cx];
         uar = uat_[cy][cx + 1];
         uat = uat_[cy + 1][cx];
     }
-    const float cci = G(f.C)[i], ccr = G(f.C)[r], cct = G(f.C)[t];
-    const float tm = G(f.thetamean)[i], ueu_i = G(f.ueu)[i], uu_i = G(f.uu)[i], vu_i = G(f.vu)[i];
-    const float vev_i = G(f.vev)[i], uv_i = G(f.uv)[i], vv_i = G(f.vv)[i];
-    const float zb_r = G(f.zb)[r], zb_i = G(f.zb)[i], zb_t = G(f.zb)[t];
-    const float hu = G(f.hu)[i], hv = G(f.hv)[i], hh_i = G(f.hh)[i], DR_i = G(f.DR)[i];
+    const float cci = ld32(f.C, i), ccr = ld32(f.C, r), cct = ld32(f.C, t);
+    const float tm = ld32(f.thetamean, i), ueu_i = ld32(f.ueu, i), uu_i = ld32(f.uu, i), vu_i = ld32(f.vu, i);
+    const float vev_i = ld32(f.vev, i), uv_i = ld32(f.uv, i), vv_i = ld32(f.vv, i);
+    const float zb_r = ld32(f.zb, r), zb_i = ld32(f.zb, i), zb_t = ld32(f.zb, t);
+    const float hu = ld32(f.hu, i), hv = ld32(f.hv, i), hh_i = ld32(f.hh, i), DR_i = ld32(f.DR, i);
     need(rol_i, dzsdt_i, c_i, uai);
     need(uar, uat, cci, ccr);
     need(cct, tm, ueu_i, uu_i);
@@ -1860,8 +1882,9 @@ struct CcOps {
 };
 XDEV CcOps cc_load(const KP& p, const Fld& f, int x, int y) {
     const size_t i = ix2(p, x, y), l = ix2(p, m1(x), y), b = ix2(p, x, m1(y));
-    return CcOps{G(f.hh)[i],   G(f.ueu)[l], G(f.ueu)[i], G(f.vev)[b], G(f.vev)[i], G(f.R)[i],   G(f.c)[i], G(f.DR)[i],
-                 G(f.urms)[i], G(f.stdep)[i], G(f.Sus)[i], G(f.Sus)[l], G(f.Svs)[i], G(f.Svs)[b], G(f.C)[i]};
+    return CcOps{ld32(f.hh, i),    ld32(f.ueu, l),   ld32(f.ueu, i), ld32(f.vev, b), ld32(f.vev, i),
+                 ld32(f.R, i),     ld32(f.c, i),     ld32(f.DR, i),  ld32(f.urms, i), ld32(f.stdep, i),
+                 ld32(f.Sus, i),   ld32(f.Sus, l),   ld32(f.Svs, i), ld32(f.Svs, b), ld32(f.C, i)};
 }
 XDEV void need(const CcOps& o) {
     need(o.h, o.ue_l, o.ue_i, o.ve_b);
@@ -1958,10 +1981,10 @@ XDEV void sed_conc_body(const KP& p, const SP& s, const Fld& f, int morph, BI bi
     float sd_s = 0.0f, zb_s = 0.0f, zbc_s = 0.0f;
     double zb64_s = 0.0;
     if (morph) {
-        sd_s = G(f.stdep)[is];
-        if (p.zb_accum == 2) zb64_s = G(f.zb64)[is];
-        else zb_s = G(f.zb)[is];
-        if (p.zb_accum == 1) zbc_s = G(f.zbc)[is];
+        sd_s = ld32(f.stdep, is);
+        if (p.zb_accum == 2) zb64_s = ld32(f.zb64, is);
+        else zb_s = ld32(f.zb, is);
+        if (p.zb_accum == 1) zbc_s = ld32(f.zbc, is);
     }
     need(o);
     need(sd_s, zb_s, zbc_s, (float)zb64_s);
@@ -2013,9 +2036,8 @@ struct AvOps {
 };
 XDEV AvOps av_load(const KP& p, const Fld& f, int x, int y) {
     size_t i = ix2(p, x, y), r = ix2(p, p1(x, p.nx), y), t = ix2(p, x, p1(y, p.ny));
-    const auto zb = G(f.zb_b);
-    const auto sd = G(f.stdep_b);
-    return AvOps{G(f.hh_b)[i], zb[i], zb[r], zb[t], sd[i], sd[r], sd[t]};
+    return AvOps{ld32(f.hh_b, i),    ld32(f.zb_b, i),    ld32(f.zb_b, r),   ld32(f.zb_b, t),
+                 ld32(f.stdep_b, i), ld32(f.stdep_b, r), ld32(f.stdep_b, t)};
 }
 XDEV void need(const AvOps& o) {
     need(o.h_i, o.zb_i, o.zb_r, o.zb_t);
@@ -2096,12 +2118,12 @@ XDEV void sed_morph_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     int sx, sy;
     zsrc(p, xc, yc, sx, sy);
     size_t is = ix2(p, sx, sy);
-    const float dzb_i = G(f.dzb)[i], zs_i = G(f.zs)[i], hhb_i = G(f.hh_b)[i], sdb_s = G(f.stdep_b)[is];
+    const float dzb_i = ld32(f.dzb, i), zs_i = ld32(f.zs, i), hhb_i = ld32(f.hh_b, i), sdb_s = ld32(f.stdep_b, is);
     double zb64_s = 0.0;
     float zbb_s = 0.0f, zbcb_s = 0.0f;
-    if (p.zb_accum == 2) zb64_s = G(f.zb64_b)[is];
-    else zbb_s = G(f.zb_b)[is];
-    if (p.zb_accum == 1) zbcb_s = G(f.zbc_b)[is];
+    if (p.zb_accum == 2) zb64_s = ld32(f.zb64_b, is);
+    else zbb_s = ld32(f.zb_b, is);
+    if (p.zb_accum == 1) zbcb_s = ld32(f.zbc_b, is);
 #pragma unroll
     for (int k = 0; k < MSLOT; k++) need(o[k]);
     need(dzb_i, zs_i, hhb_i, sdb_s);
@@ -2658,6 +2680,10 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
         return fail(nullptr, XBG_E_ARG, "xbg_create: ntheta must be in [1, 64]");
     if ((double)P.nx * P.ny * P.ntheta >= 2147483647.0)
         return fail(nullptr, XBG_E_ARG, "xbg_create: nx*ny*ntheta must be < 2^31");
+    // the flow and sediment kernels address a cell field through a 32-bit
+    // byte offset (ld32): every 2D field, FP64 ones included, within 4 GiB
+    if ((double)P.nx * P.ny >= 536870912.0)
+        return fail(nullptr, XBG_E_ARG, "xbg_create: nx*ny must be < 2^29");
     if (P.zb_accum < 0 || P.zb_accum > 2) return fail(nullptr, XBG_E_ARG, "xbg_create: zb_accum must be 0, 1 or 2");
     xbg_ctx* c = new xbg_ctx();
     c->P = P;
