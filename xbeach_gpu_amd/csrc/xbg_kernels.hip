@@ -1366,13 +1366,40 @@ XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
     const int nx = p.nx, ny = p.ny;
     const int tid = threadIdx.x + threadIdx.y * BX;
     const int x0 = bi.x * BX, y0 = bi.y * CY;
-    const float* src[T_N] = {f.zs, f.zb, f.qx, f.qy, f.uu_b, f.uuold, f.vv_b, f.vvold};
-    for (int l = tid; l < DW * DH; l += BX * CY) {
-        int r = l / DW, c = l - r * DW;
-        int gxx = min(max(x0 - 3 + c, 0), nx - 1), gyy = min(max(y0 - 3 + r, 0), ny - 1);
-        size_t g = ix2(p, gxx, gyy);
+    // the tile's slots of the eight fields and the cell's own depths (used
+    // after the flux phase), all loaded before the first wait
+    constexpr int DSLOT = (DW * DH + BX * CY - 1) / (BX * CY);
+    float stg[DSLOT][T_N];
 #pragma unroll
-        for (int k = 0; k < T_N; k++) dt_[k][r][c] = src[k][g];
+    for (int k = 0; k < DSLOT; k++) {
+        int l = min(tid + k * BX * CY, DW * DH - 1), r = l / DW, c = l - r * DW;
+        size_t g = ix2(p, min(max(x0 - 3 + c, 0), nx - 1), min(max(y0 - 3 + r, 0), ny - 1));
+        stg[k][T_ZS] = ld32(f.zs, g);
+        stg[k][T_ZB] = ld32(f.zb, g);
+        stg[k][T_QX] = ld32(f.qx, g);
+        stg[k][T_QY] = ld32(f.qy, g);
+        stg[k][T_U] = ld32(f.uu_b, g);
+        stg[k][T_UO] = ld32(f.uuold, g);
+        stg[k][T_V] = ld32(f.vv_b, g);
+        stg[k][T_VO] = ld32(f.vvold, g);
+    }
+    const int xq = min(x0 + (int)threadIdx.x, nx - 1), yq = min(y0 + (int)threadIdx.y, ny - 1);
+    const float h_c = ld32(f.hh, ix2(p, xq, yq)), h_cr = ld32(f.hh, ix2(p, p1(xq, nx), yq));
+    const float h_ct = ld32(f.hh, ix2(p, xq, p1(yq, ny)));
+#pragma unroll
+    for (int k = 0; k < DSLOT; k++) {
+        need(stg[k][0], stg[k][1], stg[k][2], stg[k][3]);
+        need(stg[k][4], stg[k][5], stg[k][6], stg[k][7]);
+    }
+    need(h_c, h_cr, h_ct, h_c);
+#pragma unroll
+    for (int k = 0; k < DSLOT; k++) {
+        int l = tid + k * BX * CY;
+        if (l < DW * DH) {
+            int r = l / DW, c = l - r * DW;
+#pragma unroll
+            for (int q = 0; q < T_N; q++) dt_[q][r][c] = stg[k][q];
+        }
     }
     __syncthreads();
     // field k at unclamped (ux, uy)
@@ -1456,13 +1483,13 @@ XDEV void flow_uvcorr_body(const KP& p, const SP& s, const Fld& f, BI bi) {
         // the neighbours of the reference's uu2Ocorr are clamped: at the
         // last column xplus is the cell itself
         int xp = p1(x, nx), ym = m1(y), xm = m1(x), yp = p1(y, ny);
-        u = u - dts / hum_at(p, f, x, y) *
+        u = u - dts / fmx(0.5f * (h_c + h_cr), p.eps) *  // udepthmomcont at (x, y)
                     ((qe * K(0, xp, y) - qw * K(0, x, y)) / dx + (qs * K(1, x, y) - qn * K(1, x, ym)) / dx);
         qe = 0.5f * (D(T_QX, x, y + 1) + D(T_QX, x, y));
         qw = 0.5f * (D(T_QX, x - 1, y) + D(T_QX, x - 1, y + 1));
         qn = 0.5f * (D(T_QY, x, y) + D(T_QY, x, y - 1));
         qs = 0.5f * (D(T_QY, x, y + 1) + D(T_QY, x, y));
-        v = v - dts / hvm_at(p, f, x, y) *
+        v = v - dts / fmx(0.5f * (h_c + h_ct), p.eps) *  // vdepthmomcont at (x, y)
                     ((qe * K(3, x, y) - qw * K(3, xm, y)) / dx + (qs * K(2, x, yp) - qn * K(2, x, y)) / dx);
     }
     G(f.uu)[i] = u;
