@@ -2428,8 +2428,10 @@ void prof_begin(xbg_ctx* c, const char* name, hipStream_t st) {
     if (c->evused == c->evpool.size()) {
         xbg_ctx::Ev e;
         e.name = name;
-        hipEventCreate(&e.a);
-        hipEventCreate(&e.b);
+        if (hipEventCreate(&e.a) != hipSuccess || hipEventCreate(&e.b) != hipSuccess) {
+            c->prof = false;  // timing is optional: stop it rather than record on invalid events
+            return;
+        }
         c->evpool.push_back(e);
     }
     c->evpool[c->evused].name = name;
@@ -2782,8 +2784,14 @@ int xbg_create(const xbg_params* pin, xbg_ctx** out) {
     }
     sync_ptrs(c);
     register_fields(c);
-    // sigm-free initial wave dt partials
-    hipDeviceSynchronize();
+    // the zero fills of the allocations are done (and any error of theirs
+    // is reported) before the context is handed out
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipDeviceSynchronize: ") + hipGetErrorString(e);
+        xbg_destroy(c);
+        return XBG_E_HIP;
+    }
     *out = c;
     return XBG_OK;
 }
