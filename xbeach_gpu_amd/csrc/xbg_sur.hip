@@ -746,9 +746,8 @@ int xbg_sur_create(int nmax, int H, int W, unsigned seed, xbg_sur** out) {
         return bail(sfail(nullptr, XBG_E_HIP, "xbg_sur_create: stream"));
     auto A = [&](void** p, size_t bytes) -> bool {
         if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) return false;
-        hipMemset(*p, 0, bytes);
-        s->allocs.push_back(*p);
-        return true;
+        s->allocs.push_back(*p);  // freed by xbg_sur_destroy on any failure below
+        return hipMemset(*p, 0, bytes) == hipSuccess;
     };
     const size_t P0 = (size_t)nmax * H * W, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
     bool ok = A((void**)&s->P, off * 4) && A((void**)&s->G, off * 4) && A((void**)&s->M, off * 4) &&
